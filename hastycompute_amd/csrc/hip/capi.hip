@@ -1,0 +1,412 @@
+// extern "C" entry points of libhasty_hip.so (declared in include/hasty_hip.h) and the
+// host-side drivers behind them (LLR block grid, workspace layout).
+#include <algorithm>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../../include/hasty_hip.h"
+#include "common.hpp"
+#include "nufft.hpp"
+#include "toeplitz.hpp"
+#include "cg.hpp"
+#include "llr.hpp"
+
+namespace hasty_hip {
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename Fn> int guarded(Fn&& fn)
+{
+    try {
+        fn();
+        return kOk;
+    } catch (const HastyError& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host out of memory");
+        return kOutOfMemory;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return kHipError;
+    } catch (...) {
+        set_last_error("unknown error");
+        return kHipError;
+    }
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Enumerate the strided block grid of one pass (svt.cpp:393-436), skip rule included.
+std::vector<int32_t> grid_blocks(const int64_t* dims5, const int64_t* strides, const int64_t* shape, int sx_, int sy_,
+                                 int sz_)
+{
+    const int64_t Sx = dims5[2], Sy = dims5[3], Sz = dims5[4];
+    const int64_t bx = Sx / strides[0], by = Sy / strides[1], bz = Sz / strides[2];
+    std::vector<int32_t> out;
+    for (int64_t nx = 0; nx < bx; ++nx) {
+        const int64_t sx = nx * strides[0] + sx_;
+        if (sx + shape[0] >= Sx) continue;
+        for (int64_t ny = 0; ny < by; ++ny) {
+            const int64_t sy = ny * strides[1] + sy_;
+            if (sy + shape[1] >= Sy) continue;
+            for (int64_t nz = 0; nz < bz; ++nz) {
+                const int64_t sz = nz * strides[2] + sz_;
+                if (sz + shape[2] >= Sz) continue;
+                out.push_back((int32_t)sx);
+                out.push_back((int32_t)sy);
+                out.push_back((int32_t)sz);
+            }
+        }
+    }
+    return out;
+}
+
+int choose_nsplit(int64_t nblk, int64_t R)
+{
+    int64_t ns = ceil_div(512, std::max<int64_t>(1, nblk));
+    ns = std::min<int64_t>(ns, 8);
+    ns = std::min<int64_t>(ns, std::max<int64_t>(1, R / 256));
+    return (int)std::max<int64_t>(1, ns);
+}
+
+size_t llr_ws_bytes_for(int64_t nblk, int64_t F, int64_t R)
+{
+    const int ns = choose_nsplit(nblk, R);
+    return 256 + sizeof(int32_t) * 3 * (size_t)nblk + llr_workspace_bytes(nblk, F, ns);
+}
+
+// Largest workspace any batch of 1..nmax blocks needs (the split count depends on it).
+size_t llr_ws_bound(int64_t nmax, int64_t F, int64_t R)
+{
+    size_t m = 0;
+    for (int64_t n = 1; n <= nmax; ++n) m = std::max(m, llr_ws_bytes_for(n, F, R));
+    return m;
+}
+
+// Run one batch of disjoint blocks. The corner list is copied into the head of ws.
+void run_llr_batch(float2* data, const LlrGeom& g, const std::vector<int32_t>& corners, float thresh, int soft,
+                   void* ws, size_t ws_bytes, hipStream_t s)
+{
+    const int64_t nblk = (int64_t)corners.size() / 3;
+    if (nblk == 0) return;
+    const int64_t R = g.E * g.bx * g.by * g.bz;
+    const int nsplit = choose_nsplit(nblk, R);
+    const size_t cbytes = ((sizeof(int32_t) * 3 * (size_t)nblk + 255) / 256) * 256;
+    HASTY_REQUIRE(ws_bytes >= cbytes + llr_workspace_bytes(nblk, g.F, nsplit), "LLR workspace too small");
+    int3* cdev = reinterpret_cast<int3*>(ws);
+    HASTY_HIP_CHECK(hipMemcpyAsync(cdev, corners.data(), sizeof(int32_t) * corners.size(), hipMemcpyHostToDevice, s));
+    const int rank = soft ? 0 : (int)(thresh + 0.5);
+    llr_blocks(data, g, cdev, nblk, thresh, soft, rank, static_cast<char*>(ws) + cbytes, ws_bytes - cbytes, nsplit,
+               nullptr, s);
+    // The host corner list must outlive the async copy.
+    HASTY_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+LlrGeom image_geom(const int64_t* dims5, const int64_t* shape)
+{
+    LlrGeom g;
+    g.F = dims5[0];
+    g.E = dims5[1];
+    g.X = dims5[2];
+    g.Y = dims5[3];
+    g.Z = dims5[4];
+    g.bx = (int)shape[0];
+    g.by = (int)shape[1];
+    g.bz = (int)shape[2];
+    g.fstride = g.E * g.X * g.Y * g.Z;
+    g.rstride = 1;
+    return g;
+}
+
+void check_dims5(const int64_t* dims5, const int64_t* shape)
+{
+    for (int i = 0; i < 5; ++i) HASTY_REQUIRE(dims5[i] >= 1, "image dims must be >= 1");
+    for (int i = 0; i < 3; ++i) {
+        HASTY_REQUIRE(shape[i] >= 1 && shape[i] <= dims5[2 + i], "block shape must fit the image");
+    }
+    llr_check(dims5[0]);
+}
+
+}  // namespace
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+void* dev_alloc(size_t bytes)
+{
+    void* p = nullptr;
+    if (bytes == 0) bytes = 16;
+    HASTY_HIP_CHECK(hipMalloc(&p, bytes));
+    return p;
+}
+
+void dev_free(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+int current_device()
+{
+    int d = 0;
+    HASTY_HIP_CHECK(hipGetDevice(&d));
+    return d;
+}
+
+}  // namespace hasty_hip
+
+using namespace hasty_hip;
+
+struct hasty_nufft_plan_s {
+    NufftPlan* impl;
+};
+
+extern "C" {
+
+const char* hasty_last_error(void) { return g_last_error.c_str(); }
+
+int hasty_version(void) { return 100; }
+
+int hasty_nufft_makeplan(int type, int dim, const int64_t* nmodes, int iflag, int ntransf, double tol, int fp64,
+                         hasty_nufft_plan* plan)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(plan != nullptr && nmodes != nullptr, "null argument");
+        *plan = nullptr;
+        NufftPlan* p = nufft_makeplan(type, dim, nmodes, iflag, ntransf, tol, fp64);
+        *plan = new hasty_nufft_plan_s{p};
+    });
+}
+
+int hasty_nufft_setpts(hasty_nufft_plan plan, int64_t M, const void* x, const void* y, const void* z, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(plan != nullptr, "null plan");
+        nufft_setpts(plan->impl, M, x, y, z, as_stream(stream));
+    });
+}
+
+int hasty_nufft_execute(hasty_nufft_plan plan, void* c, void* f, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(plan != nullptr, "null plan");
+        nufft_execute(plan->impl, c, f, as_stream(stream));
+    });
+}
+
+int hasty_nufft_set_scale(hasty_nufft_plan plan, double scale)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(plan != nullptr, "null plan");
+        plan->impl->scale = scale;
+    });
+}
+
+int hasty_nufft_destroy(hasty_nufft_plan plan)
+{
+    return guarded([&] {
+        if (!plan) return;
+        nufft_destroy(plan->impl);
+        delete plan;
+    });
+}
+
+int hasty_nufft_info(hasty_nufft_plan plan, int64_t* nf, int* ns, double* beta)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(plan != nullptr, "null plan");
+        const NufftPlan* p = plan->impl;
+        for (int d = 0; d < p->dim; ++d)
+            if (nf) nf[d] = p->nf[2 - d];
+        if (ns) *ns = p->ns;
+        if (beta) *beta = p->beta;
+    });
+}
+
+int hasty_toeplitz_build(const void* coords, int coord_fp64, int64_t M, int ndim, const int64_t* nmodes,
+                         const float* weights, void* diag_ref, float* kernel, double tol, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(coords != nullptr || M == 0, "null coords");
+        HASTY_REQUIRE(nmodes != nullptr, "null nmodes");
+        toeplitz_build(coords, coord_fp64, M, ndim, nmodes, weights, reinterpret_cast<float2*>(diag_ref), kernel, tol,
+                       as_stream(stream));
+    });
+}
+
+int hasty_toeplitz_apply(const void* in, void* out, const void* mul, const void* diag, int ntransf, int ndim,
+                         const int64_t* nmodes, int mode, void* storage1, void* storage2, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(in && out && diag && storage1 && storage2 && nmodes, "null argument");
+        toeplitz_apply_ref((const float2*)in, (float2*)out, (const float2*)mul, (const float2*)diag, ntransf, ndim, nmodes,
+                           mode, (float2*)storage1, (float2*)storage2, as_stream(stream));
+    });
+}
+
+size_t hasty_sense_normal_workspace_size(const int64_t* nmodes3, int coil_batch)
+{
+    if (!nmodes3 || coil_batch < 1) return 0;
+    return sense_normal_workspace_bytes(nmodes3, coil_batch);
+}
+
+int hasty_sense_normal_toeplitz(const void* x, void* out, const void* smaps, const int32_t* coils, int ncoil,
+                                const float* kernels, const int32_t* kernel_index, const void* sub, int nproblems,
+                                const int64_t* nmodes3, void* workspace, size_t workspace_bytes, int coil_batch,
+                                void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(x && out && smaps && coils && kernels && kernel_index && nmodes3 && workspace, "null argument");
+        HASTY_REQUIRE(nproblems >= 1, "nproblems must be >= 1");
+        for (int d = 0; d < 3; ++d) HASTY_REQUIRE(nmodes3[d] >= 1, "nmodes must be >= 1");
+        sense_normal_toeplitz((const float2*)x, (float2*)out, (const float2*)smaps, coils, ncoil, kernels, kernel_index,
+                              (const float2*)sub, nproblems, nmodes3, workspace, workspace_bytes, coil_batch,
+                              as_stream(stream));
+    });
+}
+
+size_t hasty_cg_workspace_size(int64_t nproblems, int64_t n)
+{
+    return sizeof(double) * cg_workspace_doubles(nproblems, n);
+}
+
+int hasty_cg_init(int64_t nproblems, int64_t n, const void* b, const void* Ax, void* r, void* p, double* state,
+                  void* workspace, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(b && Ax && r && p && state && workspace, "null argument");
+        cg_init(nproblems, n, (const float2*)b, (const float2*)Ax, (float2*)r, (float2*)p, state, (double*)workspace,
+                as_stream(stream));
+    });
+}
+
+int hasty_cg_iterate(int64_t nproblems, int64_t n, void* x, void* r, void* p, const void* Ap, double* state,
+                     void* workspace, double tol, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(x && r && p && Ap && state && workspace, "null argument");
+        cg_iterate(nproblems, n, (float2*)x, (float2*)r, (float2*)p, (const float2*)Ap, state, (double*)workspace, tol,
+                   as_stream(stream));
+    });
+}
+
+int hasty_llr_shifts(int64_t block_iter, const int64_t* block_shape, int32_t* shifts)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(block_iter >= 1, "Can't perform Normal3DBlocksSVT with block_iter < 1");
+        HASTY_REQUIRE(block_shape && shifts, "null argument");
+        std::default_random_engine generator;
+        for (int d = 0; d < 3; ++d) {
+            shifts[d * block_iter] = 0;
+            if (block_iter > 1) {
+                std::uniform_int_distribution<int> distribution(0, (int)block_shape[d]);
+                for (int64_t b = 1; b < block_iter; ++b) shifts[d * block_iter + b] = distribution(generator);
+            }
+        }
+    });
+}
+
+size_t hasty_llr_workspace_size(const int64_t* dims5, const int64_t* block_strides, const int64_t* block_shape)
+{
+    if (!dims5 || !block_strides || !block_shape) return 0;
+    int64_t nmax = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (block_strides[d] < 1) return 0;
+        nmax *= std::max<int64_t>(1, dims5[2 + d] / block_strides[d]);
+    }
+    const int64_t R = dims5[1] * block_shape[0] * block_shape[1] * block_shape[2];
+    return llr_ws_bound(nmax, dims5[0], R);
+}
+
+int hasty_llr_normal_blocks(void* data, const int64_t* dims5, const int64_t* block_strides,
+                            const int64_t* block_shape, int64_t block_iter, const int32_t* shifts, double thresh,
+                            int soft, void* workspace, size_t workspace_bytes, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(data && dims5 && block_strides && block_shape && workspace, "null argument");
+        HASTY_REQUIRE(block_iter >= 1, "Can't perform Normal3DBlocksSVT with block_iter < 1");
+        HASTY_REQUIRE(thresh >= 0.0, "threshold must be >= 0");
+        for (int d = 0; d < 3; ++d) HASTY_REQUIRE(block_strides[d] >= 1, "block strides must be >= 1");
+        check_dims5(dims5, block_shape);
+        std::vector<int32_t> sh(3 * block_iter);
+        if (shifts) {
+            std::memcpy(sh.data(), shifts, sizeof(int32_t) * sh.size());
+        } else {
+            if (hasty_llr_shifts(block_iter, block_shape, sh.data())) throw HastyError(kInvalidArgument, g_last_error);
+        }
+        const LlrGeom g = image_geom(dims5, block_shape);
+        const bool disjoint = block_strides[0] >= block_shape[0] && block_strides[1] >= block_shape[1] &&
+                              block_strides[2] >= block_shape[2];
+        hipStream_t s = as_stream(stream);
+        for (int64_t it = 0; it < block_iter; ++it) {
+            std::vector<int32_t> blocks = grid_blocks(dims5, block_strides, block_shape, sh[0 * block_iter + it],
+                                                      sh[1 * block_iter + it], sh[2 * block_iter + it]);
+            if (disjoint) {
+                run_llr_batch((float2*)data, g, blocks, (float)thresh, soft, workspace, workspace_bytes, s);
+            } else {
+                // Overlapping blocks within a pass: keep the reference's sequential order.
+                for (size_t b = 0; b < blocks.size(); b += 3) {
+                    std::vector<int32_t> one(blocks.begin() + b, blocks.begin() + b + 3);
+                    run_llr_batch((float2*)data, g, one, (float)thresh, soft, workspace, workspace_bytes, s);
+                }
+            }
+        }
+    });
+}
+
+int hasty_llr_blocks(void* data, const int64_t* dims5, const int64_t* block_shape, const int32_t* corners,
+                     int64_t nblk, double thresh, int soft, void* workspace, size_t workspace_bytes, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(data && dims5 && block_shape && workspace, "null argument");
+        HASTY_REQUIRE(nblk >= 0 && (nblk == 0 || corners), "null corners");
+        HASTY_REQUIRE(thresh >= 0.0, "threshold must be >= 0");
+        check_dims5(dims5, block_shape);
+        for (int64_t b = 0; b < nblk; ++b)
+            for (int d = 0; d < 3; ++d)
+                HASTY_REQUIRE(corners[3 * b + d] >= 0 && corners[3 * b + d] + block_shape[d] <= dims5[2 + d],
+                              "block outside the image");
+        std::vector<int32_t> c(corners, corners + 3 * nblk);
+        run_llr_batch((float2*)data, image_geom(dims5, block_shape), c, (float)thresh, soft, workspace,
+                      workspace_bytes, as_stream(stream));
+    });
+}
+
+size_t hasty_svt_workspace_size(int64_t R, int64_t F) { return llr_ws_bound(1, F, R); }
+
+size_t hasty_llr_blocks_workspace_size(const int64_t* dims5, const int64_t* block_shape, int64_t nblk)
+{
+    if (!dims5 || !block_shape || nblk < 0) return 0;
+    const int64_t R = dims5[1] * block_shape[0] * block_shape[1] * block_shape[2];
+    return llr_ws_bytes_for(std::max<int64_t>(1, nblk), dims5[0], R);
+}
+
+int hasty_svt_matrix(void* A, int64_t R, int64_t F, double thresh, int soft, void* workspace, size_t workspace_bytes,
+                     void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(A && workspace, "null argument");
+        HASTY_REQUIRE(R >= 1, "matrix must have rows");
+        HASTY_REQUIRE(thresh >= 0.0, "threshold must be >= 0");
+        llr_check(F);
+        // A row-major [R][F] matrix is a one-block "image" F x 1 x 1 x 1 x R whose
+        // column stride is 1 and row stride F.
+        LlrGeom g;
+        g.F = F;
+        g.E = 1;
+        g.X = 1;
+        g.Y = 1;
+        g.Z = R;
+        g.bx = 1;
+        g.by = 1;
+        g.bz = (int)R;
+        g.fstride = 1;
+        g.rstride = F;
+        std::vector<int32_t> c = {0, 0, 0};
+        run_llr_batch((float2*)A, g, c, (float)thresh, soft, workspace, workspace_bytes, as_stream(stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,14 @@
+// Device-side batched conjugate gradient, see cg.hip.
+#pragma once
+
+#include "common.hpp"
+
+namespace hasty_hip {
+
+size_t cg_workspace_doubles(int64_t P, int64_t n);
+void cg_init(int64_t P, int64_t n, const float2* b, const float2* Ax, float2* r, float2* p, double* state, double* ws,
+             hipStream_t s);
+void cg_iterate(int64_t P, int64_t n, float2* x, float2* r, float2* p, const float2* Ap, double* state, double* ws,
+                double tol, hipStream_t s);
+
+}  // namespace hasty_hip

@@ -1,0 +1,91 @@
+// Shared helpers for the gfx950 kernels behind libhasty_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace hasty_hip {
+
+template <typename R> struct cplx_of;
+template <> struct cplx_of<float> { using type = float2; };
+template <> struct cplx_of<double> { using type = double2; };
+template <typename R> using cplx = typename cplx_of<R>::type;
+
+template <typename T> __host__ __device__ __forceinline__ T cmul(T a, T b)
+{
+    return T{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// conj(a) * b
+template <typename T> __host__ __device__ __forceinline__ T cmulc(T a, T b)
+{
+    return T{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+template <typename T> __host__ __device__ __forceinline__ T cadd(T a, T b) { return T{a.x + b.x, a.y + b.y}; }
+template <typename T> __host__ __device__ __forceinline__ T csub(T a, T b) { return T{a.x - b.x, a.y - b.y}; }
+template <typename T, typename S> __host__ __device__ __forceinline__ T cscale(T a, S s)
+{
+    return T{a.x * s, a.y * s};
+}
+template <typename T> __host__ __device__ __forceinline__ T cconj(T a) { return T{a.x, -a.y}; }
+
+// Error type thrown inside the library; every extern "C" entry point converts it to a
+// nonzero return code and a message readable through hasty_last_error().
+struct HastyError : std::runtime_error {
+    int code;
+    HastyError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+enum ErrorCode : int {
+    kOk = 0,
+    kInvalidArgument = 1,
+    kHipError = 2,
+    kNotPositiveDefinite = 3,
+    kUnsupported = 4,
+    kOutOfMemory = 5,
+};
+
+#define HASTY_HIP_CHECK(expr)                                                                  \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            throw ::hasty_hip::HastyError(                                                     \
+                _e == hipErrorOutOfMemory ? ::hasty_hip::kOutOfMemory : ::hasty_hip::kHipError, \
+                std::string(#expr) + ": " + hipGetErrorString(_e));                           \
+        } while (0)
+
+#define HASTY_REQUIRE(cond, msg)                                                               \
+    do {                                                                                       \
+        if (!(cond)) throw ::hasty_hip::HastyError(::hasty_hip::kInvalidArgument, (msg));      \
+    } while (0)
+
+// Records the message for hasty_last_error() (thread-local).
+void set_last_error(const std::string& msg);
+
+// Kernel launch check (launch configuration errors surface here).
+inline void check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw HastyError(kHipError, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Raise a kernel's dynamic-LDS limit before a launch that needs more than 64 KiB
+// (gfx950 has 160 KiB of LDS per CU).
+template <typename K> inline void allow_lds(K kernel, size_t bytes)
+{
+    HASTY_REQUIRE(bytes <= 160 * 1024, "kernel needs more LDS than a gfx950 CU has");
+    if (bytes > 64 * 1024)
+        HASTY_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+// Device-memory helpers (synchronous allocation; never called inside hot loops).
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+int current_device();
+
+}  // namespace hasty_hip

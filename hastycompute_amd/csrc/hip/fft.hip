@@ -1,0 +1,393 @@
+// LDS-staged mixed-radix (2,3,4,5,8) Stockham FFT passes for gfx950.
+//
+// One workgroup of 256 threads transforms a bundle of B lines held in LDS (two
+// ping-pong buffers, line stride L+1 complex to spread banks). Loads and stores go
+// straight between HBM and LDS with the zero-pad / crop / centre mapping, optional
+// per-element multipliers and per-element real scales, so pad+deapodise+FFT or
+// FFT+crop+coil-multiply never need a separate elementwise pass.
+//
+// Radix passes follow the self-sorting Stockham formulation of Govindaraju et al.,
+// "High performance discrete Fourier transforms on graphics processors" (SC08).
+#include <map>
+#include <mutex>
+#include <vector>
+#include <cmath>
+
+#include "fft.hpp"
+
+namespace hasty_hip {
+
+namespace {
+
+constexpr int kFftThreads = 256;
+
+__device__ __forceinline__ int64_t line_base(const LineAddr& a, int64_t l)
+{
+    int64_t q = l / a.inner;
+    int64_t r = l - q * a.inner;
+    if (a.map) q = a.map[q];
+    return q * a.outer + r * a.unit;
+}
+
+// Multiply by i^s (s = +-1): s=+1 -> i z, s=-1 -> -i z.
+template <int S, typename T> __device__ __forceinline__ T mul_is(T z)
+{
+    if (S > 0) return T{-z.y, z.x};
+    return T{z.y, -z.x};
+}
+
+template <int RAD, int S, typename T, typename R> struct Dft;
+
+template <int S, typename T, typename R> struct Dft<2, S, T, R> {
+    __device__ __forceinline__ static void run(T* v)
+    {
+        T a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <int S, typename T, typename R> struct Dft<4, S, T, R> {
+    __device__ __forceinline__ static void run(T* v)
+    {
+        T t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        T t2 = cadd(v[1], v[3]), t3 = mul_is<S>(csub(v[1], v[3]));
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    }
+};
+
+template <int S, typename T, typename R> struct Dft<8, S, T, R> {
+    __device__ __forceinline__ static void run(T* v)
+    {
+        T e[4] = {v[0], v[2], v[4], v[6]};
+        T o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, S, T, R>::run(e);
+        Dft<4, S, T, R>::run(o);
+        const R h = (R)0.70710678118654752440084436210485;
+        // w8^k = exp(S 2 pi i k / 8)
+        T w1 = T{h * (o[1].x - (R)S * o[1].y), h * (o[1].y + (R)S * o[1].x)};   // (1 + S i) h * o1
+        T w2 = mul_is<S>(o[2]);
+        T w3 = T{h * (-o[3].x - (R)S * o[3].y), h * (-o[3].y + (R)S * o[3].x)};  // (-1 + S i) h * o3
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], w1);
+        v[5] = csub(e[1], w1);
+        v[2] = cadd(e[2], w2);
+        v[6] = csub(e[2], w2);
+        v[3] = cadd(e[3], w3);
+        v[7] = csub(e[3], w3);
+    }
+};
+
+template <int S, typename T, typename R> struct Dft<3, S, T, R> {
+    __device__ __forceinline__ static void run(T* v)
+    {
+        const R c = (R)0.86602540378443864676372317075294;  // sqrt(3)/2
+        T t = cadd(v[1], v[2]);
+        T d = mul_is<S>(csub(v[1], v[2]));
+        T m = T{v[0].x - (R)0.5 * t.x, v[0].y - (R)0.5 * t.y};
+        v[0] = cadd(v[0], t);
+        v[1] = T{m.x + c * d.x, m.y + c * d.y};
+        v[2] = T{m.x - c * d.x, m.y - c * d.y};
+    }
+};
+
+template <int S, typename T, typename R> struct Dft<5, S, T, R> {
+    __device__ __forceinline__ static void run(T* v)
+    {
+        const R c1 = (R)0.30901699437494742410229341718282;   // cos(2pi/5)
+        const R c2 = (R)-0.80901699437494742410229341718282;  // cos(4pi/5)
+        const R s1 = (R)0.95105651629515357211643933337938;   // sin(2pi/5)
+        const R s2 = (R)0.58778525229247312916870595463907;   // sin(4pi/5)
+        T a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+        T a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+        T x0 = v[0];
+        v[0] = cadd(x0, cadd(a1, a2));
+        T p1 = T{x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y};
+        T p2 = T{x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y};
+        T q1 = mul_is<S>(T{s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y});
+        T q2 = mul_is<S>(T{s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y});
+        v[1] = cadd(p1, q1);
+        v[4] = csub(p1, q1);
+        v[2] = cadd(p2, q2);
+        v[3] = csub(p2, q2);
+    }
+};
+
+template <typename R, int RAD, int S>
+__device__ __forceinline__ void stockham_pass(const cplx<R>* __restrict__ src, cplx<R>* __restrict__ dst, int L,
+                                              int Lp, int Ns, int nb, const cplx<R>* __restrict__ tw, int tid)
+{
+    const int nbf = L / RAD;
+    const int twstep = L / (Ns * RAD);
+    for (int b = tid; b < nb * nbf; b += kFftThreads) {
+        const int li = b / nbf;
+        const int j = b - li * nbf;
+        const int k = j % Ns;
+        const cplx<R>* s = src + li * Lp + j;
+        cplx<R> v[RAD];
+        v[0] = s[0];
+#pragma unroll
+        for (int r = 1; r < RAD; ++r) {
+            cplx<R> w = tw[r * k * twstep];
+            if (S > 0) w.y = -w.y;
+            v[r] = cmul(s[r * nbf], w);
+        }
+        Dft<RAD, S, cplx<R>, R>::run(v);
+        cplx<R>* d = dst + li * Lp + (j / Ns) * Ns * RAD + k;
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) d[r * Ns] = v[r];
+    }
+}
+
+template <typename R, int S>
+__device__ void fft_bundle(cplx<R>*& src, cplx<R>*& dst, int L, int Lp, int nb, uint64_t radcode, int nrad,
+                           const cplx<R>* tw, int tid)
+{
+    int Ns = 1;
+    for (int p = 0; p < nrad; ++p) {
+        const int rad = (int)((radcode >> (4 * p)) & 15u);
+        switch (rad) {
+        case 2: stockham_pass<R, 2, S>(src, dst, L, Lp, Ns, nb, tw, tid); break;
+        case 3: stockham_pass<R, 3, S>(src, dst, L, Lp, Ns, nb, tw, tid); break;
+        case 4: stockham_pass<R, 4, S>(src, dst, L, Lp, Ns, nb, tw, tid); break;
+        case 5: stockham_pass<R, 5, S>(src, dst, L, Lp, Ns, nb, tw, tid); break;
+        default: stockham_pass<R, 8, S>(src, dst, L, Lp, Ns, nb, tw, tid); break;
+        }
+        __syncthreads();
+        cplx<R>* t = src;
+        src = dst;
+        dst = t;
+        Ns *= rad;
+    }
+}
+
+struct RadixPlan {
+    uint64_t code = 0;
+    int n = 0;
+};
+
+// Input position i (0..L-1) -> input element n, or -1 for a padded zero.
+__device__ __forceinline__ int in_element(int i, int L, int n_in, int centered)
+{
+    if (!centered) return i < n_in ? i : -1;
+    const int h = n_in / 2;
+    if (i < n_in - h) return i + h;
+    if (i >= L - h) return i - (L - h);
+    return -1;
+}
+
+template <typename R> struct AxisKernelArgs {
+    AxisFft<R> a;
+    const cplx<R>* tw;
+    uint64_t radcode;
+    int nrad;
+    int lines_per_block;
+};
+
+// MODE 0: plain FFT with sign a.sign; MODE 1: conv with real kernel; MODE 2: conv with
+// complex kernel (forward FFT, multiply, inverse FFT).
+template <typename R, int MODE>
+__global__ void __launch_bounds__(kFftThreads) axis_fft_kernel(AxisKernelArgs<R> args)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const AxisFft<R>& a = args.a;
+    const int L = a.L;
+    const int Lp = L + 1;
+    const int B = args.lines_per_block;
+    const int tid = threadIdx.x;
+    const int64_t l0 = (int64_t)blockIdx.x * B;
+    const int nb = (int)min((int64_t)B, a.nlines - l0);
+
+    cplx<R>* bufA = reinterpret_cast<cplx<R>*>(smem_raw);
+    cplx<R>* bufB = bufA + B * Lp;
+    int64_t* bases = reinterpret_cast<int64_t*>(bufB + B * Lp);  // 5 * B entries
+    int64_t* b_in = bases;
+    int64_t* b_mul = bases + B;
+    int64_t* b_out = bases + 2 * B;
+    int64_t* b_omul = bases + 3 * B;
+    int64_t* b_conv = bases + 4 * B;
+
+    for (int li = tid; li < nb; li += kFftThreads) {
+        const int64_t l = l0 + li;
+        b_in[li] = line_base(a.ain, l);
+        b_out[li] = line_base(a.aout, l);
+        if (a.inmul) b_mul[li] = line_base(a.amul, l);
+        if (a.outmul) b_omul[li] = line_base(a.aomul, l);
+        if (MODE != 0) b_conv[li] = line_base(a.aconv, l);
+    }
+    __syncthreads();
+
+    // Load (pad / centre / multiply / scale).
+    const bool in_fast = a.ain.stride != 1;
+    for (int e = tid; e < nb * L; e += kFftThreads) {
+        int li, i;
+        if (in_fast) {
+            li = e % nb;
+            i = e / nb;
+        } else {
+            li = e / L;
+            i = e - li * L;
+        }
+        const int n = in_element(i, L, a.n_in, a.in_centered);
+        cplx<R> v = {(R)0, (R)0};
+        if (n >= 0) {
+            v = a.in[b_in[li] + (int64_t)n * a.ain.stride];
+            if (a.inmul) {
+                const cplx<R> m = a.inmul[b_mul[li] + (int64_t)n * a.amul.stride];
+                v = a.inmul_conj ? cmulc(m, v) : cmul(m, v);
+            }
+            if (a.pre_scale) v = cscale(v, a.pre_scale[n]);
+        }
+        bufA[li * Lp + i] = v;
+    }
+    __syncthreads();
+
+    cplx<R>* src = bufA;
+    cplx<R>* dst = bufB;
+    if (MODE == 0) {
+        if (a.sign < 0)
+            fft_bundle<R, -1>(src, dst, L, Lp, nb, args.radcode, args.nrad, args.tw, tid);
+        else
+            fft_bundle<R, 1>(src, dst, L, Lp, nb, args.radcode, args.nrad, args.tw, tid);
+    } else {
+        fft_bundle<R, -1>(src, dst, L, Lp, nb, args.radcode, args.nrad, args.tw, tid);
+        for (int e = tid; e < nb * L; e += kFftThreads) {
+            const int li = e / L;
+            const int i = e - li * L;
+            const int64_t kidx = b_conv[li] + (int64_t)i * a.aconv.stride;
+            cplx<R> v = src[li * Lp + i];
+            if (MODE == 1)
+                v = cscale(v, a.conv_real[kidx]);
+            else
+                v = cmul(v, a.conv_cplx[kidx]);
+            src[li * Lp + i] = v;
+        }
+        __syncthreads();
+        fft_bundle<R, 1>(src, dst, L, Lp, nb, args.radcode, args.nrad, args.tw, tid);
+    }
+
+    // Store (crop / centre / scale / multiply / accumulate).
+    const bool out_fast = a.aout.stride != 1;
+    for (int e = tid; e < nb * a.n_out; e += kFftThreads) {
+        int li, n;
+        if (out_fast) {
+            li = e % nb;
+            n = e / nb;
+        } else {
+            li = e / a.n_out;
+            n = e - li * a.n_out;
+        }
+        const int i = a.out_centered ? (n - a.n_out / 2 + L) % L : n;
+        cplx<R> v = cscale(src[li * Lp + i], a.scale);
+        if (a.post_scale) v = cscale(v, a.post_scale[n]);
+        if (a.outmul) {
+            const cplx<R> m = a.outmul[b_omul[li] + (int64_t)n * a.aomul.stride];
+            v = a.outmul_conj ? cmulc(m, v) : cmul(m, v);
+        }
+        const int64_t addr = b_out[li] + (int64_t)n * a.aout.stride;
+        if (a.accumulate) v = cadd(a.out[addr], v);
+        a.out[addr] = v;
+    }
+}
+
+RadixPlan make_radix_plan(int L)
+{
+    RadixPlan p;
+    int m = L;
+    const int order[5] = {8, 4, 2, 3, 5};
+    for (int oi = 0; oi < 5; ++oi) {
+        const int r = order[oi];
+        while (m % r == 0) {
+            if (p.n >= 16) throw HastyError(kUnsupported, "FFT length has too many factors");
+            p.code |= (uint64_t)r << (4 * p.n);
+            ++p.n;
+            m /= r;
+        }
+    }
+    if (m != 1) throw HastyError(kUnsupported, "FFT length " + std::to_string(L) + " is not 2,3,5-smooth");
+    return p;
+}
+
+std::mutex g_tw_mutex;
+std::map<std::tuple<int, int, int>, void*> g_tw_cache;  // (device, L, sizeof(R)) -> table
+
+}  // namespace
+
+bool fft_size_supported(int L)
+{
+    if (L < 1 || L > 8192) return false;
+    int m = L;
+    for (int r : {2, 3, 5})
+        while (m % r == 0) m /= r;
+    return m == 1;
+}
+
+int64_t next235even(int64_t n)
+{
+    if (n <= 2) return 2;
+    if (n % 2) ++n;
+    for (;; n += 2) {
+        int64_t m = n;
+        for (int p : {2, 3, 5})
+            while (m % p == 0) m /= p;
+        if (m == 1) return n;
+    }
+}
+
+template <typename R> const cplx<R>* twiddles(int L)
+{
+    const int dev = current_device();
+    std::lock_guard<std::mutex> lock(g_tw_mutex);
+    auto key = std::make_tuple(dev, L, (int)sizeof(R));
+    auto it = g_tw_cache.find(key);
+    if (it != g_tw_cache.end()) return reinterpret_cast<const cplx<R>*>(it->second);
+    std::vector<cplx<R>> host(L);
+    const long double twopi = 6.283185307179586476925286766559L;
+    for (int k = 0; k < L; ++k) {
+        const long double ang = -twopi * (long double)k / (long double)L;
+        host[k] = cplx<R>{(R)std::cos(ang), (R)std::sin(ang)};
+    }
+    void* d = dev_alloc(sizeof(cplx<R>) * L);
+    HASTY_HIP_CHECK(hipMemcpy(d, host.data(), sizeof(cplx<R>) * L, hipMemcpyHostToDevice));
+    g_tw_cache[key] = d;
+    return reinterpret_cast<const cplx<R>*>(d);
+}
+
+template <typename R> void run_axis_fft(const AxisFft<R>& a, hipStream_t stream)
+{
+    if (a.nlines <= 0 || a.n_out <= 0) return;
+    HASTY_REQUIRE(a.L >= 1 && fft_size_supported(a.L), "unsupported FFT length " + std::to_string(a.L));
+    HASTY_REQUIRE(a.n_in >= 0 && a.n_in <= a.L && a.n_out <= a.L, "FFT pad/crop sizes exceed the line length");
+    const RadixPlan rp = make_radix_plan(a.L);
+    AxisKernelArgs<R> args;
+    args.a = a;
+    args.tw = twiddles<R>(a.L);
+    args.radcode = rp.code;
+    args.nrad = rp.n;
+    const int target = sizeof(R) == 4 ? 2048 : 1024;
+    int B = target / a.L;
+    B = B < 1 ? 1 : (B > 64 ? 64 : B);
+    args.lines_per_block = B;
+    const size_t smem = sizeof(cplx<R>) * 2 * (size_t)B * (a.L + 1) + sizeof(int64_t) * 5 * B;
+    const int64_t blocks = ceil_div(a.nlines, B);
+    HASTY_REQUIRE(blocks < (1ll << 31), "too many FFT lines for one launch");
+    const int mode = a.conv_real ? 1 : (a.conv_cplx ? 2 : 0);
+    if (mode == 0)
+        hipLaunchKernelGGL((axis_fft_kernel<R, 0>), dim3((unsigned)blocks), dim3(kFftThreads), smem, stream, args);
+    else if (mode == 1)
+        hipLaunchKernelGGL((axis_fft_kernel<R, 1>), dim3((unsigned)blocks), dim3(kFftThreads), smem, stream, args);
+    else
+        hipLaunchKernelGGL((axis_fft_kernel<R, 2>), dim3((unsigned)blocks), dim3(kFftThreads), smem, stream, args);
+    check_launch("axis_fft_kernel");
+}
+
+template const cplx<float>* twiddles<float>(int);
+template const cplx<double>* twiddles<double>(int);
+template void run_axis_fft<float>(const AxisFft<float>&, hipStream_t);
+template void run_axis_fft<double>(const AxisFft<double>&, hipStream_t);
+
+}  // namespace hasty_hip

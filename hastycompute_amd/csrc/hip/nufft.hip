@@ -1,0 +1,807 @@
+// ES-kernel NUFFT type-1 (spread -> FFT -> deconvolve/crop) and type-2
+// (deconvolve/pad -> FFT -> interpolate) on gfx950.
+//
+// Method: Barnett, Magland, af Klinteberg, "A parallel non-uniform fast Fourier
+// transform library based on an 'exponential of semicircle' kernel", SIAM J. Sci.
+// Comput. 41(5), 2019: kernel phi(z) = exp(beta (sqrt(1 - (2z/ns)^2) - 1)), ns =
+// ceil(-log10(tol/10)) fine-grid cells wide, fine grid 2x oversampled.
+//
+// setpts bins points into tiles of the fine grid with a device counting sort; the
+// spreader gives each (tile, chunk of <= kMaxSub points) one workgroup that
+// accumulates into an LDS copy of the tile plus its halo with LDS atomics and then
+// flushes the tile to HBM with one global atomic per non-zero cell. Interpolation
+// walks points in tile order so the w^3 gathers hit L2.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "fft.hpp"
+#include "nufft.hpp"
+
+namespace hasty_hip {
+
+namespace {
+
+constexpr int kMaxSub = 1024;
+constexpr int kThreads = 256;
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+__device__ __forceinline__ double fine_pos(double x, int64_t nf)
+{
+    double u = x * (1.0 / kTwoPi);
+    u -= floor(u);
+    double t = u * (double)nf;
+    if (t >= (double)nf) t -= (double)nf;
+    return t;
+}
+
+__device__ __forceinline__ int64_t wrapi(int64_t i, int64_t n)
+{
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+template <typename R> __device__ __forceinline__ R es_eval(R z, R beta, R c)
+{
+    const R t = (R)1 - c * z * z;
+    return t > (R)0 ? exp(beta * (sqrt(t) - (R)1)) : (R)0;
+}
+
+template <typename R> struct PtsArgs {
+    const R* pts[3];
+    int64_t nf[3];
+    int bs[3];
+    int64_t nbins[3];
+    int64_t M;
+};
+
+template <typename R> __device__ __forceinline__ int64_t point_bin(const PtsArgs<R>& a, int64_t j)
+{
+    int64_t b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (a.pts[d]) {
+            const double t = fine_pos((double)a.pts[d][j], a.nf[d]);
+            int64_t bi = (int64_t)(t / (double)a.bs[d]);
+            if (bi >= a.nbins[d]) bi = a.nbins[d] - 1;
+            b[d] = bi;
+        } else {
+            b[d] = 0;
+        }
+    }
+    return (b[0] * a.nbins[1] + b[1]) * a.nbins[2] + b[2];
+}
+
+template <typename R> __global__ void bin_count_kernel(PtsArgs<R> a, int* counts, int* rank)
+{
+    const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (j >= a.M) return;
+    rank[j] = atomicAdd(&counts[point_bin(a, j)], 1);
+}
+
+template <typename R> __global__ void bin_scatter_kernel(PtsArgs<R> a, const int* binstart, const int* rank, int* sortidx)
+{
+    const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (j >= a.M) return;
+    sortidx[binstart[point_bin(a, j)] + rank[j]] = (int)j;
+}
+
+__global__ void fill_int_kernel(int* p, int64_t n, int v)
+{
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void subcount_kernel(const int* counts, int* nsub, int64_t nbins)
+{
+    const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (b < nbins) nsub[b] = (counts[b] + kMaxSub - 1) / kMaxSub;
+}
+
+__global__ void subfill_kernel(const int* counts, const int* nsubstart, int* subbin, int* suboff, int64_t nbins)
+{
+    const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (b >= nbins) return;
+    const int s0 = nsubstart[b];
+    const int ns = nsubstart[b + 1] - s0;
+    for (int s = 0; s < ns; ++s) {
+        subbin[s0 + s] = (int)b;
+        suboff[s0 + s] = s * kMaxSub;
+    }
+}
+
+// Exclusive scan of n ints into out[0..n] (out[n] = total), one workgroup.
+__global__ void __launch_bounds__(1024) scan_kernel(const int* in, int* out, int64_t n)
+{
+    __shared__ int64_t part[1024];
+    const int tid = threadIdx.x;
+    const int64_t chunk = (n + 1023) / 1024;
+    const int64_t b0 = tid * chunk;
+    const int64_t b1 = min(n, b0 + chunk);
+    int64_t s = 0;
+    for (int64_t i = b0; i < b1; ++i) s += in[i];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int64_t v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int64_t run = part[tid] - s;
+    for (int64_t i = b0; i < b1; ++i) {
+        out[i] = (int)run;
+        run += in[i];
+    }
+    if (tid == 1023) out[n] = (int)part[1023];
+}
+
+template <typename R> __global__ void zero_kernel(cplx<R>* p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) p[i] = cplx<R>{(R)0, (R)0};
+}
+
+template <typename R> struct SpreadArgs {
+    PtsArgs<R> pa;
+    const int* sortidx;
+    const int* binstart;
+    const int* subbin;
+    const int* suboff;
+    const cplx<R>* c;  // strengths [M] (type 1 input) or values out (type 2)
+    cplx<R>* cout;
+    cplx<R>* fine;
+    R beta;
+    R cfac;  // 4 / ns^2
+};
+
+template <typename R, int NS, int DIM>
+__global__ void __launch_bounds__(kThreads) spread_kernel(SpreadArgs<R> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<R>* tile = reinterpret_cast<cplx<R>*>(smem_raw);
+    const PtsArgs<R>& pa = a.pa;
+    const int sp = blockIdx.x;
+    const int bin = a.subbin[sp];
+    const int start = a.binstart[bin] + a.suboff[sp];
+    const int end = min(a.binstart[bin + 1], start + kMaxSub);
+
+    int64_t bb[3];
+    {
+        int64_t r = bin;
+        bb[2] = r % pa.nbins[2];
+        r /= pa.nbins[2];
+        bb[1] = r % pa.nbins[1];
+        bb[0] = r / pa.nbins[1];
+    }
+    int T[3];
+    int64_t o[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const bool act = d >= 3 - DIM;
+        T[d] = act ? pa.bs[d] + NS + 1 : 1;
+        o[d] = act ? bb[d] * pa.bs[d] - (NS + 1) / 2 : 0;
+    }
+    const int cells = T[0] * T[1] * T[2];
+    for (int i = threadIdx.x; i < cells; i += kThreads) tile[i] = cplx<R>{(R)0, (R)0};
+    __syncthreads();
+
+    for (int p = start + threadIdx.x; p < end; p += kThreads) {
+        const int j = a.sortidx[p];
+        const cplx<R> cj = a.c[j];
+        R w[2][NS];
+        int rel[3] = {0, 0, 0};
+        double t0 = 0.0, l00 = 0.0;
+#pragma unroll
+        for (int d = 3 - DIM; d < 3; ++d) {
+            const double t = fine_pos((double)pa.pts[d][j], pa.nf[d]);
+            const double l0 = ceil(t - 0.5 * NS);
+            rel[d] = (int)((int64_t)l0 - o[d]);
+            if (d == 0) {
+                t0 = t;
+                l00 = l0;
+            } else {
+#pragma unroll
+                for (int q = 0; q < NS; ++q) w[d - 1][q] = es_eval<R>((R)(l0 + q - t), a.beta, a.cfac);
+            }
+        }
+        if (DIM == 3) {
+            for (int ia = 0; ia < NS; ++ia) {
+                const R wa = es_eval<R>((R)(l00 + ia - t0), a.beta, a.cfac);
+#pragma unroll
+                for (int ib = 0; ib < NS; ++ib) {
+                    const R wab = wa * w[0][ib];
+                    cplx<R>* row = tile + ((rel[0] + ia) * T[1] + rel[1] + ib) * T[2] + rel[2];
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) {
+                        const R ww = wab * w[1][q];
+                        atomicAdd(&row[q].x, ww * cj.x);
+                        atomicAdd(&row[q].y, ww * cj.y);
+                    }
+                }
+            }
+        } else if (DIM == 2) {
+#pragma unroll
+            for (int ib = 0; ib < NS; ++ib) {
+                const R wb = w[0][ib];
+                cplx<R>* row = tile + (rel[1] + ib) * T[2] + rel[2];
+#pragma unroll
+                for (int q = 0; q < NS; ++q) {
+                    const R ww = wb * w[1][q];
+                    atomicAdd(&row[q].x, ww * cj.x);
+                    atomicAdd(&row[q].y, ww * cj.y);
+                }
+            }
+        } else {
+            cplx<R>* row = tile + rel[2];
+#pragma unroll
+            for (int q = 0; q < NS; ++q) {
+                atomicAdd(&row[q].x, w[1][q] * cj.x);
+                atomicAdd(&row[q].y, w[1][q] * cj.y);
+            }
+        }
+    }
+    __syncthreads();
+
+    for (int i = threadIdx.x; i < cells; i += kThreads) {
+        const cplx<R> v = tile[i];
+        if (v.x == (R)0 && v.y == (R)0) continue;
+        const int i2 = i % T[2];
+        const int i1 = (i / T[2]) % T[1];
+        const int i0 = i / (T[2] * T[1]);
+        const int64_t g0 = wrapi(o[0] + i0, pa.nf[0]);
+        const int64_t g1 = wrapi(o[1] + i1, pa.nf[1]);
+        const int64_t g2 = wrapi(o[2] + i2, pa.nf[2]);
+        cplx<R>* dst = a.fine + (g0 * pa.nf[1] + g1) * pa.nf[2] + g2;
+        atomicAdd(&dst->x, v.x);
+        atomicAdd(&dst->y, v.y);
+    }
+}
+
+template <typename R, int NS, int DIM>
+__global__ void __launch_bounds__(kThreads) interp_kernel(SpreadArgs<R> a)
+{
+    const PtsArgs<R>& pa = a.pa;
+    const int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (p >= pa.M) return;
+    const int j = a.sortidx[p];
+    R w[2][NS];
+    int64_t idx[2][NS];
+    double t0 = 0.0, l00 = 0.0;
+#pragma unroll
+    for (int d = 3 - DIM; d < 3; ++d) {
+        const double t = fine_pos((double)pa.pts[d][j], pa.nf[d]);
+        const double l0 = ceil(t - 0.5 * NS);
+        if (d == 0) {
+            t0 = t;
+            l00 = l0;
+        } else {
+            const int64_t il0 = (int64_t)l0;
+#pragma unroll
+            for (int q = 0; q < NS; ++q) {
+                w[d - 1][q] = es_eval<R>((R)(l0 + q - t), a.beta, a.cfac);
+                idx[d - 1][q] = wrapi(il0 + q, pa.nf[d]);
+            }
+        }
+    }
+    R sr = 0, si = 0;
+    if (DIM == 3) {
+        for (int ia = 0; ia < NS; ++ia) {
+            const R wa = es_eval<R>((R)(l00 + ia - t0), a.beta, a.cfac);
+            const int64_t i0 = wrapi((int64_t)l00 + ia, pa.nf[0]);
+#pragma unroll
+            for (int ib = 0; ib < NS; ++ib) {
+                const R wab = wa * w[0][ib];
+                const cplx<R>* row = a.fine + (i0 * pa.nf[1] + idx[0][ib]) * pa.nf[2];
+                R tr = 0, ti = 0;
+#pragma unroll
+                for (int q = 0; q < NS; ++q) {
+                    const cplx<R> g = row[idx[1][q]];
+                    tr += w[1][q] * g.x;
+                    ti += w[1][q] * g.y;
+                }
+                sr += wab * tr;
+                si += wab * ti;
+            }
+        }
+    } else if (DIM == 2) {
+#pragma unroll
+        for (int ib = 0; ib < NS; ++ib) {
+            const cplx<R>* row = a.fine + idx[0][ib] * pa.nf[2];
+            R tr = 0, ti = 0;
+#pragma unroll
+            for (int q = 0; q < NS; ++q) {
+                const cplx<R> g = row[idx[1][q]];
+                tr += w[1][q] * g.x;
+                ti += w[1][q] * g.y;
+            }
+            sr += w[0][ib] * tr;
+            si += w[0][ib] * ti;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const cplx<R> g = a.fine[idx[1][q]];
+            sr += w[1][q] * g.x;
+            si += w[1][q] * g.y;
+        }
+    }
+    a.cout[j] = cplx<R>{sr, si};
+}
+
+template <typename R, int NS, int DIM> void launch_spread_ns(const SpreadArgs<R>& a, int64_t nsub, size_t smem, hipStream_t s)
+{
+    allow_lds(spread_kernel<R, NS, DIM>, smem);
+    hipLaunchKernelGGL((spread_kernel<R, NS, DIM>), dim3((unsigned)nsub), dim3(kThreads), smem, s, a);
+}
+template <typename R, int NS, int DIM> void launch_interp_ns(const SpreadArgs<R>& a, int64_t M, hipStream_t s)
+{
+    hipLaunchKernelGGL((interp_kernel<R, NS, DIM>), dim3((unsigned)ceil_div(M, kThreads)), dim3(kThreads), 0, s, a);
+}
+
+template <typename R, int DIM>
+void dispatch_ns(bool spread, int ns, const SpreadArgs<R>& a, int64_t n, size_t smem, hipStream_t s)
+{
+#define HASTY_NS_CASE(K)                                              \
+    case K:                                                           \
+        if (spread)                                                   \
+            launch_spread_ns<R, K, DIM>(a, n, smem, s);               \
+        else                                                          \
+            launch_interp_ns<R, K, DIM>(a, n, s);                     \
+        break;
+    switch (ns) {
+        HASTY_NS_CASE(2)
+        HASTY_NS_CASE(3)
+        HASTY_NS_CASE(4)
+        HASTY_NS_CASE(5)
+        HASTY_NS_CASE(6)
+        HASTY_NS_CASE(7)
+        HASTY_NS_CASE(8)
+        HASTY_NS_CASE(9)
+        HASTY_NS_CASE(10)
+        HASTY_NS_CASE(11)
+        HASTY_NS_CASE(12)
+        HASTY_NS_CASE(13)
+        HASTY_NS_CASE(14)
+        HASTY_NS_CASE(15)
+        HASTY_NS_CASE(16)
+    default: throw HastyError(kUnsupported, "kernel width out of range");
+    }
+#undef HASTY_NS_CASE
+    check_launch(spread ? "spread_kernel" : "interp_kernel");
+}
+
+template <typename R> void dispatch_dim(bool spread, const NufftPlan* p, const SpreadArgs<R>& a, int64_t n, size_t smem,
+                                        hipStream_t s)
+{
+    if (p->dim == 3)
+        dispatch_ns<R, 3>(spread, p->ns, a, n, smem, s);
+    else if (p->dim == 2)
+        dispatch_ns<R, 2>(spread, p->ns, a, n, smem, s);
+    else
+        dispatch_ns<R, 1>(spread, p->ns, a, n, smem, s);
+}
+
+// Gauss-Legendre nodes/weights on [-1, 1] by Newton's method on P_n.
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w)
+{
+    x.resize(n);
+    w.resize(n);
+    const double pi = 3.141592653589793238462643383279502884;
+    for (int i = 0; i < n; ++i) {
+        double z = std::cos(pi * (i + 0.75) / (n + 0.5));
+        double dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        x[i] = z;
+        w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+}
+
+// 1/Phi(k), k = m - n/2, Phi(k) = integral of the ES kernel times cos(2 pi k z / nf).
+std::vector<double> es_deconv(int64_t nf, int ns, double beta, int64_t n)
+{
+    std::vector<double> xg, wg;
+    gauss_legendre(200, xg, wg);
+    std::vector<double> out(n);
+    for (int64_t m = 0; m < n; ++m) {
+        const double k = (double)(m - n / 2);
+        double s = 0.0;
+        for (size_t q = 0; q < xg.size(); ++q) {
+            const double z = (xg[q] + 1.0) * (ns / 4.0);
+            const double u = 2.0 * z / ns;
+            const double phi = std::exp(beta * (std::sqrt(std::max(1.0 - u * u, 0.0)) - 1.0));
+            s += wg[q] * (ns / 4.0) * phi * std::cos(kTwoPi * k * z / (double)nf);
+        }
+        out[m] = 1.0 / (2.0 * s);
+    }
+    return out;
+}
+
+template <typename R> void upload_real(void*& dst, const std::vector<double>& v)
+{
+    std::vector<R> h(v.begin(), v.end());
+    dst = dev_alloc(sizeof(R) * h.size());
+    HASTY_HIP_CHECK(hipMemcpy(dst, h.data(), sizeof(R) * h.size(), hipMemcpyHostToDevice));
+}
+
+void launch_fill(int* p, int64_t n, int v, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)ceil_div(n, kThreads)), dim3(kThreads), 0, s, p, n, v);
+    check_launch("fill_int_kernel");
+}
+
+template <typename R> PtsArgs<R> pts_args(const NufftPlan* p)
+{
+    PtsArgs<R> a;
+    for (int d = 0; d < 3; ++d) {
+        a.pts[d] = reinterpret_cast<const R*>(p->pts[d]);
+        a.nf[d] = p->nf[d];
+        a.bs[d] = p->bs[d];
+        a.nbins[d] = p->nbins[d];
+    }
+    a.M = p->M;
+    return a;
+}
+
+}  // namespace
+
+void es_params(double tol, int* ns, double* beta)
+{
+    int w = (int)std::ceil(-std::log10(tol / 10.0));
+    w = std::max(2, std::min(16, w));
+    double c = 2.30;
+    if (w == 2) c = 2.20;
+    else if (w == 3) c = 2.26;
+    else if (w == 4) c = 2.38;
+    *ns = w;
+    *beta = c * w;
+}
+
+template <typename R> void zero_fill(cplx<R>* p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL((zero_kernel<R>), dim3((unsigned)ceil_div(n, kThreads)), dim3(kThreads), 0, s, p, n);
+    check_launch("zero_kernel");
+}
+template void zero_fill<float>(cplx<float>*, int64_t, hipStream_t);
+template void zero_fill<double>(cplx<double>*, int64_t, hipStream_t);
+
+NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int iflag, int ntransf, double tol, int fp64)
+{
+    HASTY_REQUIRE(type == 1 || type == 2, "Only Nufft Type 1 and Type 2 is supported");
+    HASTY_REQUIRE(dim >= 1 && dim <= 3, "Dimension must be 1,2 or 3");
+    HASTY_REQUIRE(ntransf >= 1, "ntransf must be >= 1");
+    HASTY_REQUIRE(tol > 0.0, "tol must be positive");
+    auto* p = new NufftPlan();
+    try {
+        p->type = type;
+        p->dim = dim;
+        p->isign = iflag >= 0 ? 1 : -1;
+        p->ntransf = ntransf;
+        p->fp64 = fp64 ? 1 : 0;
+        p->tol = tol;
+        p->device = current_device();
+        for (int d = 0; d < dim; ++d) {
+            HASTY_REQUIRE(nmodes[d] >= 1, "nmodes must be >= 1");
+            p->n[2 - d] = nmodes[d];
+        }
+        es_params(tol, &p->ns, &p->beta);
+        const size_t csz = p->fp64 ? sizeof(double2) : sizeof(float2);
+        for (int d = 3 - dim; d < 3; ++d) {
+            p->nf[d] = next235even(std::max<int64_t>(2 * p->n[d], 2 * p->ns));
+            HASTY_REQUIRE(fft_size_supported((int)p->nf[d]), "fine grid too large");
+        }
+        // Tile sizes: tile + halo fits the LDS budget (32 KiB for complex64, 128 KiB for complex128 in 3-D).
+        if (dim == 3) {
+            const double cells = p->fp64 ? 8000.0 : 4096.0;
+            int t = (int)std::floor(std::cbrt(cells) + 1e-9);
+            int b = t - p->ns - 1;
+            b = std::max(2, b);
+            for (int d = 0; d < 3; ++d) p->bs[d] = b;
+        } else if (dim == 2) {
+            const int b = p->fp64 ? 24 : 32;
+            p->bs[1] = p->bs[2] = b;
+        } else {
+            p->bs[2] = 512;
+        }
+        for (int d = 0; d < 3; ++d) p->nbins[d] = (d >= 3 - dim) ? ceil_div(p->nf[d], p->bs[d]) : 1;
+        const int64_t nfine = p->nf[0] * p->nf[1] * p->nf[2];
+        p->fine = dev_alloc(csz * nfine);
+        p->tmp = dev_alloc(csz * std::max<int64_t>(1, p->n[0] * p->n[1] * p->nf[2]));
+        for (int d = 3 - dim; d < 3; ++d) {
+            auto v = es_deconv(p->nf[d], p->ns, p->beta, p->n[d]);
+            if (p->fp64)
+                upload_real<double>(p->phiinv[d], v);
+            else
+                upload_real<float>(p->phiinv[d], v);
+            if (p->fp64)
+                twiddles<double>((int)p->nf[d]);
+            else
+                twiddles<float>((int)p->nf[d]);
+        }
+        const int64_t nb = p->nbins[0] * p->nbins[1] * p->nbins[2];
+        p->counts = (int*)dev_alloc(sizeof(int) * nb);
+        p->binstart = (int*)dev_alloc(sizeof(int) * (nb + 1));
+        p->nsubbin = (int*)dev_alloc(sizeof(int) * (nb + 1));
+    } catch (...) {
+        nufft_destroy(p);
+        throw;
+    }
+    return p;
+}
+
+void nufft_destroy(NufftPlan* p)
+{
+    if (!p) return;
+    dev_free(p->fine);
+    dev_free(p->tmp);
+    for (int d = 0; d < 3; ++d) dev_free(p->phiinv[d]);
+    dev_free(p->counts);
+    dev_free(p->binstart);
+    dev_free(p->nsubbin);
+    dev_free(p->rank);
+    dev_free(p->sortidx);
+    dev_free(p->subbin);
+    dev_free(p->suboff);
+    delete p;
+}
+
+namespace {
+
+template <typename R> void setpts_typed(NufftPlan* p, hipStream_t s)
+{
+    const int64_t nb = p->nbins[0] * p->nbins[1] * p->nbins[2];
+    PtsArgs<R> a = pts_args<R>(p);
+    launch_fill(p->counts, nb, 0, s);
+    if (p->M > 0) {
+        hipLaunchKernelGGL((bin_count_kernel<R>), dim3((unsigned)ceil_div(p->M, kThreads)), dim3(kThreads), 0, s, a,
+                           p->counts, p->rank);
+        check_launch("bin_count_kernel");
+    }
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, (const int*)p->counts, p->binstart, nb);
+    check_launch("scan_kernel");
+    if (p->M > 0) {
+        hipLaunchKernelGGL((bin_scatter_kernel<R>), dim3((unsigned)ceil_div(p->M, kThreads)), dim3(kThreads), 0, s, a,
+                           (const int*)p->binstart, (const int*)p->rank, p->sortidx);
+        check_launch("bin_scatter_kernel");
+    }
+    // Spreader subproblems: ceil(count / kMaxSub) per tile. The per-tile counts go into
+    // `rank` (free again after the scatter) to avoid another buffer.
+    int* nsub = p->rank;
+    hipLaunchKernelGGL(subcount_kernel, dim3((unsigned)ceil_div(nb, kThreads)), dim3(kThreads), 0, s,
+                       (const int*)p->counts, nsub, nb);
+    check_launch("subcount_kernel");
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, (const int*)nsub, p->nsubbin, nb);
+    check_launch("scan_kernel");
+    int total = 0;
+    HASTY_HIP_CHECK(hipMemcpyAsync(&total, p->nsubbin + nb, sizeof(int), hipMemcpyDeviceToHost, s));
+    HASTY_HIP_CHECK(hipStreamSynchronize(s));
+    p->nsub = total;
+    if (total > p->cap_sub) {
+        dev_free(p->subbin);
+        dev_free(p->suboff);
+        p->subbin = nullptr;
+        p->suboff = nullptr;
+        p->subbin = (int*)dev_alloc(sizeof(int) * total);
+        p->suboff = (int*)dev_alloc(sizeof(int) * total);
+        p->cap_sub = total;
+    }
+    if (total > 0) {
+        hipLaunchKernelGGL(subfill_kernel, dim3((unsigned)ceil_div(nb, kThreads)), dim3(kThreads), 0, s,
+                           (const int*)p->counts, (const int*)p->nsubbin, p->subbin, p->suboff, nb);
+        check_launch("subfill_kernel");
+    }
+}
+
+template <typename R> SpreadArgs<R> spread_args(const NufftPlan* p)
+{
+    SpreadArgs<R> a;
+    a.pa = pts_args<R>(p);
+    a.sortidx = p->sortidx;
+    a.binstart = p->binstart;
+    a.subbin = p->subbin;
+    a.suboff = p->suboff;
+    a.fine = reinterpret_cast<cplx<R>*>(p->fine);
+    a.beta = (R)p->beta;
+    a.cfac = (R)(4.0 / ((double)p->ns * p->ns));
+    a.c = nullptr;
+    a.cout = nullptr;
+    return a;
+}
+
+template <typename R> const R* phi(const NufftPlan* p, int d) { return reinterpret_cast<const R*>(p->phiinv[d]); }
+
+template <typename R> void execute_type2(NufftPlan* p, cplx<R>* c, const cplx<R>* f, hipStream_t s)
+{
+    const int64_t n0 = p->n[0], n1 = p->n[1], n2 = p->n[2];
+    const int64_t f0 = p->nf[0], f1 = p->nf[1], f2 = p->nf[2];
+    cplx<R>* fine = reinterpret_cast<cplx<R>*>(p->fine);
+    cplx<R>* tmp = reinterpret_cast<cplx<R>*>(p->tmp);
+    const int64_t nvox = n0 * n1 * n2;
+    for (int t = 0; t < p->ntransf; ++t) {
+        const cplx<R>* ft = f + t * nvox;
+        cplx<R>* ct = c + t * p->M;
+        // pass along dim 2 (contiguous): rows (i0, i1) -> [n0][n1][f2]
+        {
+            AxisFft<R> a;
+            a.in = ft;
+            a.ain = LineAddr{1, n2, 0, 1, nullptr};
+            a.out = p->dim == 1 ? fine : tmp;
+            a.aout = LineAddr{1, f2, 0, 1, nullptr};
+            a.L = (int)f2;
+            a.n_in = (int)n2;
+            a.n_out = (int)f2;
+            a.in_centered = 1;
+            a.pre_scale = phi<R>(p, 2);
+            a.scale = (R)p->scale;
+            a.sign = p->isign;
+            a.nlines = n0 * n1;
+            run_axis_fft<R>(a, s);
+        }
+        if (p->dim >= 2) {  // dim 1: [n0][n1][f2] -> [n0][f1][f2] (first n0 planes of fine)
+            AxisFft<R> a;
+            a.in = tmp;
+            a.ain = LineAddr{f2, n1 * f2, 1, f2, nullptr};
+            a.out = fine;
+            a.aout = LineAddr{f2, f1 * f2, 1, f2, nullptr};
+            a.L = (int)f1;
+            a.n_in = (int)n1;
+            a.n_out = (int)f1;
+            a.in_centered = 1;
+            a.pre_scale = phi<R>(p, 1);
+            a.sign = p->isign;
+            a.nlines = n0 * f2;
+            run_axis_fft<R>(a, s);
+        }
+        if (p->dim == 3) {  // dim 0 in place: columns (j1, j2)
+            AxisFft<R> a;
+            a.in = fine;
+            a.ain = LineAddr{f1 * f2, 0, 1, f1 * f2, nullptr};
+            a.out = fine;
+            a.aout = a.ain;
+            a.L = (int)f0;
+            a.n_in = (int)n0;
+            a.n_out = (int)f0;
+            a.in_centered = 1;
+            a.pre_scale = phi<R>(p, 0);
+            a.sign = p->isign;
+            a.nlines = f1 * f2;
+            run_axis_fft<R>(a, s);
+        }
+        if (p->M > 0) {
+            SpreadArgs<R> sa = spread_args<R>(p);
+            sa.cout = ct;
+            dispatch_dim<R>(false, p, sa, p->M, 0, s);
+        }
+    }
+}
+
+template <typename R> void execute_type1(NufftPlan* p, const cplx<R>* c, cplx<R>* f, hipStream_t s)
+{
+    const int64_t n0 = p->n[0], n1 = p->n[1], n2 = p->n[2];
+    const int64_t f0 = p->nf[0], f1 = p->nf[1], f2 = p->nf[2];
+    cplx<R>* fine = reinterpret_cast<cplx<R>*>(p->fine);
+    cplx<R>* tmp = reinterpret_cast<cplx<R>*>(p->tmp);
+    const int64_t nvox = n0 * n1 * n2;
+    int T[3];
+    for (int d = 0; d < 3; ++d) T[d] = (d >= 3 - p->dim) ? p->bs[d] + p->ns + 1 : 1;
+    const size_t smem = sizeof(cplx<R>) * (size_t)T[0] * T[1] * T[2];
+    HASTY_REQUIRE(smem <= 160 * 1024, "spreading tile exceeds LDS");
+    for (int t = 0; t < p->ntransf; ++t) {
+        const cplx<R>* ct = c + t * p->M;
+        cplx<R>* ft = f + t * nvox;
+        zero_fill<R>(fine, f0 * f1 * f2, s);
+        if (p->nsub > 0) {
+            SpreadArgs<R> sa = spread_args<R>(p);
+            sa.c = ct;
+            dispatch_dim<R>(true, p, sa, p->nsub, smem, s);
+        }
+        if (p->dim == 3) {  // dim 0 in place, crop to n0
+            AxisFft<R> a;
+            a.in = fine;
+            a.ain = LineAddr{f1 * f2, 0, 1, f1 * f2, nullptr};
+            a.out = fine;
+            a.aout = a.ain;
+            a.L = (int)f0;
+            a.n_in = (int)f0;
+            a.n_out = (int)n0;
+            a.out_centered = 1;
+            a.post_scale = phi<R>(p, 0);
+            a.sign = p->isign;
+            a.nlines = f1 * f2;
+            run_axis_fft<R>(a, s);
+        }
+        if (p->dim >= 2) {  // dim 1: [n0][f1][f2] -> [n0][n1][f2]
+            AxisFft<R> a;
+            a.in = fine;
+            a.ain = LineAddr{f2, f1 * f2, 1, f2, nullptr};
+            a.out = tmp;
+            a.aout = LineAddr{f2, n1 * f2, 1, f2, nullptr};
+            a.L = (int)f1;
+            a.n_in = (int)f1;
+            a.n_out = (int)n1;
+            a.out_centered = 1;
+            a.post_scale = phi<R>(p, 1);
+            a.sign = p->isign;
+            a.nlines = n0 * f2;
+            run_axis_fft<R>(a, s);
+        }
+        {  // dim 2: rows -> image
+            AxisFft<R> a;
+            a.in = p->dim == 1 ? fine : tmp;
+            a.ain = LineAddr{1, f2, 0, 1, nullptr};
+            a.out = ft;
+            a.aout = LineAddr{1, n2, 0, 1, nullptr};
+            a.L = (int)f2;
+            a.n_in = (int)f2;
+            a.n_out = (int)n2;
+            a.out_centered = 1;
+            a.post_scale = phi<R>(p, 2);
+            a.scale = (R)p->scale;
+            a.sign = p->isign;
+            a.nlines = n0 * n1;
+            run_axis_fft<R>(a, s);
+        }
+    }
+}
+
+}  // namespace
+
+void nufft_setpts(NufftPlan* p, int64_t M, const void* x, const void* y, const void* z, hipStream_t s)
+{
+    HASTY_REQUIRE(p != nullptr, "null plan");
+    HASTY_REQUIRE(M >= 0, "M must be >= 0");
+    const void* xyz[3] = {x, y, z};
+    for (int d = 0; d < 3; ++d) p->pts[d] = nullptr;
+    for (int d = 0; d < p->dim; ++d) {
+        HASTY_REQUIRE(xyz[d] != nullptr || M == 0, "missing coordinate array");
+        p->pts[2 - d] = xyz[d];
+    }
+    p->M = M;
+    const int64_t nb = p->nbins[0] * p->nbins[1] * p->nbins[2];
+    const int64_t need = std::max<int64_t>(M, nb);
+    if (need > p->cap_M) {
+        dev_free(p->rank);
+        dev_free(p->sortidx);
+        p->rank = nullptr;
+        p->sortidx = nullptr;
+        p->rank = (int*)dev_alloc(sizeof(int) * need);
+        p->sortidx = (int*)dev_alloc(sizeof(int) * need);
+        p->cap_M = need;
+    }
+    if (p->fp64)
+        setpts_typed<double>(p, s);
+    else
+        setpts_typed<float>(p, s);
+}
+
+void nufft_execute(NufftPlan* p, void* c, void* f, hipStream_t s)
+{
+    HASTY_REQUIRE(p != nullptr, "null plan");
+    HASTY_REQUIRE(c != nullptr && f != nullptr, "null data pointer");
+    if (p->fp64) {
+        if (p->type == 1)
+            execute_type1<double>(p, (const double2*)c, (double2*)f, s);
+        else
+            execute_type2<double>(p, (double2*)c, (const double2*)f, s);
+    } else {
+        if (p->type == 1)
+            execute_type1<float>(p, (const float2*)c, (float2*)f, s);
+        else
+            execute_type2<float>(p, (float2*)c, (const float2*)f, s);
+    }
+}
+
+}  // namespace hasty_hip

@@ -1,0 +1,59 @@
+// ES-kernel NUFFT type-1 / type-2 plans on gfx950 (the MI355X replacement for the
+// cufinufft plan/setpts/execute/destroy calls of lib/fft/nufft.cpp:537-768).
+#pragma once
+
+#include "common.hpp"
+
+namespace hasty_hip {
+
+struct NufftPlan {
+    int type = 1;       // 1: points -> modes, 2: modes -> points
+    int dim = 3;
+    int isign = 1;
+    int ntransf = 1;
+    int fp64 = 0;
+    double tol = 1e-6;
+    double scale = 1.0;  // applied to every output (1/sqrt(prod modes) in the reference)
+    int64_t n[3] = {1, 1, 1};   // modes, slowest dim first (torch order)
+    int64_t nf[3] = {1, 1, 1};  // fine grid
+    int ns = 2;
+    double beta = 0;
+    int bs[3] = {1, 1, 1};
+    int64_t nbins[3] = {1, 1, 1};
+    // device buffers owned by the plan
+    void* fine = nullptr;
+    void* tmp = nullptr;
+    void* phiinv[3] = {nullptr, nullptr, nullptr};
+    // points (borrowed: must outlive the plan, as with cufinufft)
+    int64_t M = 0;
+    const void* pts[3] = {nullptr, nullptr, nullptr};
+    int* counts = nullptr;    // nbins
+    int* binstart = nullptr;  // nbins + 1
+    int* rank = nullptr;      // M
+    int* sortidx = nullptr;   // M
+    int* subbin = nullptr;    // nsub
+    int* suboff = nullptr;    // nsub
+    int* nsubbin = nullptr;   // nbins + 1 (scan)
+    int64_t cap_M = 0;
+    int64_t cap_sub = 0;
+    int64_t nsub = 0;
+    int device = 0;
+};
+
+// nmodes in FINUFFT order (fastest-varying first), as the reference passes
+// _nmodes_flipped (lib/fft/nufft.cpp:430-432).
+NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int iflag, int ntransf, double tol,
+                          int fp64);
+// x pairs with nmodes[0] (fastest), y with nmodes[1], z with nmodes[2].
+void nufft_setpts(NufftPlan* p, int64_t M, const void* x, const void* y, const void* z, hipStream_t s);
+// type 1: c [ntransf][M] -> f [ntransf][modes]; type 2: f -> c.
+void nufft_execute(NufftPlan* p, void* c, void* f, hipStream_t s);
+void nufft_destroy(NufftPlan* p);
+
+// ES kernel parameters for upsampling factor 2.
+void es_params(double tol, int* ns, double* beta);
+
+// Fill a complex buffer with zeros (a kernel, so it is safe under graph capture).
+template <typename R> void zero_fill(cplx<R>* p, int64_t n, hipStream_t s);
+
+}  // namespace hasty_hip

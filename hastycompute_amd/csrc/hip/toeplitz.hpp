@@ -1,0 +1,38 @@
+// Toeplitz-embedded normal operator: PSF build (FP64) and applies.
+#pragma once
+
+#include "common.hpp"
+
+namespace hasty_hip {
+
+// Build the doubled-grid PSF of a trajectory in FP64 and derive
+//  - diag_ref: the reference's complex64 diagonal fftn(ifftshift(psf)) with psf scaled
+//    by (2N)^-d (CUDANormalNufftToeplitz::build_diagonal, lib/fft/nufft.cpp:932-1053);
+//  - kernel: the real float32 kernel used by the fused SENSE normal operator,
+//    Re fftn(ifftshift(psf')) / ((2N)^d N^d), where psf' zeroes the k = -N planes that
+//    never reach the cropped output (so the kernel is real); it already carries the
+//    2^d that turns the reference's Toeplitz scale into SenseNormal's.
+// coords: [ndim][M] (float or double), nmodes: image dims (slowest first), weights: [M]
+// real or null. Either output may be null.
+void toeplitz_build(const void* coords, int coord_fp64, int64_t M, int ndim, const int64_t* nmodes,
+                    const float* weights, float2* diag_ref, float* kernel, double tol, hipStream_t s);
+
+// CUDANormalNufftToeplitz::apply* (nufft.cpp:1163-1197) on a complex diagonal:
+//   res = ifftn(fftn(in, s=2N) * diag)[:N]; mode 0: out = res, 1: out += res,
+//   2: out += res * mul, 3: out = res with out == in allowed.
+// in/out: [ntransf][N...]; diag: [ntransf][2N...]; storage: >= ntransf*N0*(2N1)*(2N2)
+// complex each (the reference's storage1/storage2 are larger and suffice).
+void toeplitz_apply_ref(const float2* in, float2* out, const float2* mul, const float2* diag, int ntransf, int ndim,
+                        const int64_t* nmodes, int mode, float2* storage1, float2* storage2, hipStream_t s);
+
+// Fused Toeplitz SENSE normal operator over a batch of problems:
+//   out[p] = sum_{c in coils} conj(S_c) * crop(IFFT(K[kidx[p]] * FFT(pad(S_c * x[p])))) - (sub ? sub[p] : 0)
+// x, out, sub: [P][N0][N1][N2]; smaps: [C][N0][N1][N2]; kernels: [nk][2N0][2N1][2N2] real.
+// coils: device int array of ncoil coil indices. Workspace from
+// sense_normal_workspace_bytes.
+size_t sense_normal_workspace_bytes(const int64_t* n3, int coil_batch);
+void sense_normal_toeplitz(const float2* x, float2* out, const float2* smaps, const int* coils_dev, int ncoil,
+                           const float* kernels, const int* kidx_host, const float2* sub, int P, const int64_t* n3,
+                           void* workspace, size_t ws_bytes, int coil_batch, hipStream_t s);
+
+}  // namespace hasty_hip

@@ -1,0 +1,255 @@
+"""Torch-tensor wrappers over the C ABI (include/hasty_hip.h).
+
+These are the thinnest typed layer over libhasty_hip.so: tensors must already be on
+the GPU with the documented dtype and layout; results are written into caller- or
+wrapper-allocated tensors on the current stream. The reference-named operator classes
+(hasty::fft::CUDANufft, ...) are built on the same ABI in the C++ layer
+(csrc/host) and exposed as torch.classes.Hasty*; these wrappers serve tests, the
+batched driver and the benchmark.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import torch
+
+from . import _capi as C
+
+
+def _require(cond: bool, msg: str):
+    if not cond:
+        raise RuntimeError(msg)
+
+
+def _cuda(t: torch.Tensor, name: str):
+    _require(t.is_cuda, f"{name} must reside on a GPU")
+    _require(t.is_contiguous(), f"{name} must be contiguous")
+
+
+class NufftPlan:
+    """A type-1 or type-2 NUFFT plan on the points `coords` ([ndim, M], float32 or
+    float64 on the GPU; row d pairs with image dim d). `nmodes` are the image dims
+    (slowest first). Scale multiplies every output (1/sqrt(prod nmodes) reproduces
+    hasty::fft::CUDANufft::apply, lib/fft/nufft.cpp:487-506)."""
+
+    def __init__(self, coords: torch.Tensor, nmodes: Sequence[int], nufft_type: int, isign: int,
+                 ntransf: int = 1, tol: float = 1e-6, scale: float = 1.0):
+        _cuda(coords, "coords")
+        _require(coords.dtype in (torch.float32, torch.float64), "coords must be real")
+        self.coords = coords  # borrowed by the plan
+        self.ndim, self.M = int(coords.shape[0]), int(coords.shape[1])
+        _require(len(nmodes) == self.ndim, "nmodes must match coords.size(0)")
+        self.nmodes = [int(n) for n in nmodes]
+        self.type = nufft_type
+        self.ntransf = ntransf
+        self.fp64 = coords.dtype == torch.float64
+        self.cdtype = torch.complex128 if self.fp64 else torch.complex64
+        h = ctypes.c_void_p()
+        C.call("hasty_nufft_makeplan", nufft_type, self.ndim, C.i64arr(list(reversed(self.nmodes))), isign, ntransf,
+               float(tol), int(self.fp64), ctypes.byref(h))
+        self._h = h
+        C.call("hasty_nufft_set_scale", h, float(scale))
+        es = coords.element_size()
+        base = coords.data_ptr()
+        rows = [base + d * self.M * es for d in range(self.ndim)]
+        xyz = [rows[self.ndim - 1 - i] if i < self.ndim else None for i in range(3)]
+        C.call("hasty_nufft_setpts", h, self.M, xyz[0], xyz[1], xyz[2], C.stream_ptr())
+
+    def info(self):
+        nf = C.i64arr([0, 0, 0])
+        ns = ctypes.c_int()
+        beta = ctypes.c_double()
+        C.call("hasty_nufft_info", self._h, nf, ctypes.byref(ns), ctypes.byref(beta))
+        return list(reversed([nf[i] for i in range(self.ndim)])), ns.value, beta.value
+
+    def execute(self, c: torch.Tensor, f: torch.Tensor):
+        _cuda(c, "c")
+        _cuda(f, "f")
+        _require(c.dtype == self.cdtype and f.dtype == self.cdtype, "data dtype does not match the plan")
+        _require(c.numel() == self.ntransf * self.M, "c must hold ntransf x M values")
+        _require(f.numel() == self.ntransf * int(torch.tensor(self.nmodes).prod()), "f must hold ntransf x modes")
+        C.call("hasty_nufft_execute", self._h, c.data_ptr(), f.data_ptr(), C.stream_ptr())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            C.call("hasty_nufft_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def nufft(coords: torch.Tensor, data: torch.Tensor, nmodes: Sequence[int], nufft_type: int, isign: int,
+          tol: float = 1e-6, scale: float = 1.0) -> torch.Tensor:
+    """One-shot NUFFT: type 1 maps [ntransf, M] -> [ntransf, *nmodes], type 2 the reverse."""
+    ntransf = data.shape[0]
+    plan = NufftPlan(coords, nmodes, nufft_type, isign, ntransf, tol, scale)
+    if nufft_type == 1:
+        out = torch.empty((ntransf, *nmodes), dtype=plan.cdtype, device=data.device)
+        plan.execute(data, out)
+    else:
+        out = torch.empty((ntransf, plan.M), dtype=plan.cdtype, device=data.device)
+        plan.execute(out, data)
+    plan.close()
+    return out
+
+
+def toeplitz_build(coords: torch.Tensor, nmodes: Sequence[int], weights: Optional[torch.Tensor] = None,
+                   tol: float = 1e-10, diag: bool = True, kernel: bool = True):
+    """(diag_ref complex64 [2N...], kernel float32 [2N...]) of a trajectory; see
+    hasty_toeplitz_build."""
+    _cuda(coords, "coords")
+    L = [2 * int(n) for n in nmodes]
+    d = torch.empty(L, dtype=torch.complex64, device=coords.device) if diag else None
+    k = torch.empty(L, dtype=torch.float32, device=coords.device) if kernel else None
+    if weights is not None:
+        _cuda(weights, "weights")
+        _require(weights.dtype == torch.float32, "weights must be float32")
+    C.call("hasty_toeplitz_build", coords.data_ptr(), int(coords.dtype == torch.float64), int(coords.shape[1]),
+           len(nmodes), C.i64arr(nmodes), C.ptr(weights), C.ptr(d), C.ptr(k), float(tol), C.stream_ptr())
+    return d, k
+
+
+def toeplitz_apply(diag: torch.Tensor, x: torch.Tensor, mode: int = 0, out: Optional[torch.Tensor] = None,
+                   mul: Optional[torch.Tensor] = None, storage1=None, storage2=None) -> torch.Tensor:
+    """CUDANormalNufftToeplitz::apply family on [ntransf, N...] complex64 data."""
+    nt, nm = x.shape[0], list(x.shape[1:])
+    if out is None:
+        out = x if mode == 3 else torch.empty_like(x)
+    need = nt
+    for n in nm:
+        need *= 2 * n
+    if storage1 is None:
+        storage1 = torch.empty(need, dtype=torch.complex64, device=x.device)
+    if storage2 is None:
+        storage2 = torch.empty_like(storage1)
+    C.call("hasty_toeplitz_apply", x.data_ptr(), out.data_ptr(), C.ptr(mul), diag.data_ptr(), nt, len(nm),
+           C.i64arr(nm), mode, storage1.data_ptr(), storage2.data_ptr(), C.stream_ptr())
+    return out
+
+
+class SenseNormalToeplitz:
+    """Batched Toeplitz SENSE normal operator (hasty_sense_normal_toeplitz):
+    out[p] = sum_c conj(S_c) T_{k(p)}(S_c x[p]) - sub[p]."""
+
+    def __init__(self, smaps: torch.Tensor, kernels: torch.Tensor, coils: Optional[Sequence[int]] = None,
+                 coil_batch: int = 4):
+        _cuda(smaps, "smaps")
+        _cuda(kernels, "kernels")
+        _require(smaps.dtype == torch.complex64 and smaps.dim() == 4, "smaps must be complex64 [C, N0, N1, N2]")
+        if kernels.dim() == 3:
+            kernels = kernels.unsqueeze(0)
+        self.smaps, self.kernels = smaps, kernels
+        self.n3 = list(smaps.shape[1:])
+        _require(list(kernels.shape[1:]) == [2 * n for n in self.n3], "kernel must be [nk, 2N0, 2N1, 2N2]")
+        coils = list(range(smaps.shape[0])) if coils is None else [int(c) for c in coils]
+        self.coils = torch.tensor(coils, dtype=torch.int32, device=smaps.device)
+        self.ncoil = len(coils)
+        self.coil_batch = max(1, min(coil_batch, self.ncoil))
+        ws = C.lib().hasty_sense_normal_workspace_size(C.i64arr(self.n3), self.coil_batch)
+        self.workspace = torch.empty(ws, dtype=torch.uint8, device=smaps.device)
+
+    def apply(self, x: torch.Tensor, out: Optional[torch.Tensor] = None, kernel_index: Optional[Sequence[int]] = None,
+              sub: Optional[torch.Tensor] = None) -> torch.Tensor:
+        _cuda(x, "x")
+        P = x.numel() // int(torch.tensor(self.n3).prod())
+        if out is None:
+            out = torch.empty_like(x)
+        kidx = [0] * P if kernel_index is None else list(kernel_index)
+        C.call("hasty_sense_normal_toeplitz", x.data_ptr(), out.data_ptr(), self.smaps.data_ptr(),
+               self.coils.data_ptr(), self.ncoil, self.kernels.data_ptr(), C.i32arr(kidx), C.ptr(sub), P,
+               C.i64arr(self.n3), self.workspace.data_ptr(), self.workspace.numel(), self.coil_batch,
+               C.stream_ptr())
+        return out
+
+
+class DeviceCG:
+    """Batched device CG (hasty_cg_*) over `nproblems` vectors of n complex64 values."""
+
+    def __init__(self, nproblems: int, n: int, device):
+        self.P, self.n = nproblems, n
+        self.state = torch.zeros((nproblems, 8), dtype=torch.float64, device=device)
+        ws = C.lib().hasty_cg_workspace_size(nproblems, n)
+        self.ws = torch.empty(ws, dtype=torch.uint8, device=device)
+
+    def init(self, b, Ax, r, p):
+        C.call("hasty_cg_init", self.P, self.n, b.data_ptr(), Ax.data_ptr(), r.data_ptr(), p.data_ptr(),
+               self.state.data_ptr(), self.ws.data_ptr(), C.stream_ptr())
+
+    def iterate(self, x, r, p, Ap, tol: float = 0.0):
+        C.call("hasty_cg_iterate", self.P, self.n, x.data_ptr(), r.data_ptr(), p.data_ptr(), Ap.data_ptr(),
+               self.state.data_ptr(), self.ws.data_ptr(), float(tol), C.stream_ptr())
+
+
+def cg_solve(apply_A, b: torch.Tensor, x0: torch.Tensor, iters: int, tol: float = 0.0, check_pd: bool = True):
+    """ConjugateGradient::run (lib/op/opalgs.cpp:50-96) for a batch [P, ...] with the
+    device CG; apply_A(v, out) writes A v into out. Returns (x, state)."""
+    P = b.shape[0]
+    n = b[0].numel()
+    x = x0.clone()
+    r = torch.empty_like(b)
+    p = torch.empty_like(b)
+    Ap = torch.empty_like(b)
+    cg = DeviceCG(P, n, b.device)
+    if iters < 1:
+        return x, cg.state
+    apply_A(x, Ap)
+    cg.init(b, Ap, r, p)
+    for _ in range(iters):
+        apply_A(p, Ap)
+        cg.iterate(x, r, p, Ap, tol)
+    if check_pd:
+        st = cg.state.cpu()
+        if bool((st[:, 4] != 0).any()):
+            raise RuntimeError("A was not positive definite")
+    return x, cg.state
+
+
+def llr_shifts(block_iter: int, block_shape: Sequence[int]):
+    out = C.i32arr([0] * (3 * block_iter))
+    C.call("hasty_llr_shifts", block_iter, C.i64arr(block_shape), out)
+    return [[out[d * block_iter + b] for b in range(block_iter)] for d in range(3)]
+
+
+def llr_normal_blocks(x: torch.Tensor, block_strides, block_shape, block_iter: int, thresh: float, soft: bool,
+                      shifts=None, workspace: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """OldNormal3DBlocksSVT::apply on a [F, E, X, Y, Z] complex64 GPU tensor, in place."""
+    _cuda(x, "input")
+    _require(x.dtype == torch.complex64 and x.dim() == 5, "input must be complex64 [F, E, X, Y, Z]")
+    dims = list(x.shape)
+    ws_bytes = C.lib().hasty_llr_workspace_size(C.i64arr(dims), C.i64arr(block_strides), C.i64arr(block_shape))
+    if workspace is None or workspace.numel() < ws_bytes:
+        workspace = torch.empty(ws_bytes, dtype=torch.uint8, device=x.device)
+    sh = None
+    if shifts is not None:
+        sh = C.i32arr([int(v) for row in shifts for v in row])
+    C.call("hasty_llr_normal_blocks", x.data_ptr(), C.i64arr(dims), C.i64arr(block_strides), C.i64arr(block_shape),
+           block_iter, sh, float(thresh), int(bool(soft)), workspace.data_ptr(), workspace.numel(), C.stream_ptr())
+    return x
+
+
+def llr_blocks(x: torch.Tensor, block_shape, corners, thresh: float, soft: bool) -> torch.Tensor:
+    _cuda(x, "input")
+    dims = list(x.shape)
+    n = len(corners)
+    ws_bytes = C.lib().hasty_llr_blocks_workspace_size(C.i64arr(dims), C.i64arr(block_shape), n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=x.device)
+    flat = C.i32arr([int(v) for c in corners for v in c])
+    C.call("hasty_llr_blocks", x.data_ptr(), C.i64arr(dims), C.i64arr(block_shape), flat, n, float(thresh),
+           int(bool(soft)), ws.data_ptr(), ws.numel(), C.stream_ptr())
+    return x
+
+
+def svt_matrix(A: torch.Tensor, thresh: float, soft: bool) -> torch.Tensor:
+    """svt_soft / svt_hard on a contiguous [R, F] complex64 GPU matrix, in place."""
+    _cuda(A, "A")
+    _require(A.dtype == torch.complex64 and A.dim() == 2, "A must be complex64 [R, F]")
+    R, F = A.shape
+    ws = torch.empty(C.lib().hasty_svt_workspace_size(R, F), dtype=torch.uint8, device=A.device)
+    C.call("hasty_svt_matrix", A.data_ptr(), R, F, float(thresh), int(bool(soft)), ws.data_ptr(), ws.numel(),
+           C.stream_ptr())
+    return A

@@ -1,0 +1,150 @@
+/*
+ * libhasty_hip: C ABI of the MI355X (gfx950) implementation of HastyCompute's batched
+ * SENSE hot path. Plain pointers and sizes only; every pointer named "device" or any
+ * data array is HIP device memory; `stream` is a hipStream_t (NULL = default stream).
+ *
+ * Every function returns 0 on success and a nonzero code on error (1 invalid
+ * argument, 2 HIP runtime error, 3 matrix not positive definite, 4 unsupported,
+ * 5 out of memory); hasty_last_error() returns the message of the calling thread's
+ * last failure. No exception crosses this boundary (the reference's C++ layer throws
+ * std::runtime_error and its FFI converts through future_catcher,
+ * lib/torch_util.cpp:160-200; the C++ layer over this ABI rethrows the message).
+ *
+ * Complex data is interleaved (re, im) float32 ("complex64") unless a function says
+ * otherwise. Multi-dimensional arrays are row-major with the slowest dimension first,
+ * as torch tensors are.
+ */
+#ifndef HASTY_HIP_H
+#define HASTY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* hasty_last_error(void);
+/* ABI version (major * 100 + minor). */
+int hasty_version(void);
+
+/* ---------------------------------------------------------------------------------
+ * NUFFT plans. Same call shape as the cufinufft API the reference binds:
+ * cufinufft(f)_makeplan / _setpts / _execute / _destroy at lib/fft/nufft.cpp:537-637
+ * (makeplan + setpts), 639-768 (execute), 441-485 (destroy); plus an explicit stream.
+ * nmodes is in FINUFFT order (fastest-varying first), as the reference passes
+ * _nmodes_flipped (nufft.cpp:430-432); x pairs with nmodes[0] (the reference passes
+ * coords row ndim-1 as x, nufft.cpp:579-583). Frequencies use modeord 0 (mode index n
+ * is k = n - floor(N/2)); type 1 (points -> modes) and type 2 (modes -> points) sums
+ * use sign iflag (>= 0: +1). fp64 selects complex128/float64 data and coordinates.
+ * Coordinates are borrowed: they must stay valid until setpts is called again or
+ * the plan is destroyed (as with cufinufft).
+ * ------------------------------------------------------------------------------- */
+typedef struct hasty_nufft_plan_s* hasty_nufft_plan;
+
+int hasty_nufft_makeplan(int type, int dim, const int64_t* nmodes, int iflag, int ntransf, double tol, int fp64,
+                         hasty_nufft_plan* plan);
+int hasty_nufft_setpts(hasty_nufft_plan plan, int64_t M, const void* x, const void* y, const void* z, void* stream);
+/* type 1: c [ntransf][M] -> f [ntransf][modes]; type 2: f -> c. Unscaled, like cufinufft. */
+int hasty_nufft_execute(hasty_nufft_plan plan, void* c, void* f, void* stream);
+/* Multiply every output by `scale` inside the pipeline (the reference divides by
+ * sqrt(prod nmodes) after each transform, nufft.cpp:487-506). Default 1. */
+int hasty_nufft_set_scale(hasty_nufft_plan plan, double scale);
+int hasty_nufft_destroy(hasty_nufft_plan plan);
+/* Fine grid (FINUFFT order), kernel width and ES shape of a plan. */
+int hasty_nufft_info(hasty_nufft_plan plan, int64_t* nf, int* ns, double* beta);
+
+/* ---------------------------------------------------------------------------------
+ * Toeplitz-embedded normal operator.
+ * ------------------------------------------------------------------------------- */
+/* Replaces CUDANormalNufftToeplitz::build_diagonal (lib/fft/nufft.cpp:932-1053).
+ * coords [ndim][M] (float32, or float64 if coord_fp64), nmodes: image dims (slowest
+ * first), weights [M] real float32 or NULL (= 1). Built with an FP64 type-1 NUFFT at
+ * `tol` on the GPU. Outputs (either may be NULL):
+ *   diag_ref: complex64 [2N...], the reference's diagonal fftn(ifftshift(psf)) with the
+ *             doubled-grid scaling (2N)^-d;
+ *   kernel:   float32 [2N...], the real kernel of hasty_sense_normal_toeplitz. */
+int hasty_toeplitz_build(const void* coords, int coord_fp64, int64_t M, int ndim, const int64_t* nmodes,
+                         const float* weights, void* diag_ref, float* kernel, double tol, void* stream);
+
+/* Replaces CUDANormalNufftToeplitz::apply / apply_add / apply_addcmul / apply_inplace
+ * (nufft.cpp:1163-1197): res = ifftn(fftn(in, s=2N) * diag)[:N];
+ * mode 0: out = res; 1: out += res; 2: out += res * mul; 3: out = res (out may be in).
+ * in/out/mul [ntransf][N...], diag [ntransf][2N...], storage1/2 each
+ * >= ntransf * N0 * (2 N1) * (2 N2) complex64. */
+int hasty_toeplitz_apply(const void* in, void* out, const void* mul, const void* diag, int ntransf, int ndim,
+                         const int64_t* nmodes, int mode, void* storage1, void* storage2, void* stream);
+
+/* ---------------------------------------------------------------------------------
+ * Toeplitz SENSE normal operator (3-D), batched over problems.
+ * Replaces the coil loop of sense_normal_apply (lib/mri/sense.ixx:500-578) as used by
+ * CUDASenseNormal::apply and BatchedSenseNormal::apply_outer_batch
+ * (lib/mri/batch_sense.cpp:383-450): for each problem p
+ *   out[p] = sum_{c in coils} conj(S_c) N1(w N2(S_c x[p])) - sub[p]
+ * with N1 N2 through the problem's Toeplitz kernel (built with weights w).
+ * x, out, sub: [P][N0][N1][N2] complex64 (sub may be NULL); smaps [C][N0][N1][N2];
+ * coils: DEVICE int32 array of ncoil indices into smaps; kernels: float32
+ * [nk][2N0][2N1][2N2]; kernel_index: HOST int32 [P], kernel of each problem.
+ * ------------------------------------------------------------------------------- */
+size_t hasty_sense_normal_workspace_size(const int64_t* nmodes3, int coil_batch);
+int hasty_sense_normal_toeplitz(const void* x, void* out, const void* smaps, const int32_t* coils, int ncoil,
+                                const float* kernels, const int32_t* kernel_index, const void* sub, int nproblems,
+                                const int64_t* nmodes3, void* workspace, size_t workspace_bytes, int coil_batch,
+                                void* stream);
+
+/* ---------------------------------------------------------------------------------
+ * Device-side batched conjugate gradient, the loop body of ConjugateGradient::run
+ * (lib/op/opalgs.cpp:50-96) for P problems of n complex64 unknowns with FP64 scalars
+ * kept on the device (no host synchronisation per iteration).
+ * state: device double [P][8]: rzold, alpha, beta, done, not_positive_definite, pAp,
+ * iterations. The caller computes Ax / Ap with the normal operator between calls:
+ *   hasty_cg_init(b, A x0) -> r, p; loop { Ap = A p; hasty_cg_iterate(...) }.
+ * A problem stops (alpha = 0) once sqrt(rzold) < tol or when pAp <= 0, which also
+ * sets its not_positive_definite flag (the reference throws "A was not positive
+ * definite", opalgs.cpp:71-72; the C++ layer throws the same after reading state).
+ * ------------------------------------------------------------------------------- */
+size_t hasty_cg_workspace_size(int64_t nproblems, int64_t n);
+int hasty_cg_init(int64_t nproblems, int64_t n, const void* b, const void* Ax, void* r, void* p, double* state,
+                  void* workspace, void* stream);
+int hasty_cg_iterate(int64_t nproblems, int64_t n, void* x, void* r, void* p, const void* Ap, double* state,
+                     void* workspace, double tol, void* stream);
+
+/* ---------------------------------------------------------------------------------
+ * Locally-low-rank SVT.
+ * ------------------------------------------------------------------------------- */
+/* The block shifts of OldNormal3DBlocksSVT::apply (lib/op/svt.cpp:378-391): a default-
+ * constructed std::default_random_engine, uniform_int_distribution(0, block_shape[d]).
+ * shifts_out: int32 [3][block_iter]. */
+int hasty_llr_shifts(int64_t block_iter, const int64_t* block_shape, int32_t* shifts_out);
+
+size_t hasty_llr_workspace_size(const int64_t* dims5, const int64_t* block_strides, const int64_t* block_shape);
+
+/* Replaces OldNormal3DBlocksSVT::apply (svt.cpp:354-449, the TorchScript target of
+ * HastySVT.Normal3DBlocksSVT, python/cpp/py_svt.cpp:47-60) on device data
+ * [F][E][X][Y][Z] complex64 (dims5), in place: block_iter passes over the strided block
+ * grid (shifts from hasty_llr_shifts if shifts == NULL), skipping blocks whose end
+ * reaches the image edge (svt.cpp:404); each block's Casorati matrix (E*bx*by*bz x F,
+ * svt.cpp:37-60) is replaced by svt_soft(., thresh) or svt_hard(., int(thresh + 0.5)).
+ * Blocks of one pass run as one batched launch when they are disjoint. */
+int hasty_llr_normal_blocks(void* data, const int64_t* dims5, const int64_t* block_strides,
+                            const int64_t* block_shape, int64_t block_iter, const int32_t* shifts, double thresh,
+                            int soft, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Threshold an explicit list of disjoint blocks (corners: HOST int32 [nblk][3]); the
+ * batched core of Random3DBlocksSVT::apply (svt.cpp:269-343) and of the above. */
+int hasty_llr_blocks(void* data, const int64_t* dims5, const int64_t* block_shape, const int32_t* corners,
+                     int64_t nblk, double thresh, int soft, void* workspace, size_t workspace_bytes, void* stream);
+size_t hasty_llr_blocks_workspace_size(const int64_t* dims5, const int64_t* block_shape, int64_t nblk);
+
+/* svt_soft / svt_hard (svt.cpp:62-169) on one row-major [R][F] complex64 matrix, in
+ * place. workspace >= hasty_svt_workspace_size(R, F). */
+size_t hasty_svt_workspace_size(int64_t R, int64_t F);
+int hasty_svt_matrix(void* A, int64_t R, int64_t F, double thresh, int soft, void* workspace, size_t workspace_bytes,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HASTY_HIP_H */

@@ -1,0 +1,275 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle on seeded inputs.
+
+Tolerances (relative L2 vs FP64): NUFFT 1e-5 at tol 1e-6 kernels, Toeplitz/SENSE normal
+operator 1e-5, CG image 1e-5 (against the oracle's CG on the same operator), LLR 1e-4.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures.json")
+
+
+@pytest.fixture(scope="module")
+def N():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from hastycompute_amd import native
+
+    return native
+
+
+def cx(d):
+    return (np.asarray(d["re"]) + 1j * np.asarray(d["im"])).reshape(d["shape"])
+
+
+def g(a, dtype=torch.complex64):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(dtype).cuda()
+
+
+def h(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy().astype(np.complex128)
+
+
+def rnd(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("dims", [(10, 9, 8), (12, 7), (16,), (6, 5, 4)])
+@pytest.mark.parametrize("fp64", [False, True])
+def test_nufft_small_vs_direct_sum(N, dims, fp64):
+    rng = np.random.default_rng(len(dims) * 10 + fp64)
+    M = 400
+    co = rng.uniform(-np.pi, np.pi, (len(dims), M))
+    f = rnd(rng, dims)
+    c = rnd(rng, M)
+    tol = 1e-10 if fp64 else 1e-6
+    rt, ct = (torch.float64, torch.complex128) if fp64 else (torch.float32, torch.complex64)
+    cog = g(co, rt)
+    co_used = co.astype(np.float32).astype(np.float64) if not fp64 else co
+    out2 = h(N.nufft(cog, g(f[None], ct), dims, 2, -1, tol))[0]
+    out1 = h(N.nufft(cog, g(c[None], ct), dims, 1, +1, tol))[0]
+    e2 = O.rel_l2(out2, O.nudft_type2(co_used, f, -1))
+    e1 = O.rel_l2(out1, O.nudft_type1(co_used, c, dims, +1))
+    bound = 1e-8 if fp64 else 1e-5
+    print(f"dims={dims} fp64={fp64} type2 {e2:.2e} type1 {e1:.2e}", flush=True)
+    assert e2 < bound and e1 < bound
+    # the other sign
+    out2p = h(N.nufft(cog, g(f[None], ct), dims, 2, +1, tol))[0]
+    assert O.rel_l2(out2p, O.nudft_type2(co_used, f, +1)) < bound
+
+
+def test_golden_deterministic_3d(N):
+    fx = json.load(open(GOLDEN))["deterministic_3d"]
+    co = g(np.asarray(fx["coords"], dtype=np.float32), torch.float32)
+    nm = fx["nmodes"]
+    k = N.nufft(co, g(cx(fx["input"])), nm[1:], 2, -1, 1e-6, scale=1 / math.sqrt(64))
+    assert O.rel_l2(h(k), cx(fx["type2"])) < 1e-5
+    back = N.nufft(co, k, nm[1:], 1, +1, 1e-6, scale=1 / math.sqrt(64))
+    assert O.rel_l2(h(back), cx(fx["type1_of_type2"])) < 1e-5
+
+
+def test_golden_deterministic_1d(N):
+    fx = json.load(open(GOLDEN))["deterministic_1d"]
+    co = g(np.asarray(fx["coords"], dtype=np.float32), torch.float32)
+    out = N.nufft(co, g(cx(fx["input"])), [8], 2, +1, 1e-6, scale=1 / math.sqrt(8))
+    ref = cx(fx["type2"])
+    assert np.abs(h(out) - ref).max() < 1e-5 * np.abs(ref).max()
+
+
+def test_golden_compare_normal_methods(N):
+    fx = json.load(open(GOLDEN))["compare_normal_methods"]
+    co = g(np.asarray(fx["coords"], dtype=np.float32), torch.float32)
+    nm = fx["nmodes"]
+    diag, kern = N.toeplitz_build(co, nm[1:])
+    assert O.rel_l2(h(diag)[None], cx(fx["toeplitz_diagonal"])) < 1e-5
+    ones = torch.ones(nm, dtype=torch.complex64, device="cuda")
+    out3 = N.toeplitz_apply(diag[None].contiguous(), ones)
+    assert O.rel_l2(h(out3), cx(fx["toeplitz"])) < 1e-5
+    # SENSE normal with one unit coil = the explicit normal (2^d times the Toeplitz result)
+    S = torch.ones((1, *nm[1:]), dtype=torch.complex64, device="cuda")
+    op = N.SenseNormalToeplitz(S, kern)
+    out = op.apply(ones)
+    assert O.rel_l2(h(out), cx(fx["normal"])) < 1e-5
+
+
+def test_nufft_cfg1_sizes_vs_oracle(N):
+    rng = np.random.default_rng(7)
+    co = O.kooshball(2000, 64)
+    x = O.ellipsoid_phantom(64, seed=3)[0]
+    cog = g(co, torch.float32)
+    co32 = co.astype(np.float32).astype(np.float64)
+    k = h(N.nufft(cog, g(x[None]), [64, 64, 64], 2, -1, 1e-6))[0]
+    kref = O.nufft_type2(co32, x, -1, 1e-12)
+    e2 = O.rel_l2(k, kref)
+    y = rnd(rng, co.shape[1])
+    img = h(N.nufft(cog, g(y[None]), [64, 64, 64], 1, +1, 1e-6))[0]
+    e1 = O.rel_l2(img, O.nufft_type1(co32, y, (64, 64, 64), +1, 1e-12))
+    print(f"cfg1 NUFFT type2 {e2:.2e} type1 {e1:.2e}", flush=True)
+    assert e2 < 1e-5 and e1 < 1e-5
+
+
+@pytest.mark.parametrize("nm", [[1, 8, 6, 10], [2, 6, 6, 6], [1, 12, 9], [3, 20]])
+def test_toeplitz_diag_and_apply_modes(N, nm):
+    rng = np.random.default_rng(sum(nm))
+    d = len(nm) - 1
+    co = rng.uniform(-np.pi, np.pi, (d, 300))
+    co32 = co.astype(np.float32).astype(np.float64)
+    cog = g(co, torch.float32)
+    diag, _ = N.toeplitz_build(cog, nm[1:], kernel=False)
+    dref = O.toeplitz_diagonal(co32, nm, exact=True)
+    assert O.rel_l2(h(diag)[None], dref[:1]) < 1e-5
+    dg = diag.expand(nm[0], *diag.shape).contiguous()
+    x = rnd(rng, nm)
+    mul = rnd(rng, nm)
+    ref = O.toeplitz_apply(dref, x)
+    out = N.toeplitz_apply(dg, g(x), mode=0)
+    assert O.rel_l2(h(out), ref) < 1e-5
+    base = rnd(rng, nm)
+    out = g(base)
+    N.toeplitz_apply(dg, g(x), mode=1, out=out)
+    assert O.rel_l2(h(out), base + ref) < 1e-5
+    out = g(base)
+    N.toeplitz_apply(dg, g(x), mode=2, out=out, mul=g(mul))
+    assert O.rel_l2(h(out), base + ref * mul) < 1e-5
+    xi = g(x)
+    N.toeplitz_apply(dg, xi, mode=3)
+    assert O.rel_l2(h(xi), ref) < 1e-5
+
+
+@pytest.mark.parametrize("n,C,M", [(16, 4, 6000), (12, 3, 2500)])
+def test_sense_normal_toeplitz_vs_explicit(N, n, C, M):
+    rng = np.random.default_rng(n + C)
+    co = O.kooshball(M // (2 * n), 2 * n)
+    co32 = co.astype(np.float32).astype(np.float64)
+    S = O.coil_maps(C, n, seed=1)
+    x = rnd(rng, (1, n, n, n))
+    w = rng.uniform(0.5, 1.5, co.shape[1])
+    y = rnd(rng, (C, co.shape[1]))
+    coils = list(range(C))[::-1]
+    ref = O.sense_normal_apply(co32, x, S, coils)
+    cog = g(co, torch.float32)
+    _, kern = N.toeplitz_build(cog, [n, n, n])
+    op = N.SenseNormalToeplitz(g(S), kern, coils=coils, coil_batch=3)
+    out = op.apply(g(x))
+    e = O.rel_l2(h(out), ref)
+    print(f"sense normal n={n} C={C}: {e:.2e}", flush=True)
+    assert e < 1e-5
+    # gradient form A^H W (A x - y) = T_w x - A^H W y
+    wt = torch.as_tensor(w.astype(np.float32)).cuda()
+    _, kw = N.toeplitz_build(cog, [n, n, n], weights=wt)
+    refg = O.sense_normal_apply(co32, x, S, coils, kdata=y[coils], weights=w.astype(np.float32))
+    b = O.sense_adjoint_apply(co32, y[coils] * w.astype(np.float32)[None], S, coils)
+    opw = N.SenseNormalToeplitz(g(S), kw, coils=coils)
+    outg = opw.apply(g(x), sub=g(b))
+    assert O.rel_l2(h(outg), refg) < 1e-5
+
+
+def test_sense_normal_cfg1_batched_problems(N):
+    # config-1 sizes (64^3, 8 coils, 2000 spokes x 64) and two problems with distinct kernels
+    rng = np.random.default_rng(11)
+    n, C = 64, 8
+    S = O.coil_maps(C, n, seed=2)
+    cos = [O.kooshball(2000, 64, offset=0), O.kooshball(1500, 64, offset=2000)]
+    kernels = []
+    for co in cos:
+        _, k = N.toeplitz_build(g(co, torch.float32), [n, n, n])
+        kernels.append(k)
+    K = torch.stack(kernels)
+    x = rnd(rng, (2, n, n, n))
+    op = N.SenseNormalToeplitz(g(S), K, coil_batch=4)
+    out = h(op.apply(g(x), kernel_index=[1, 0]))
+    for p, kidx in enumerate([1, 0]):
+        co32 = cos[kidx].astype(np.float32).astype(np.float64)
+        diag = O.toeplitz_diagonal(co32, [1, n, n, n])
+        ref = O.sense_normal_toeplitz(diag, x[p:p + 1], S, list(range(C)))
+        e = O.rel_l2(out[p:p + 1], ref)
+        print(f"cfg1 sense normal problem {p}: {e:.2e}", flush=True)
+        assert e < 1e-5
+
+
+def test_device_cg_vs_oracle_cg(N):
+    rng = np.random.default_rng(5)
+    n, C, P, iters = 16, 4, 2, 10
+    S = O.coil_maps(C, n, seed=4)
+    cos = [O.kooshball(600, 32, offset=0), O.kooshball(500, 32, offset=700)]
+    kerns, diags = [], []
+    for co in cos:
+        d, k = N.toeplitz_build(g(co, torch.float32), [n, n, n])
+        kerns.append(k)
+        diags.append(O.toeplitz_diagonal(co.astype(np.float32).astype(np.float64), [1, n, n, n]))
+    op = N.SenseNormalToeplitz(g(S), torch.stack(kerns))
+    b = rnd(rng, (P, n, n, n))
+    x0 = np.zeros_like(b)
+
+    def A(v, out):
+        op.apply(v, out=out, kernel_index=[0, 1])
+
+    x, state = N.cg_solve(A, g(b), g(x0), iters)
+    xs = h(x)
+    for p in range(P):
+        Aref = lambda v, p=p: O.sense_normal_toeplitz(diags[p], v, S, list(range(C)))
+        xr = O.conjugate_gradient(Aref, b[p:p + 1], x0[p:p + 1], iters)
+        e = O.rel_l2(xs[p:p + 1], xr)
+        print(f"CG problem {p}: {e:.2e}", flush=True)
+        assert e < 1e-5
+    st = state.cpu().numpy()
+    assert np.all(st[:, 6] == iters) and np.all(st[:, 4] == 0)
+
+
+def test_device_cg_not_positive_definite_flag(N):
+    # A = -I: pAp < 0 on the first step; the host raises the reference's message
+    b = torch.ones((1, 4, 4, 4), dtype=torch.complex64, device="cuda")
+
+    def A(v, out):
+        out.copy_(-v)
+
+    with pytest.raises(RuntimeError, match="positive definite"):
+        N.cg_solve(A, b, torch.zeros_like(b), 3)
+
+
+def _lowrank(rng, F, E, X, rank, noise):
+    t = rnd(rng, (F, rank))
+    sp = rnd(rng, (rank, E * X * X * X))
+    return ((t @ sp).reshape(F, E, X, X, X) + noise * rnd(rng, (F, E, X, X, X))) / math.sqrt(rank)
+
+
+@pytest.mark.parametrize("F,E,X,blk,it,soft", [(20, 1, 32, 8, 3, True), (12, 2, 24, 8, 2, True),
+                                              (20, 1, 32, 8, 2, False), (100, 1, 32, 16, 2, True)])
+def test_llr_normal_blocks_vs_oracle(N, F, E, X, blk, it, soft):
+    rng = np.random.default_rng(F + X)
+    x = _lowrank(rng, F, E, X, 5, 0.05).astype(np.complex64)
+    A = O.extract_block(x.astype(np.complex128), (0, 0, 0), (blk,) * 3)
+    smax = np.linalg.svd(A, compute_uv=False)[0]
+    thresh = 0.05 * smax if soft else 3
+    ref = O.normal_3d_blocks_svt(x.astype(np.complex128), (blk,) * 3, (blk,) * 3, it, thresh, soft)
+    xg = g(x)
+    N.llr_normal_blocks(xg, (blk,) * 3, (blk,) * 3, it, thresh, soft)
+    e = O.rel_l2(h(xg), ref)
+    print(f"LLR F={F} E={E} X={X} blk={blk} iters={it} soft={soft}: {e:.2e}", flush=True)
+    assert e < 1e-4
+
+
+def test_svt_matrix_vs_numpy(N):
+    rng = np.random.default_rng(3)
+    for R, F in [(500, 20), (4096, 100), (64, 100)]:
+        A = (rnd(rng, (R, 6)) @ rnd(rng, (6, F)) + 0.1 * rnd(rng, (R, F))).astype(np.complex64)
+        s = np.linalg.svd(A.astype(np.complex128), compute_uv=False)
+        lam = 0.3 * s[2]
+        Ag = g(A)
+        N.svt_matrix(Ag, lam, True)
+        e = O.rel_l2(h(Ag), O.svt_soft(A.astype(np.complex128), lam))
+        Ah = g(A)
+        N.svt_matrix(Ah, 4, False)
+        e2 = O.rel_l2(h(Ah), O.svt_hard(A.astype(np.complex128), 4))
+        print(f"svt R={R} F={F}: soft {e:.2e} hard {e2:.2e}", flush=True)
+        assert e < 1e-4 and e2 < 1e-4

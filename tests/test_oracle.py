@@ -1,0 +1,220 @@
+"""Pins for the FP64 oracle (CPU only). Each check is chosen so that a plausible slip in
+the oracle (a transposed coordinate row, a wrong sign or mode offset, a missing scale)
+fails it; none of them re-types the oracle's own formula."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures.json")
+rng = np.random.default_rng(1234)
+
+
+def cx(d):
+    return (np.asarray(d["re"]) + 1j * np.asarray(d["im"])).reshape(d["shape"])
+
+
+def rnd(shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def cartesian(N):
+    g = [-np.pi + 2 * np.pi * np.arange(n) / n for n in N]
+    G = np.meshgrid(*g, indexing="ij")
+    return np.stack([x.ravel() for x in G])
+
+
+@pytest.mark.parametrize("N", [(7,), (6, 5), (4, 6, 5), (3, 2, 5)])
+def test_direct_sum_is_shifted_dft_on_cartesian_grid(N):
+    # x_m = -pi + 2 pi m/N makes type 2 a DFT of f with phase (-1)^k (numpy.fft pins it).
+    co = cartesian(N)
+    f = rnd(N)
+    fs = f.copy()
+    for d, n in enumerate(N):
+        sh = [1] * len(N)
+        sh[d] = n
+        fs = fs * np.exp(1j * np.pi * (np.arange(n) - n // 2)).reshape(sh)
+    # place frequency k at index k mod n, then forward DFT
+    ref = np.fft.fftn(np.roll(fs, [-(n // 2) for n in N], axis=tuple(range(len(N)))))
+    c = O.nudft_type2(co, f, -1).reshape(N)
+    assert np.abs(c - ref).max() <= 1e-11 * np.abs(ref).max()
+    # type 1 with +1 is the adjoint: on the full grid it is N^d times the inverse
+    back = O.nudft_type1(co, c.ravel(), N, +1)
+    assert np.abs(back - np.prod(N) * f).max() <= 1e-10 * np.abs(f).max() * np.prod(N)
+
+
+def test_direct_sums_adjoint_and_row_pairing():
+    co = rng.uniform(-np.pi, np.pi, (3, 40))
+    N = (3, 4, 5)
+    f, c = rnd(N), rnd(40)
+    lhs = np.vdot(O.nudft_type2(co, f, -1), c)
+    rhs = np.vdot(f, O.nudft_type1(co, c, N, +1))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    # a single mode at index n gives exp(-i (n - N//2) . x): row d pairs with dim d
+    f = np.zeros(N, dtype=complex)
+    f[2, 0, 4] = 1.0
+    k = np.array([2 - 1, 0 - 2, 4 - 2])
+    assert np.allclose(O.nudft_type2(co, f, -1), np.exp(-1j * (k @ co)), atol=1e-13)
+
+
+@pytest.mark.parametrize("tol", [1e-6, 1e-9, 1e-12])
+def test_es_nufft_matches_direct_sum(tol):
+    co = rng.uniform(-3 * np.pi, 3 * np.pi, (3, 300))
+    N = (10, 9, 8)
+    f, c = rnd(N), rnd(300)
+    e2 = O.rel_l2(O.nufft_type2(co, f, -1, tol), O.nudft_type2(co, f, -1))
+    e1 = O.rel_l2(O.nufft_type1(co, c, N, +1, tol), O.nudft_type1(co, c, N, +1))
+    assert e2 < 10 * tol and e1 < 10 * tol
+    # opposite sign too
+    assert O.rel_l2(O.nufft_type2(co, f, +1, tol), O.nudft_type2(co, f, +1)) < 10 * tol
+
+
+def test_es_nufft_2d_1d():
+    co = rng.uniform(-np.pi, np.pi, (2, 200))
+    f = rnd((12, 7))
+    assert O.rel_l2(O.nufft_type2(co, f, -1, 1e-10), O.nudft_type2(co, f, -1)) < 1e-9
+    co1 = rng.uniform(-np.pi, np.pi, (1, 100))
+    c = rnd(100)
+    assert O.rel_l2(O.nufft_type1(co1, c, (16,), 1, 1e-10), O.nudft_type1(co1, c, (16,), 1)) < 1e-9
+
+
+def test_golden_deterministic_3d_closed_form():
+    fx = json.load(open(GOLDEN))["deterministic_3d"]
+    co = np.asarray(fx["coords"], dtype=np.float64)
+    # f[0,0,0,1] = 2 -> k = (-2,-2,-1); f[0,2,1,3] = 3 -> k = (0,-1,1); type 2 with sign -1, /sqrt(64)
+    closed = (2 * np.exp(-1j * (np.array([-2, -2, -1]) @ co)) + 3 * np.exp(-1j * (np.array([0, -1, 1]) @ co))) / 8
+    assert np.allclose(cx(fx["type2"])[0], closed, atol=1e-12)
+    got = O.nufft_apply(co, fx["nmodes"], 2, cx(fx["input"]), exact=True)
+    assert np.allclose(got, cx(fx["type2"]), atol=1e-12)
+    back = O.nufft_apply(co, fx["nmodes"], 1, got, exact=True)
+    assert np.allclose(back, cx(fx["type1_of_type2"]), atol=1e-12)
+
+
+def test_golden_deterministic_1d_closed_form():
+    fx = json.load(open(GOLDEN))["deterministic_1d"]
+    co = np.asarray(fx["coords"], dtype=np.float64)
+    # sum over k=-4..3 of exp(+i k x) at x = -pi + 2 pi m / 8: 8 (-1)^m... only m = 4 (x = 0) is nonzero
+    out = cx(fx["type2"])[0]
+    expect = np.zeros(8, dtype=complex)
+    expect[4] = 8 / math.sqrt(8)
+    assert np.allclose(out, expect, atol=1e-5)  # float32 coordinates
+    got = O.nufft_apply(co, fx["nmodes"], 2, cx(fx["input"]), positive=True, exact=True)
+    assert np.allclose(got, cx(fx["type2"]), atol=1e-12)
+
+
+def test_golden_compare_normal_methods_toeplitz_scale():
+    fx = json.load(open(GOLDEN))["compare_normal_methods"]
+    normal, toep = cx(fx["normal"]), cx(fx["toeplitz"])
+    # the reference's Toeplitz operator is 2^-d times its explicit normal operator
+    assert np.allclose(toep, normal / 8, atol=1e-12 * np.abs(normal).max())
+    co = np.asarray(fx["coords"], dtype=np.float64)
+    assert np.allclose(O.nufft_normal_apply(co, fx["nmodes"], np.ones(fx["nmodes"], complex), exact=True), normal)
+
+
+def test_toeplitz_equals_explicit_normal_random():
+    co = rng.uniform(-np.pi, np.pi, (3, 150))
+    nm = [1, 6, 5, 4]
+    x = rnd(nm)
+    diag = O.toeplitz_diagonal(co, nm, exact=True)
+    t = O.toeplitz_apply(diag, x)
+    e = O.nufft_normal_apply(co, nm, x, exact=True)
+    assert O.rel_l2(t * 8, e) < 1e-12
+    # ES-built diagonal agrees with the exact one
+    diag2 = O.toeplitz_diagonal(co, nm, exact=False)
+    assert O.rel_l2(diag2, diag) < 1e-10
+
+
+def test_sense_normal_hermitian_psd_and_cartesian_identity():
+    N = (4, 5, 6)
+    co = rng.uniform(-np.pi, np.pi, (3, 200))
+    S = O.coil_maps(3, 6)[:, :4, :5, :6]
+    S = S / np.sqrt(np.sum(np.abs(S) ** 2, axis=0, keepdims=True))
+    x, y = rnd((1,) + N), rnd((1,) + N)
+    Ax = O.sense_normal_apply(co, x, S, [0, 1, 2], exact=True)
+    Ay = O.sense_normal_apply(co, y, S, [0, 1, 2], exact=True)
+    assert abs(np.vdot(y, Ax) - np.vdot(Ay, x)) < 1e-10 * abs(np.vdot(y, Ax))
+    assert np.vdot(x, Ax).real > 0
+    # Toeplitz form at SenseNormal scale equals the explicit form
+    diag = O.toeplitz_diagonal(co, [1, *N], exact=True)
+    assert O.rel_l2(O.sense_normal_toeplitz(diag, x, S, [0, 1, 2]), Ax) < 1e-12
+    # full Cartesian sampling: F^H F = N^d I and sum |S_c|^2 = 1 -> identity
+    coc = cartesian(N)
+    assert O.rel_l2(O.sense_normal_apply(coc, x, S, [0, 1, 2], exact=True), x) < 1e-11
+
+
+def test_sense_gradient_form():
+    N = (4, 4, 4)
+    co = rng.uniform(-np.pi, np.pi, (3, 100))
+    S = O.coil_maps(2, 4)
+    x = rnd((1,) + N)
+    y = rnd((2, 100))
+    w = rng.uniform(0.5, 1.5, 100)
+    g = O.sense_normal_apply(co, x, S, [0, 1], kdata=y, weights=w, exact=True)
+    # A^H W (A x - y) written out with the forward and adjoint operators
+    Ax = O.sense_apply(co, x, S, [0, 1], exact=True)
+    expect = O.sense_adjoint_apply(co, (Ax - y) * w[None], S, [0, 1], exact=True)
+    assert O.rel_l2(g, expect) < 1e-12
+
+
+def test_cg_matches_dense_solve_and_detects_indefinite():
+    n = 12
+    Q = rnd((n, n))
+    A = Q.conj().T @ Q + 0.5 * np.eye(n)
+    b = rnd(n)
+    x = O.conjugate_gradient(lambda v: A @ v, b, np.zeros(n, complex), iters=3 * n)
+    assert O.rel_l2(x, np.linalg.solve(A, b)) < 1e-8
+    with pytest.raises(O.NotPositiveDefinite):
+        O.conjugate_gradient(lambda v: -(A @ v), b, np.zeros(n, complex), iters=3)
+    # tol stops before any update when the initial residual is already below it
+    x0 = np.linalg.solve(A, b)
+    assert np.array_equal(O.conjugate_gradient(lambda v: A @ v, b, x0, iters=5, tol=1e-3), x0)
+
+
+def test_svt_closed_forms():
+    R, F, r = 300, 20, 5
+    U, _ = np.linalg.qr(rnd((R, r)))
+    V, _ = np.linalg.qr(rnd((F, r)))
+    s = np.array([10.0, 7.0, 5.0, 3.0, 1.0])
+    A = (U * s) @ V.conj().T
+    lam = 2.0
+    out = O.svt_soft(A, lam)
+    assert O.rel_l2(out, (U * np.maximum(s - lam, 0)) @ V.conj().T) < 1e-12
+    assert O.rel_l2(O.svt_hard(A, 2), (U[:, :2] * s[:2]) @ V[:, :2].conj().T) < 1e-12
+
+
+def test_llr_shift_sequence_is_libstdcxx():
+    for it, shape in [(1, (16, 16, 16)), (6, (16, 16, 16)), (20, (8, 4, 12)), (4, (1, 2, 3))]:
+        assert np.array_equal(O.llr_shifts(it, shape), O.llr_shifts_libstdcxx(it, shape))
+
+
+def test_llr_block_grid_skip_rule():
+    it = O.normal_blocks((128, 128, 128), (16, 16, 16), (16, 16, 16), 1)
+    assert len(it[0]) == 343 and max(b[0] for b in it[0]) == 96
+    assert len(O.normal_blocks((32, 32, 32), (16, 16, 16), (16, 16, 16), 1)[0]) == 1
+    with pytest.raises(RuntimeError):
+        O.normal_blocks((32, 32, 32), (16, 16, 16), (16, 16, 16), 0)
+
+
+def test_extract_insert_roundtrip_and_layout():
+    x = rnd((4, 2, 6, 6, 6))
+    A = O.extract_block(x, (1, 2, 0), (2, 3, 4))
+    assert A.shape == (2 * 2 * 3 * 4, 4)
+    # row (e, x, y, z) = (1, 1, 2, 3), column f = 3
+    assert A[((1 * 2 + 1) * 3 + 2) * 4 + 3, 3] == x[3, 1, 2, 4, 3]
+    y = x.copy()
+    O.insert_block(y, (1, 2, 0), (2, 3, 4), A)
+    assert np.array_equal(x, y)
+
+
+def test_normal_blocks_svt_sequential_semantics():
+    x = rnd((6, 1, 8, 8, 8))
+    out = O.normal_3d_blocks_svt(x, (4, 4, 4), (4, 4, 4), 1, 0.0, True)
+    # thresh 0 keeps everything; blocks touching the far edge are skipped either way
+    assert O.rel_l2(out, x) < 1e-12
+    out = O.normal_3d_blocks_svt(x, (4, 4, 4), (4, 4, 4), 1, 1e6, True)
+    assert np.abs(out[:, :, :4, :4, :4]).max() == 0.0
+    assert np.array_equal(out[:, :, 4:, :, :], x[:, :, 4:, :, :])
