@@ -17,6 +17,7 @@ namespace hasty_hip {
 
 namespace {
 thread_local std::string g_last_error;
+int g_opt_sense_fused = 1;
 
 template <typename Fn> int guarded(Fn&& fn)
 {
@@ -246,25 +247,48 @@ int hasty_toeplitz_apply(const void* in, void* out, const void* mul, const void*
     });
 }
 
-size_t hasty_sense_normal_workspace_size(const int64_t* nmodes3, int coil_batch)
+size_t hasty_sense_normal_workspace_size(const int64_t* nmodes3, int coil_batch, int problem_batch)
 {
     if (!nmodes3 || coil_batch < 1) return 0;
-    return sense_normal_workspace_bytes(nmodes3, coil_batch);
+    return sense_normal_workspace_bytes(nmodes3, coil_batch, problem_batch);
+}
+
+int hasty_sense_normal_prepare(const float* natural, float* prepared, int64_t nk, const int64_t* nmodes3, int* layout,
+                               void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(natural && prepared && nmodes3 && layout, "null argument");
+        for (int d = 0; d < 3; ++d) HASTY_REQUIRE(nmodes3[d] >= 1, "nmodes must be >= 1");
+        const int l = sense_kernel_layout(nmodes3, g_opt_sense_fused != 0);
+        HASTY_REQUIRE(l == 0 || prepared != natural, "the fused layout cannot be prepared in place");
+        sense_prepare_kernels(natural, prepared, nk, nmodes3, l, as_stream(stream));
+        *layout = l;
+    });
 }
 
 int hasty_sense_normal_toeplitz(const void* x, void* out, const void* smaps, const int32_t* coils, int ncoil,
-                                const float* kernels, const int32_t* kernel_index, const void* sub, int nproblems,
-                                const int64_t* nmodes3, void* workspace, size_t workspace_bytes, int coil_batch,
-                                void* stream)
+                                const float* kernels, int layout, const int32_t* kernel_index, const void* sub,
+                                int nproblems, const int64_t* nmodes3, void* workspace, size_t workspace_bytes,
+                                int coil_batch, void* stream)
 {
     return guarded([&] {
         HASTY_REQUIRE(x && out && smaps && coils && kernels && kernel_index && nmodes3 && workspace, "null argument");
         HASTY_REQUIRE(nproblems >= 1, "nproblems must be >= 1");
         for (int d = 0; d < 3; ++d) HASTY_REQUIRE(nmodes3[d] >= 1, "nmodes must be >= 1");
-        sense_normal_toeplitz((const float2*)x, (float2*)out, (const float2*)smaps, coils, ncoil, kernels, kernel_index,
-                              (const float2*)sub, nproblems, nmodes3, workspace, workspace_bytes, coil_batch,
-                              as_stream(stream));
+        sense_normal_toeplitz((const float2*)x, (float2*)out, (const float2*)smaps, coils, ncoil, kernels, layout,
+                              kernel_index, (const float2*)sub, nproblems, nmodes3, workspace, workspace_bytes,
+                              coil_batch, as_stream(stream));
     });
+}
+
+int hasty_set_option(const char* name, int64_t value)
+{
+    if (name && std::strcmp(name, "sense_fused") == 0) {
+        g_opt_sense_fused = (int)value;
+        return 0;
+    }
+    set_last_error(std::string("unknown option ") + (name ? name : "(null)"));
+    return 1;
 }
 
 size_t hasty_cg_workspace_size(int64_t nproblems, int64_t n)

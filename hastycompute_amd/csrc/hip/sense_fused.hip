@@ -1,0 +1,516 @@
+// Fused Toeplitz SENSE normal operator for N1 = N2 = 32*NB (NB = 1..4), any N0.
+//
+// For every problem p and coil c the operator is
+//   out[p] += conj(S_c) * crop( IFFT3( K[p] * FFT3( pad(S_c * x[p]) ) ) ),
+// a zero-padded (2N)^3 FFT pair around a real kernel. Three kernels move each coil's
+// oversampled grid through HBM once (one half-size write+read twice), instead of the
+// five strided passes of the generic path:
+//
+//  A  fwd_xy   one workgroup per (problem, z-plane): x[p] plane in registers; per coil
+//              multiply by S_c, then the padded 2-D FFT of the N x N plane to L x L,
+//              computed as P = L/32 cosets of the x-frequency: for coset s the 32
+//              frequencies k = P m + s are a 32-point DFT of the x-row folded modulo 32
+//              after the twiddle exp(-2 pi i s n / L) (one wavefront shuffle FFT per
+//              row), followed by the L-point y FFT of the 32 columns in LDS. Writes
+//              T[p][c][z][s][ky][m] (N0 planes of L*L, i.e. half the oversampled grid).
+//  B  z_conv   one workgroup per bundle of 32 (ky, kx) columns: loads the kernel
+//              column bundle once, then for each coil: z FFT (N0 -> L0 padded), times
+//              the real kernel, inverse z FFT, crop to N0, in place in T.
+//  C  inv_xy   one workgroup per (problem, z-plane): for each coil and coset, the
+//              L-point inverse y FFT (cropped to N1 rows) in LDS, a 32-point inverse
+//              wavefront FFT along m, the coset twiddle exp(+2 pi i s n / L), the
+//              conj(S_c) multiply and the coil sum, all accumulated in registers; the
+//              output plane is written once.
+// The kernel K is stored in the same permuted (s, ky, m) order as T.
+#include <vector>
+
+#include "common.hpp"
+#include "fft.hpp"
+
+namespace hasty_hip {
+
+namespace {
+
+constexpr int kW = 32;           // coset width (x frequencies per coset)
+constexpr int kXY = 1024;        // threads of the plane kernels
+constexpr int kZ = 1024;         // threads of the z kernel
+
+__device__ __forceinline__ int brev5(int b) { return (int)(__brev((unsigned)b) >> 27); }
+
+// exp(sign 2 pi i k / n) from a forward table of length n (tw[k] = exp(-2 pi i k / n)).
+template <int S> __device__ __forceinline__ float2 twid(const float2* tw, int k)
+{
+    float2 w = tw[k];
+    if (S > 0) w.y = -w.y;
+    return w;
+}
+
+__device__ __forceinline__ float2 shfl_xor2(float2 v, int m)
+{
+    return float2{__shfl_xor(v.x, m, 32), __shfl_xor(v.y, m, 32)};
+}
+
+// Forward 32-point DFT across the 32 lanes of a half-wave (lane b holds x[b]); radix-2
+// decimation in frequency; on return lane b holds X[brev5(b)]. tw32: exp(-2 pi i k/32).
+__device__ __forceinline__ float2 dft32_fwd(float2 v, int b, const float2* tw32)
+{
+#pragma unroll
+    for (int h = 16; h >= 1; h >>= 1) {
+        const float2 o = shfl_xor2(v, h);
+        if (b & h) {
+            // (x[b-h] - x[b]) * w_{2h}^{(b mod h)}, w_{2h} = exp(-2 pi i / 2h) = tw32[16/h]
+            const float2 d = csub(o, v);
+            v = cmul(d, tw32[(b & (h - 1)) * (16 / h)]);
+        } else {
+            v = cadd(v, o);
+        }
+    }
+    return v;
+}
+
+// Inverse (sign +1) 32-point DFT across lanes: lane b holds X[brev5(b)] on entry
+// (bit-reversed input), radix-2 decimation in time; on return lane b holds x[b].
+__device__ __forceinline__ float2 dft32_inv_bitrev_in(float2 v, int b, const float2* tw32)
+{
+#pragma unroll
+    for (int h = 1; h <= 16; h <<= 1) {
+        const float2 o = shfl_xor2(v, h);
+        // iterative radix-2 DIT on bit-reversed input: pair (g + j, g + j + h) of a 2h
+        // group, u = a[g+j], t = w a[g+j+h], outputs u + t and u - t
+        const int j = (b & (h - 1));
+        float2 w = tw32[j * (16 / h)];
+        w.y = -w.y;  // exp(+2 pi i j / 2h)
+        if (b & h) {
+            v = csub(o, cmul(w, v));
+        } else {
+            v = cadd(v, cmul(w, o));
+        }
+    }
+    return v;
+}
+
+template <int RAD, int S> __device__ __forceinline__ void dftR(float2* v);
+
+template <int S> __device__ __forceinline__ float2 mis(float2 z)
+{
+    if (S > 0) return float2{-z.y, z.x};
+    return float2{z.y, -z.x};
+}
+template <> __device__ __forceinline__ void dftR<2, -1>(float2* v) { float2 a = v[0]; v[0] = cadd(a, v[1]); v[1] = csub(a, v[1]); }
+template <> __device__ __forceinline__ void dftR<2, 1>(float2* v) { float2 a = v[0]; v[0] = cadd(a, v[1]); v[1] = csub(a, v[1]); }
+template <int S> __device__ __forceinline__ void dft4(float2* v)
+{
+    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]), t3 = mis<S>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+}
+template <> __device__ __forceinline__ void dftR<4, -1>(float2* v) { dft4<-1>(v); }
+template <> __device__ __forceinline__ void dftR<4, 1>(float2* v) { dft4<1>(v); }
+template <int S> __device__ __forceinline__ void dft8(float2* v)
+{
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    dft4<S>(e);
+    dft4<S>(o);
+    const float h = 0.70710678118654752440f;
+    float2 w1 = float2{h * (o[1].x - (float)S * o[1].y), h * (o[1].y + (float)S * o[1].x)};
+    float2 w2 = mis<S>(o[2]);
+    float2 w3 = float2{h * (-o[3].x - (float)S * o[3].y), h * (-o[3].y + (float)S * o[3].x)};
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], w1);
+    v[5] = csub(e[1], w1);
+    v[2] = cadd(e[2], w2);
+    v[6] = csub(e[2], w2);
+    v[3] = cadd(e[3], w3);
+    v[7] = csub(e[3], w3);
+}
+template <> __device__ __forceinline__ void dftR<8, -1>(float2* v) { dft8<-1>(v); }
+template <> __device__ __forceinline__ void dftR<8, 1>(float2* v) { dft8<1>(v); }
+template <int S> __device__ __forceinline__ void dft3(float2* v)
+{
+    const float c = 0.86602540378443864676f;
+    float2 t = cadd(v[1], v[2]);
+    float2 d = mis<S>(csub(v[1], v[2]));
+    float2 m = float2{v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y};
+    v[0] = cadd(v[0], t);
+    v[1] = float2{m.x + c * d.x, m.y + c * d.y};
+    v[2] = float2{m.x - c * d.x, m.y - c * d.y};
+}
+template <> __device__ __forceinline__ void dftR<3, -1>(float2* v) { dft3<-1>(v); }
+template <> __device__ __forceinline__ void dftR<3, 1>(float2* v) { dft3<1>(v); }
+template <int S> __device__ __forceinline__ void dft5(float2* v)
+{
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+    float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    float2 x0 = v[0];
+    v[0] = cadd(x0, cadd(a1, a2));
+    float2 p1 = float2{x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y};
+    float2 p2 = float2{x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y};
+    float2 q1 = mis<S>(float2{s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y});
+    float2 q2 = mis<S>(float2{s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y});
+    v[1] = cadd(p1, q1);
+    v[4] = csub(p1, q1);
+    v[2] = cadd(p2, q2);
+    v[3] = csub(p2, q2);
+}
+template <> __device__ __forceinline__ void dftR<5, -1>(float2* v) { dft5<-1>(v); }
+template <> __device__ __forceinline__ void dftR<5, 1>(float2* v) { dft5<1>(v); }
+
+// One Stockham pass over NCOL interleaved columns of length L held in LDS as
+// [row][NCOL] (column c, element e at src[e*NCOL + c]); src and dst distinct.
+// `zero_from` (first pass only): input rows >= zero_from are known zeros.
+template <int RAD, int S, int NCOL, int NT>
+__device__ __forceinline__ void col_pass(const float2* __restrict__ src, float2* __restrict__ dst, int L, int Ns,
+                                         const float2* __restrict__ twL, int zero_from)
+{
+    const int nbf = L / RAD;
+    const int twstep = L / (Ns * RAD);
+    for (int bf = threadIdx.x; bf < nbf * NCOL; bf += NT) {
+        const int c = bf % NCOL;
+        const int j = bf / NCOL;
+        const int k = j % Ns;
+        float2 v[RAD];
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) {
+            const int row = j + r * nbf;
+            float2 x = row < zero_from ? src[row * NCOL + c] : float2{0.f, 0.f};
+            if (r > 0) x = cmul(x, twid<S>(twL, r * k * twstep));
+            v[r] = x;
+        }
+        dftR<RAD, S>(v);
+        const int o = (j / Ns) * Ns * RAD + k;
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) dst[(o + r * Ns) * NCOL + c] = v[r];
+    }
+}
+
+// Full L-point FFT of NCOL columns, ping-pong between bufA and bufB; returns the buffer
+// holding the result. Radices are encoded 4 bits each in `code`.
+template <int S, int NCOL, int NT>
+__device__ __forceinline__ float2* col_fft(float2* bufA, float2* bufB, int L, uint32_t code, int npass, const float2* twL, int zero_from)
+{
+    float2* src = bufA;
+    float2* dst = bufB;
+    int Ns = 1;
+    for (int p = 0; p < npass; ++p) {
+        const int rad = (code >> (4 * p)) & 15;
+        const int zf = p == 0 ? zero_from : L;
+        switch (rad) {
+        case 2: col_pass<2, S, NCOL, NT>(src, dst, L, Ns, twL, zf); break;
+        case 3: col_pass<3, S, NCOL, NT>(src, dst, L, Ns, twL, zf); break;
+        case 4: col_pass<4, S, NCOL, NT>(src, dst, L, Ns, twL, zf); break;
+        case 5: col_pass<5, S, NCOL, NT>(src, dst, L, Ns, twL, zf); break;
+        default: col_pass<8, S, NCOL, NT>(src, dst, L, Ns, twL, zf); break;
+        }
+        __syncthreads();
+        float2* t = src;
+        src = dst;
+        dst = t;
+        Ns *= rad;
+    }
+    return src;
+}
+
+struct FusedArgs {
+    const float2* x;       // [P][N0][N][N]
+    float2* out;           // [P][N0][N][N]
+    const float2* sub;     // optional [P][N0][N][N]
+    const float2* smaps;   // [Cmap][N0][N][N]
+    const int* coils;      // [ncoil] device
+    const float* kernels;  // [nk][L0][L*L] permuted
+    float2* T;             // [pchunk][ccount][N0][L*L]
+    const float2* twL;     // exp(-2 pi i k / L), L entries
+    const float2* twL0;    // exp(-2 pi i k / L0)
+    const float2* tw32;    // exp(-2 pi i k / 32)
+    int N0, L0;
+    int p0;                // first problem of the chunk
+    int c0, ccount;        // coil range of this launch
+    int accumulate;        // out += (later coil chunks)
+    uint32_t codeL, codeL0;
+    int npassL, npassL0;
+};
+
+template <int NB> __global__ void __launch_bounds__(kXY) fwd_xy_kernel(FusedArgs a)
+{
+    constexpr int N = 32 * NB, L = 64 * NB, P = 2 * NB;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* bufA = reinterpret_cast<float2*>(smem_raw);
+    float2* bufB = bufA + L * kW;
+    float2* twL = bufB + L * kW;
+    float2* tw32 = twL + L;
+    const int t = threadIdx.x;
+    const int b = t & 31;
+    const int yl = t >> 5;
+    const int z = blockIdx.x;
+    const int pl = blockIdx.y;
+    const int p = a.p0 + pl;
+    for (int i = t; i < L; i += kXY) twL[i] = a.twL[i];
+    if (t < 32) tw32[t] = a.tw32[t];
+    const int64_t plane = (int64_t)N * N;
+    // the x plane is re-read per coil (from L2 after the first coil) to keep registers free
+    const float2* xp = a.x + ((int64_t)p * a.N0 + z) * plane;
+    __syncthreads();
+
+    for (int ci = 0; ci < a.ccount; ++ci) {
+        const int coil = a.coils[a.c0 + ci];
+        const float2* sp = a.smaps + ((int64_t)coil * a.N0 + z) * plane;
+        float2 v[NB][NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int64_t idx = (int64_t)(yl + 32 * k) * N + b + 32 * j;
+                v[k][j] = cmul(xp[idx], sp[idx]);
+            }
+        float2* Tp = a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L;
+        for (int s = 0; s < P; ++s) {
+            const float2 w1 = twL[(s * b) % L];
+            const float2 w32 = twL[(s * 32) % L];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                float2 acc = v[k][NB - 1];
+#pragma unroll
+                for (int j = NB - 2; j >= 0; --j) acc = cadd(cmul(acc, w32), v[k][j]);
+                acc = cmul(acc, w1);
+                acc = dft32_fwd(acc, b, tw32);
+                bufA[(yl + 32 * k) * kW + brev5(b)] = acc;
+            }
+            __syncthreads();
+            const float2* res = col_fft<-1, kW, kXY>(bufA, bufB, L, a.codeL, a.npassL, twL, N);
+            float2* dst = Tp + (int64_t)s * L * kW;
+            for (int e = t; e < L * kW; e += kXY) dst[e] = res[e];
+            __syncthreads();
+        }
+    }
+}
+
+template <int NB> __global__ void __launch_bounds__(kXY) inv_xy_kernel(FusedArgs a)
+{
+    constexpr int N = 32 * NB, L = 64 * NB, P = 2 * NB;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* bufA = reinterpret_cast<float2*>(smem_raw);
+    float2* bufB = bufA + L * kW;
+    float2* twL = bufB + L * kW;
+    float2* tw32 = twL + L;
+    const int t = threadIdx.x;
+    const int b = t & 31;
+    const int yl = t >> 5;
+    const int z = blockIdx.x;
+    const int pl = blockIdx.y;
+    const int p = a.p0 + pl;
+    for (int i = t; i < L; i += kXY) twL[i] = a.twL[i];
+    if (t < 32) tw32[t] = a.tw32[t];
+    const int64_t plane = (int64_t)N * N;
+    float2 acc[NB][NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[k][j] = float2{0.f, 0.f};
+    __syncthreads();
+
+    for (int ci = 0; ci < a.ccount; ++ci) {
+        const int coil = a.coils[a.c0 + ci];
+        const float2* sp = a.smaps + ((int64_t)coil * a.N0 + z) * plane;
+        float2 sv[NB][NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const float2 q = sp[(yl + 32 * k) * N + b + 32 * j];
+                sv[k][j] = float2{q.x, -q.y};
+            }
+        const float2* Tp = a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L;
+        for (int s = 0; s < P; ++s) {
+            const float2* src = Tp + (int64_t)s * L * kW;
+            for (int e = t; e < L * kW; e += kXY) bufA[e] = src[e];
+            __syncthreads();
+            const float2* res = col_fft<1, kW, kXY>(bufA, bufB, L, a.codeL, a.npassL, twL, L);
+            // rows y < N: inverse 32-point DFT over m (bit-reversed load), coset twiddle,
+            // conj(S) multiply, accumulate
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int y = yl + 32 * k;
+                float2 u = res[y * kW + brev5(b)];
+                u = dft32_inv_bitrev_in(u, b, tw32);
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    const int n = b + 32 * j;
+                    const float2 w = twid<1>(twL, (s * n) % L);
+                    acc[k][j] = cadd(acc[k][j], cmul(sv[k][j], cmul(w, u)));
+                }
+            }
+            __syncthreads();
+        }
+    }
+    float2* op = a.out + ((int64_t)p * a.N0 + z) * plane;
+    const float2* sb = a.sub ? a.sub + ((int64_t)p * a.N0 + z) * plane : nullptr;
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int64_t idx = (int64_t)(yl + 32 * k) * N + b + 32 * j;
+            float2 r = acc[k][j];
+            if (a.accumulate) r = cadd(r, op[idx]);
+            else if (sb) r = csub(r, sb[idx]);
+            op[idx] = r;
+        }
+}
+
+// z-axis: FFT (N0 -> L0), multiply by the real kernel, inverse FFT, crop; in place.
+// One launch per problem of the chunk (pl), with that problem's kernel K.
+__global__ void __launch_bounds__(kZ) z_conv_kernel(FusedArgs a, int LL, int pl, const float* __restrict__ K)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int L0 = a.L0;
+    float2* bufA = reinterpret_cast<float2*>(smem_raw);
+    float2* bufB = bufA + L0 * kW;
+    float2* twz = bufB + L0 * kW;
+    const int t = threadIdx.x;
+    const int pos0 = blockIdx.x * kW;  // 32 consecutive (s, ky, m) positions
+    for (int i = t; i < L0; i += kZ) twz[i] = a.twL0[i];
+    // kernel values of the elements this thread multiplies (fixed across coils)
+    constexpr int kMaxPer = 16;  // L0 * 32 / 1024 <= 16 (L0 <= 512)
+    float kv[kMaxPer];
+    const int per = (L0 * kW) / kZ;
+#pragma unroll
+    for (int i = 0; i < kMaxPer; ++i) {
+        if (i < per) {
+            const int e = t + kZ * i;
+            kv[i] = K[(int64_t)(e / kW) * LL + pos0 + (e % kW)];
+        }
+    }
+    __syncthreads();
+    for (int ci = 0; ci < a.ccount; ++ci) {
+        float2* Tc = a.T + ((int64_t)pl * a.ccount + ci) * a.N0 * (int64_t)LL + pos0;
+        for (int e = t; e < a.N0 * kW; e += kZ) bufA[e] = Tc[(int64_t)(e / kW) * LL + (e % kW)];
+        __syncthreads();
+        float2* res = col_fft<-1, kW, kZ>(bufA, bufB, L0, a.codeL0, a.npassL0, twz, a.N0);
+        float2* other = res == bufA ? bufB : bufA;
+#pragma unroll
+        for (int i = 0; i < kMaxPer; ++i) {
+            if (i < per) {
+                const int e = t + kZ * i;
+                res[e] = cscale(res[e], kv[i]);
+            }
+        }
+        __syncthreads();
+        const float2* res2 = col_fft<1, kW, kZ>(res, other, L0, a.codeL0, a.npassL0, twz, L0);
+        for (int e = t; e < a.N0 * kW; e += kZ) Tc[(int64_t)(e / kW) * LL + (e % kW)] = res2[e];
+        __syncthreads();
+    }
+}
+
+uint32_t radix_code(int L, int* npass)
+{
+    uint32_t code = 0;
+    int n = 0, m = L;
+    for (int r : {8, 4, 2, 3, 5}) {
+        while (m % r == 0) {
+            code |= (uint32_t)r << (4 * n);
+            ++n;
+            m /= r;
+        }
+    }
+    if (m != 1 || n > 8) throw HastyError(kUnsupported, "fused FFT length not 2,3,5-smooth");
+    *npass = n;
+    return code;
+}
+
+// Permute a natural-order kernel [L0][L][L] (kx fastest) into the fused (s, ky, m) order.
+__global__ void permute_kernel_kernel(const float* in, float* out, int L0, int L)
+{
+    const int64_t LL = (int64_t)L * L;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)L0 * LL) return;
+    const int64_t kz = i / LL;
+    const int64_t r = i % LL;
+    const int ky = (int)(r / L), kx = (int)(r % L);
+    const int P = L / kW;
+    const int s = kx % P, m = kx / P;
+    out[kz * LL + ((int64_t)s * L + ky) * kW + m] = in[i];
+}
+
+}  // namespace
+
+bool sense_fused_supported(const int64_t* n3)
+{
+    return n3[1] == n3[2] && n3[1] % 32 == 0 && n3[1] >= 32 && n3[1] <= 128 && fft_size_supported((int)(2 * n3[0])) &&
+           2 * n3[0] <= 512;
+}
+
+void sense_fused_permute_kernel(const float* natural, float* permuted, const int64_t* n3, hipStream_t s)
+{
+    const int L0 = (int)(2 * n3[0]), L = (int)(2 * n3[1]);
+    const int64_t n = (int64_t)L0 * L * L;
+    hipLaunchKernelGGL(permute_kernel_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, natural, permuted, L0,
+                       L);
+    check_launch("permute_kernel_kernel");
+}
+
+size_t sense_fused_workspace_bytes(const int64_t* n3, int pchunk, int ccount)
+{
+    const int64_t L = 2 * n3[1];
+    return sizeof(float2) * (size_t)pchunk * ccount * n3[0] * L * L;
+}
+
+// kernels: permuted layout (sense_fused_permute_kernel); kidx: host int [P].
+void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const int* coils_dev, int ncoil,
+                       const float* kernels, const int* kidx, const float2* sub, int P, const int64_t* n3,
+                       void* ws, size_t ws_bytes, int pchunk, int cchunk, hipStream_t s)
+{
+    HASTY_REQUIRE(sense_fused_supported(n3), "fused SENSE normal operator needs N1 == N2 in {32, 64, 96, 128}");
+    const int N0 = (int)n3[0], N = (int)n3[1];
+    const int NB = N / 32;
+    const int L0 = 2 * N0, L = 2 * N;
+    pchunk = std::max(1, std::min(std::min(pchunk, P), 64));
+    cchunk = std::max(1, std::min(cchunk, ncoil));
+    HASTY_REQUIRE(ws_bytes >= sense_fused_workspace_bytes(n3, pchunk, cchunk), "fused workspace too small");
+    FusedArgs a;
+    a.x = x;
+    a.out = out;
+    a.sub = sub;
+    a.smaps = smaps;
+    a.coils = coils_dev;
+    a.kernels = kernels;
+    a.T = reinterpret_cast<float2*>(ws);
+    a.twL = twiddles<float>(L);
+    a.twL0 = twiddles<float>(L0);
+    a.tw32 = twiddles<float>(32);
+    a.N0 = N0;
+    a.L0 = L0;
+    a.codeL = radix_code(L, &a.npassL);
+    a.codeL0 = radix_code(L0, &a.npassL0);
+    const size_t smem_xy = sizeof(float2) * (2 * (size_t)L * kW + L + 32);
+    const size_t smem_z = sizeof(float2) * (2 * (size_t)L0 * kW + L0);
+    auto fwd = NB == 1 ? fwd_xy_kernel<1> : NB == 2 ? fwd_xy_kernel<2> : NB == 3 ? fwd_xy_kernel<3> : fwd_xy_kernel<4>;
+    auto inv = NB == 1 ? inv_xy_kernel<1> : NB == 2 ? inv_xy_kernel<2> : NB == 3 ? inv_xy_kernel<3> : inv_xy_kernel<4>;
+    allow_lds(fwd, smem_xy);
+    allow_lds(inv, smem_xy);
+    allow_lds(z_conv_kernel, smem_z);
+    for (int p0 = 0; p0 < P; p0 += pchunk) {
+        const int np = std::min(pchunk, P - p0);
+        a.p0 = p0;
+        for (int c0 = 0; c0 < ncoil; c0 += cchunk) {
+            a.c0 = c0;
+            a.ccount = std::min(cchunk, ncoil - c0);
+            a.accumulate = c0 > 0 ? 1 : 0;
+            hipLaunchKernelGGL(fwd, dim3(N0, np), dim3(kXY), smem_xy, s, a);
+            check_launch("fwd_xy_kernel");
+            for (int pl = 0; pl < np; ++pl) {
+                const float* K = kernels + (int64_t)kidx[p0 + pl] * L0 * (int64_t)L * L;
+                hipLaunchKernelGGL(z_conv_kernel, dim3((unsigned)(L * L / kW)), dim3(kZ), smem_z, s, a, L * L, pl, K);
+                check_launch("z_conv_kernel");
+            }
+            hipLaunchKernelGGL(inv, dim3(N0, np), dim3(kXY), smem_xy, s, a);
+            check_launch("inv_xy_kernel");
+        }
+    }
+}
+
+}  // namespace hasty_hip

@@ -275,21 +275,48 @@ void toeplitz_apply_ref(const float2* in, float2* out, const float2* mul, const 
     (void)nvox;
 }
 
-size_t sense_normal_workspace_bytes(const int64_t* n3, int coil_batch)
+int sense_kernel_layout(const int64_t* n3, bool allow_fused) { return allow_fused && sense_fused_supported(n3) ? 1 : 0; }
+
+void sense_prepare_kernels(const float* natural, float* prepared, int64_t nk, const int64_t* n3, int layout,
+                           hipStream_t s)
+{
+    const int64_t kvox = 8 * n3[0] * n3[1] * n3[2];
+    if (layout == 1) {
+        HASTY_REQUIRE(sense_fused_supported(n3), "fused kernel layout not available for these sizes");
+        for (int64_t k = 0; k < nk; ++k) sense_fused_permute_kernel(natural + k * kvox, prepared + k * kvox, n3, s);
+    } else {
+        HASTY_REQUIRE(layout == 0, "unknown kernel layout");
+        if (prepared != natural)
+            HASTY_HIP_CHECK(hipMemcpyAsync(prepared, natural, sizeof(float) * kvox * nk, hipMemcpyDeviceToDevice, s));
+    }
+}
+
+size_t sense_normal_workspace_bytes(const int64_t* n3, int coil_batch, int problem_batch)
 {
     const int64_t N0 = n3[0], N1 = n3[1], N2 = n3[2];
     const int64_t L1 = 2 * N1, L2 = 2 * N2;
     const int64_t t0 = N0 * N1 * L2;
     const int64_t t1 = N0 * L1 * L2;
-    return sizeof(float2) * (size_t)coil_batch * (size_t)(t0 + t1);
+    size_t generic = sizeof(float2) * (size_t)coil_batch * (size_t)(t0 + t1);
+    size_t fused = sense_fused_supported(n3) ? sense_fused_workspace_bytes(n3, std::max(1, problem_batch), coil_batch) : 0;
+    return std::max(generic, fused);
 }
 
 void sense_normal_toeplitz(const float2* x, float2* out, const float2* smaps, const int* coils_dev, int ncoil,
-                           const float* kernels, const int* kidx, const float2* sub, int P, const int64_t* n3,
-                           void* workspace, size_t ws_bytes, int coil_batch, hipStream_t s)
+                           const float* kernels, int layout, const int* kidx, const float2* sub, int P,
+                           const int64_t* n3, void* workspace, size_t ws_bytes, int coil_batch, hipStream_t s)
 {
     HASTY_REQUIRE(ncoil >= 1 && coil_batch >= 1, "need at least one coil");
-    HASTY_REQUIRE(ws_bytes >= sense_normal_workspace_bytes(n3, std::min(coil_batch, ncoil)),
+    if (layout == 1) {
+        const int cb = std::min(coil_batch, ncoil);
+        const size_t per = sense_fused_workspace_bytes(n3, 1, cb);
+        const int pchunk = (int)std::max<size_t>(1, std::min<size_t>(64, ws_bytes / per));
+        sense_fused_apply(x, out, smaps, coils_dev, ncoil, kernels, kidx, sub, P, n3, workspace, ws_bytes, pchunk, cb,
+                          s);
+        return;
+    }
+    HASTY_REQUIRE(layout == 0, "unknown kernel layout");
+    HASTY_REQUIRE(ws_bytes >= sense_normal_workspace_bytes(n3, std::min(coil_batch, ncoil), 0),
                   "sense normal workspace too small");
     const int cb = std::min(coil_batch, ncoil);
     const int64_t N0 = n3[0], N1 = n3[1], N2 = n3[2];

@@ -30,9 +30,22 @@ void toeplitz_apply_ref(const float2* in, float2* out, const float2* mul, const 
 // x, out, sub: [P][N0][N1][N2]; smaps: [C][N0][N1][N2]; kernels: [nk][2N0][2N1][2N2] real.
 // coils: device int array of ncoil coil indices. Workspace from
 // sense_normal_workspace_bytes.
-size_t sense_normal_workspace_bytes(const int64_t* n3, int coil_batch);
+// Kernel layouts: 0 natural [kz][ky][kx] (generic five-pass path), 1 the fused path's
+// permuted [kz][s][ky][m] order (kx = P m + s, P = 2 N2 / 32).
+int sense_kernel_layout(const int64_t* n3, bool allow_fused);
+void sense_prepare_kernels(const float* natural, float* prepared, int64_t nk, const int64_t* n3, int layout,
+                           hipStream_t s);
+size_t sense_normal_workspace_bytes(const int64_t* n3, int coil_batch, int problem_batch);
 void sense_normal_toeplitz(const float2* x, float2* out, const float2* smaps, const int* coils_dev, int ncoil,
-                           const float* kernels, const int* kidx_host, const float2* sub, int P, const int64_t* n3,
-                           void* workspace, size_t ws_bytes, int coil_batch, hipStream_t s);
+                           const float* kernels, int layout, const int* kidx_host, const float2* sub, int P,
+                           const int64_t* n3, void* workspace, size_t ws_bytes, int coil_batch, hipStream_t s);
+
+// sense_fused.hip
+bool sense_fused_supported(const int64_t* n3);
+void sense_fused_permute_kernel(const float* natural, float* permuted, const int64_t* n3, hipStream_t s);
+size_t sense_fused_workspace_bytes(const int64_t* n3, int pchunk, int ccount);
+void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const int* coils_dev, int ncoil,
+                       const float* kernels, const int* kidx, const float2* sub, int P, const int64_t* n3, void* ws,
+                       size_t ws_bytes, int pchunk, int cchunk, hipStream_t s);
 
 }  // namespace hasty_hip

@@ -137,20 +137,31 @@ class SenseNormalToeplitz:
     out[p] = sum_c conj(S_c) T_{k(p)}(S_c x[p]) - sub[p]."""
 
     def __init__(self, smaps: torch.Tensor, kernels: torch.Tensor, coils: Optional[Sequence[int]] = None,
-                 coil_batch: int = 4):
+                 coil_batch: int = 4, problem_batch: int = 1, fused: Optional[bool] = None):
         _cuda(smaps, "smaps")
         _cuda(kernels, "kernels")
         _require(smaps.dtype == torch.complex64 and smaps.dim() == 4, "smaps must be complex64 [C, N0, N1, N2]")
         if kernels.dim() == 3:
             kernels = kernels.unsqueeze(0)
-        self.smaps, self.kernels = smaps, kernels
+        self.smaps = smaps
         self.n3 = list(smaps.shape[1:])
         _require(list(kernels.shape[1:]) == [2 * n for n in self.n3], "kernel must be [nk, 2N0, 2N1, 2N2]")
         coils = list(range(smaps.shape[0])) if coils is None else [int(c) for c in coils]
         self.coils = torch.tensor(coils, dtype=torch.int32, device=smaps.device)
         self.ncoil = len(coils)
         self.coil_batch = max(1, min(coil_batch, self.ncoil))
-        ws = C.lib().hasty_sense_normal_workspace_size(C.i64arr(self.n3), self.coil_batch)
+        if fused is not None:
+            C.call("hasty_set_option", b"sense_fused", int(bool(fused)))
+        self.kernels = torch.empty_like(kernels)
+        layout = ctypes.c_int()
+        try:
+            C.call("hasty_sense_normal_prepare", kernels.data_ptr(), self.kernels.data_ptr(), kernels.shape[0],
+                   C.i64arr(self.n3), ctypes.byref(layout), C.stream_ptr())
+        finally:
+            if fused is not None:
+                C.call("hasty_set_option", b"sense_fused", 1)
+        self.layout = layout.value
+        ws = C.lib().hasty_sense_normal_workspace_size(C.i64arr(self.n3), self.coil_batch, max(1, problem_batch))
         self.workspace = torch.empty(ws, dtype=torch.uint8, device=smaps.device)
 
     def apply(self, x: torch.Tensor, out: Optional[torch.Tensor] = None, kernel_index: Optional[Sequence[int]] = None,
@@ -161,8 +172,8 @@ class SenseNormalToeplitz:
             out = torch.empty_like(x)
         kidx = [0] * P if kernel_index is None else list(kernel_index)
         C.call("hasty_sense_normal_toeplitz", x.data_ptr(), out.data_ptr(), self.smaps.data_ptr(),
-               self.coils.data_ptr(), self.ncoil, self.kernels.data_ptr(), C.i32arr(kidx), C.ptr(sub), P,
-               C.i64arr(self.n3), self.workspace.data_ptr(), self.workspace.numel(), self.coil_batch,
+               self.coils.data_ptr(), self.ncoil, self.kernels.data_ptr(), self.layout, C.i32arr(kidx), C.ptr(sub),
+               P, C.i64arr(self.n3), self.workspace.data_ptr(), self.workspace.numel(), self.coil_batch,
                C.stream_ptr())
         return out
 

@@ -85,13 +85,25 @@ int hasty_toeplitz_apply(const void* in, void* out, const void* mul, const void*
  * with N1 N2 through the problem's Toeplitz kernel (built with weights w).
  * x, out, sub: [P][N0][N1][N2] complex64 (sub may be NULL); smaps [C][N0][N1][N2];
  * coils: DEVICE int32 array of ncoil indices into smaps; kernels: float32
- * [nk][2N0][2N1][2N2]; kernel_index: HOST int32 [P], kernel of each problem.
+ * [nk][2N0][2N1][2N2] as written by hasty_sense_normal_prepare with `layout`;
+ * kernel_index: HOST int32 [P], kernel of each problem. coil_batch coils share one
+ * oversampled-grid workspace; problem_batch problems go through each launch.
+ * Layout 1 (N1 == N2 in {32, 64, 96, 128}) runs the fused three-kernel path; layout 0
+ * the generic five-pass path.
  * ------------------------------------------------------------------------------- */
-size_t hasty_sense_normal_workspace_size(const int64_t* nmodes3, int coil_batch);
+size_t hasty_sense_normal_workspace_size(const int64_t* nmodes3, int coil_batch, int problem_batch);
+/* Convert nk kernels from hasty_toeplitz_build's natural layout into the operator's
+ * layout (*layout receives 0 or 1; prepared may equal natural only for layout 0). */
+int hasty_sense_normal_prepare(const float* natural, float* prepared, int64_t nk, const int64_t* nmodes3,
+                               int* layout, void* stream);
 int hasty_sense_normal_toeplitz(const void* x, void* out, const void* smaps, const int32_t* coils, int ncoil,
-                                const float* kernels, const int32_t* kernel_index, const void* sub, int nproblems,
-                                const int64_t* nmodes3, void* workspace, size_t workspace_bytes, int coil_batch,
-                                void* stream);
+                                const float* kernels, int layout, const int32_t* kernel_index, const void* sub,
+                                int nproblems, const int64_t* nmodes3, void* workspace, size_t workspace_bytes,
+                                int coil_batch, void* stream);
+
+/* Library options: "sense_fused" (default 1) lets hasty_sense_normal_prepare choose the
+ * fused layout. Returns 0, or 1 for an unknown name. */
+int hasty_set_option(const char* name, int64_t value);
 
 /* ---------------------------------------------------------------------------------
  * Device-side batched conjugate gradient, the loop body of ConjugateGradient::run

@@ -291,12 +291,16 @@ def toeplitz_diagonal(coords: np.ndarray, nmodes: Sequence[int], weights: Option
     return np.fft.fftn(np.fft.ifftshift(p, axes=axes), axes=axes)
 
 
-def toeplitz_apply(diag: np.ndarray, x: np.ndarray) -> np.ndarray:
+def toeplitz_apply(diag: np.ndarray, x: np.ndarray, workers: Optional[int] = None) -> np.ndarray:
     """CUDANormalNufftToeplitz::apply (nufft.cpp:1163-1170): fftn(x, s=2N) (zero pad at
     the end of each dim), multiply by diag, ifftn ('backward', 1/prod(2N)), crop [:N]."""
     axes = tuple(range(1, x.ndim))
     s = [2 * n for n in x.shape[1:]]
-    y = np.fft.ifftn(np.fft.fftn(x, s=s, axes=axes) * diag, axes=axes)
+    if workers:
+        import scipy.fft as sfft
+        y = sfft.ifftn(sfft.fftn(x, s=s, axes=axes, workers=workers) * diag, axes=axes, workers=workers)
+    else:
+        y = np.fft.ifftn(np.fft.fftn(x, s=s, axes=axes) * diag, axes=axes)
     return y[(slice(None),) + tuple(slice(0, n) for n in x.shape[1:])]
 
 
@@ -341,14 +345,14 @@ def sense_normal_apply(coords, x, smaps, coils, kdata=None, weights=None, exact=
     return out
 
 
-def sense_normal_toeplitz(diag, x, smaps, coils):
+def sense_normal_toeplitz(diag, x, smaps, coils, workers: Optional[int] = None):
     """The Toeplitz-embedded SENSE normal operator at the scale of sense_normal_apply:
     sum_c conj(S_c) * 2^d * toeplitz_apply(diag, S_c x). The 2^d undoes the doubled-grid
     scaling of the reference's PSF (nufft.cpp:932-971 vs 862-880)."""
     d = x.ndim - 1
     out = np.zeros_like(x, dtype=np.complex128)
     for c in coils:
-        out += np.conj(smaps[c])[None] * (2 ** d) * toeplitz_apply(diag, x * smaps[c][None])
+        out += np.conj(smaps[c])[None] * (2 ** d) * toeplitz_apply(diag, x * smaps[c][None], workers)
     return out
 
 
