@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--spokes-min", type=int, default=800)
     ap.add_argument("--spokes-max", type=int, default=1500)
     ap.add_argument("--coil-batch", type=int, default=16)
+    ap.add_argument("--problem-batch", type=int, default=4)
+    ap.add_argument("--generic", action="store_true", help="use the generic five-pass path")
     ap.add_argument("--cg-iters", type=int, default=15)
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -122,7 +124,8 @@ def main():
     torch.cuda.set_device(0)
     K, smaps, x, kidx, npts = make_workload(args, torch, native)
     P = x.shape[0]
-    op = native.SenseNormalToeplitz(smaps, K, coil_batch=args.coil_batch)
+    op = native.SenseNormalToeplitz(smaps, K, coil_batch=args.coil_batch, problem_batch=args.problem_batch,
+                                    fused=False if args.generic else None)
     out = torch.empty_like(x)
     stream = torch.cuda.current_stream()
 
@@ -175,6 +178,7 @@ def main():
         "config": {"workload": "4D-flow batched normal op: 5 enc x 20 frames = 100 problems, 128^3, 16 coils, "
                                "grid 256^3, 800-1500 spokes x 128 per frame",
                    "problems": P, "n": args.n, "coils": args.coils, "coil_batch": args.coil_batch,
+                   "problem_batch": args.problem_batch, "path": "fused" if op.layout == 1 else "generic",
                    "points_per_frame_mean": float(sum(npts) / len(npts))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

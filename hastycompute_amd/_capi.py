@@ -39,8 +39,8 @@ _SIGS = {
     "hasty_toeplitz_apply": ([c_vp, c_vp, c_vp, c_vp, c_int, c_int, P_i64, c_int, c_vp, c_vp, c_vp], c_int),
     "hasty_sense_normal_workspace_size": ([P_i64, c_int, c_int], c_sz),
     "hasty_sense_normal_prepare": ([c_vp, c_vp, c_i64, P_i64, ctypes.POINTER(c_int), c_vp], c_int),
-    "hasty_sense_normal_toeplitz": ([c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_int, P_i32, c_vp, c_int, P_i64, c_vp,
-                                     c_sz, c_int, c_vp], c_int),
+    "hasty_sense_normal_toeplitz": ([c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_int, c_int, P_i32, c_vp, c_int, P_i64,
+                                     c_vp, c_sz, c_int, c_vp], c_int),
     "hasty_set_option": ([ctypes.c_char_p, c_i64], c_int),
     "hasty_cg_workspace_size": ([c_i64, c_i64], c_sz),
     "hasty_cg_init": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),

@@ -267,13 +267,15 @@ int hasty_sense_normal_prepare(const float* natural, float* prepared, int64_t nk
 }
 
 int hasty_sense_normal_toeplitz(const void* x, void* out, const void* smaps, const int32_t* coils, int ncoil,
-                                const float* kernels, int layout, const int32_t* kernel_index, const void* sub,
-                                int nproblems, const int64_t* nmodes3, void* workspace, size_t workspace_bytes,
-                                int coil_batch, void* stream)
+                                const float* kernels, int nkernels, int layout, const int32_t* kernel_index,
+                                const void* sub, int nproblems, const int64_t* nmodes3, void* workspace,
+                                size_t workspace_bytes, int coil_batch, void* stream)
 {
     return guarded([&] {
         HASTY_REQUIRE(x && out && smaps && coils && kernels && kernel_index && nmodes3 && workspace, "null argument");
         HASTY_REQUIRE(nproblems >= 1, "nproblems must be >= 1");
+        for (int p = 0; p < nproblems; ++p)
+            HASTY_REQUIRE(kernel_index[p] >= 0 && kernel_index[p] < nkernels, "kernel index out of range");
         for (int d = 0; d < 3; ++d) HASTY_REQUIRE(nmodes3[d] >= 1, "nmodes must be >= 1");
         sense_normal_toeplitz((const float2*)x, (float2*)out, (const float2*)smaps, coils, ncoil, kernels, layout,
                               kernel_index, (const float2*)sub, nproblems, nmodes3, workspace, workspace_bytes,

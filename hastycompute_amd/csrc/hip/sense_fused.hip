@@ -37,6 +37,12 @@ constexpr int kZ = 1024;         // threads of the z kernel
 
 __device__ __forceinline__ int brev5(int b) { return (int)(__brev((unsigned)b) >> 27); }
 
+// Workgroup barrier for LDS data exchange only: waits for this wave's LDS operations,
+// not for its outstanding global stores. Threads of a workgroup here never read global
+// data another thread of the same workgroup wrote, so stores of one coset (or coil) can
+// drain while the next one is computed.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // exp(sign 2 pi i k / n) from a forward table of length n (tw[k] = exp(-2 pi i k / n)).
 template <int S> __device__ __forceinline__ float2 twid(const float2* tw, int k)
 {
@@ -208,13 +214,113 @@ __device__ __forceinline__ float2* col_fft(float2* bufA, float2* bufB, int L, ui
         case 5: col_pass<5, S, NCOL, NT>(src, dst, L, Ns, twL, zf); break;
         default: col_pass<8, S, NCOL, NT>(src, dst, L, Ns, twL, zf); break;
         }
-        __syncthreads();
+        lds_barrier();
         float2* t = src;
         src = dst;
         dst = t;
         Ns *= rad;
     }
     return src;
+}
+
+// Compile-time pass geometry (no integer division at run time). Ping-pong form.
+template <int RAD, int S, int NCOL, int NT, int L, int NS>
+__device__ __forceinline__ void col_pass_ct(const float2* __restrict__ src, float2* __restrict__ dst,
+                                           const float2* __restrict__ twL, int zero_from)
+{
+    constexpr int nbf = L / RAD;
+    constexpr int twstep = L / (NS * RAD);
+    constexpr int total = nbf * NCOL;
+    constexpr int iters = (total + NT - 1) / NT;
+#pragma unroll
+    for (int it = 0; it < iters; ++it) {
+        const int bf = threadIdx.x + it * NT;
+        if (total % NT != 0 && bf >= total) break;
+        const int c = bf % NCOL;
+        const int j = bf / NCOL;
+        const int k = j % NS;
+        float2 v[RAD];
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) {
+            const int row = j + r * nbf;
+            float2 x = row < zero_from ? src[row * NCOL + c] : float2{0.f, 0.f};
+            if (r > 0 && NS > 1) x = cmul(x, twid<S>(twL, r * k * twstep));
+            v[r] = x;
+        }
+        dftR<RAD, S>(v);
+        const int o = (j / NS) * NS * RAD + k;
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) dst[(o + r * NS) * NCOL + c] = v[r];
+    }
+}
+
+// Same pass in place: every thread first gathers all its butterflies into registers,
+// the workgroup synchronises, then the results are written back (one LDS buffer).
+template <int RAD, int S, int NCOL, int NT, int L, int NS>
+__device__ __forceinline__ void col_pass_ip(float2* __restrict__ buf, const float2* __restrict__ twL, int zero_from)
+{
+    constexpr int nbf = L / RAD;
+    constexpr int twstep = L / (NS * RAD);
+    constexpr int total = nbf * NCOL;
+    constexpr int BPT = (total + NT - 1) / NT;
+    float2 v[BPT][RAD];
+#pragma unroll
+    for (int q = 0; q < BPT; ++q) {
+        const int bf = threadIdx.x + q * NT;
+        if (total % NT != 0 && bf >= total) break;
+        const int c = bf % NCOL;
+        const int j = bf / NCOL;
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) {
+            const int row = j + r * nbf;
+            float2 x = row < zero_from ? buf[row * NCOL + c] : float2{0.f, 0.f};
+            if (r > 0 && NS > 1) x = cmul(x, twid<S>(twL, r * k * twstep));
+            v[q][r] = x;
+        }
+        dftR<RAD, S>(v[q]);
+    }
+    lds_barrier();
+#pragma unroll
+    for (int q = 0; q < BPT; ++q) {
+        const int bf = threadIdx.x + q * NT;
+        if (total % NT != 0 && bf >= total) break;
+        const int c = bf % NCOL;
+        const int j = bf / NCOL;
+        const int k = j % NS;
+        const int o = (j / NS) * NS * RAD + k;
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) buf[(o + r * NS) * NCOL + c] = v[q][r];
+    }
+    lds_barrier();
+}
+
+// L = 64 * NB as radix 8 x 8 (x NB): ping-pong form, returns the result buffer.
+template <int S, int NCOL, int NT, int NB>
+__device__ __forceinline__ float2* col_fft_ct(float2* a, float2* b, const float2* twL, int zero_from)
+{
+    constexpr int L = 64 * NB;
+    col_pass_ct<8, S, NCOL, NT, L, 1>(a, b, twL, zero_from);
+    lds_barrier();
+    col_pass_ct<8, S, NCOL, NT, L, 8>(b, a, twL, L);
+    lds_barrier();
+    if constexpr (NB == 1) {
+        return a;
+    } else {
+        col_pass_ct<NB, S, NCOL, NT, L, 64>(a, b, twL, L);
+        lds_barrier();
+        return b;
+    }
+}
+
+// In-place form (result in buf).
+template <int S, int NCOL, int NT, int NB>
+__device__ __forceinline__ void col_fft_ip(float2* buf, const float2* twL, int zero_from)
+{
+    constexpr int L = 64 * NB;
+    col_pass_ip<8, S, NCOL, NT, L, 1>(buf, twL, zero_from);
+    col_pass_ip<8, S, NCOL, NT, L, 8>(buf, twL, L);
+    if constexpr (NB > 1) col_pass_ip<NB, S, NCOL, NT, L, 64>(buf, twL, L);
 }
 
 struct FusedArgs {
@@ -255,7 +361,7 @@ template <int NB> __global__ void __launch_bounds__(kXY) fwd_xy_kernel(FusedArgs
     const int64_t plane = (int64_t)N * N;
     // the x plane is re-read per coil (from L2 after the first coil) to keep registers free
     const float2* xp = a.x + ((int64_t)p * a.N0 + z) * plane;
-    __syncthreads();
+    lds_barrier();
 
     for (int ci = 0; ci < a.ccount; ++ci) {
         const int coil = a.coils[a.c0 + ci];
@@ -281,11 +387,11 @@ template <int NB> __global__ void __launch_bounds__(kXY) fwd_xy_kernel(FusedArgs
                 acc = dft32_fwd(acc, b, tw32);
                 bufA[(yl + 32 * k) * kW + brev5(b)] = acc;
             }
-            __syncthreads();
-            const float2* res = col_fft<-1, kW, kXY>(bufA, bufB, L, a.codeL, a.npassL, twL, N);
+            lds_barrier();
+            const float2* res = col_fft_ct<-1, kW, kXY, NB>(bufA, bufB, twL, N);
             float2* dst = Tp + (int64_t)s * L * kW;
             for (int e = t; e < L * kW; e += kXY) dst[e] = res[e];
-            __syncthreads();
+            lds_barrier();
         }
     }
 }
@@ -312,7 +418,7 @@ template <int NB> __global__ void __launch_bounds__(kXY) inv_xy_kernel(FusedArgs
     for (int k = 0; k < NB; ++k)
 #pragma unroll
         for (int j = 0; j < NB; ++j) acc[k][j] = float2{0.f, 0.f};
-    __syncthreads();
+    lds_barrier();
 
     for (int ci = 0; ci < a.ccount; ++ci) {
         const int coil = a.coils[a.c0 + ci];
@@ -329,8 +435,8 @@ template <int NB> __global__ void __launch_bounds__(kXY) inv_xy_kernel(FusedArgs
         for (int s = 0; s < P; ++s) {
             const float2* src = Tp + (int64_t)s * L * kW;
             for (int e = t; e < L * kW; e += kXY) bufA[e] = src[e];
-            __syncthreads();
-            const float2* res = col_fft<1, kW, kXY>(bufA, bufB, L, a.codeL, a.npassL, twL, L);
+            lds_barrier();
+            const float2* res = col_fft_ct<1, kW, kXY, NB>(bufA, bufB, twL, L);
             // rows y < N: inverse 32-point DFT over m (bit-reversed load), coset twiddle,
             // conj(S) multiply, accumulate
 #pragma unroll
@@ -345,7 +451,7 @@ template <int NB> __global__ void __launch_bounds__(kXY) inv_xy_kernel(FusedArgs
                     acc[k][j] = cadd(acc[k][j], cmul(sv[k][j], cmul(w, u)));
                 }
             }
-            __syncthreads();
+            lds_barrier();
         }
     }
     float2* op = a.out + ((int64_t)p * a.N0 + z) * plane;
@@ -377,32 +483,198 @@ __global__ void __launch_bounds__(kZ) z_conv_kernel(FusedArgs a, int LL, int pl,
     // kernel values of the elements this thread multiplies (fixed across coils)
     constexpr int kMaxPer = 16;  // L0 * 32 / 1024 <= 16 (L0 <= 512)
     float kv[kMaxPer];
-    const int per = (L0 * kW) / kZ;
+    const int nel = L0 * kW;
 #pragma unroll
     for (int i = 0; i < kMaxPer; ++i) {
-        if (i < per) {
-            const int e = t + kZ * i;
-            kv[i] = K[(int64_t)(e / kW) * LL + pos0 + (e % kW)];
-        }
+        const int e = t + kZ * i;
+        kv[i] = e < nel ? K[(int64_t)(e / kW) * LL + pos0 + (e % kW)] : 0.f;
     }
-    __syncthreads();
+    lds_barrier();
     for (int ci = 0; ci < a.ccount; ++ci) {
         float2* Tc = a.T + ((int64_t)pl * a.ccount + ci) * a.N0 * (int64_t)LL + pos0;
         for (int e = t; e < a.N0 * kW; e += kZ) bufA[e] = Tc[(int64_t)(e / kW) * LL + (e % kW)];
-        __syncthreads();
+        lds_barrier();
         float2* res = col_fft<-1, kW, kZ>(bufA, bufB, L0, a.codeL0, a.npassL0, twz, a.N0);
         float2* other = res == bufA ? bufB : bufA;
 #pragma unroll
         for (int i = 0; i < kMaxPer; ++i) {
-            if (i < per) {
-                const int e = t + kZ * i;
-                res[e] = cscale(res[e], kv[i]);
-            }
+            const int e = t + kZ * i;
+            if (e < nel) res[e] = cscale(res[e], kv[i]);
         }
-        __syncthreads();
+        lds_barrier();
         const float2* res2 = col_fft<1, kW, kZ>(res, other, L0, a.codeL0, a.npassL0, twz, L0);
         for (int e = t; e < a.N0 * kW; e += kZ) Tc[(int64_t)(e / kW) * LL + (e % kW)] = res2[e];
-        __syncthreads();
+        lds_barrier();
+    }
+}
+
+// z-axis pass for L0 = 64 * NB0: 16-column bundles, 512 threads, one in-place LDS buffer
+// (L0 * 16 complex), so several workgroups share a CU and overlap loads with FFT work.
+constexpr int kZC = 16;
+constexpr int kZT = 512;
+
+template <int NB0>
+__global__ void __launch_bounds__(kZT) z_conv_ct_kernel(FusedArgs a, int LL, int pl, const float* __restrict__ K)
+{
+    constexpr int L0 = 64 * NB0;
+    constexpr int N0 = L0 / 2;
+    constexpr int NEL = L0 * kZC;
+    constexpr int PER = (NEL + kZT - 1) / kZT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* buf = reinterpret_cast<float2*>(smem_raw);
+    float2* twz = buf + NEL;
+    const int t = threadIdx.x;
+    const int pos0 = blockIdx.x * kZC;
+    for (int i = t; i < L0; i += kZT) twz[i] = a.twL0[i];
+    float kv[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = t + kZT * i;
+        kv[i] = e < NEL ? K[(int64_t)(e / kZC) * LL + pos0 + (e % kZC)] : 0.f;
+    }
+    lds_barrier();
+    // the next coil's bundle is prefetched into registers during the current transform
+    constexpr int PFZ = (N0 * kZC + kZT - 1) / kZT;
+    float2 pf[PFZ];
+    {
+        const float2* T0 = a.T + (int64_t)pl * a.ccount * N0 * (int64_t)LL + pos0;
+#pragma unroll
+        for (int i = 0; i < PFZ; ++i) {
+            const int e = t + kZT * i;
+            if (e < N0 * kZC) pf[i] = T0[(int64_t)(e / kZC) * LL + (e % kZC)];
+        }
+    }
+    for (int ci = 0; ci < a.ccount; ++ci) {
+        float2* Tc = a.T + ((int64_t)pl * a.ccount + ci) * N0 * (int64_t)LL + pos0;
+#pragma unroll
+        for (int i = 0; i < PFZ; ++i) {
+            const int e = t + kZT * i;
+            if (e < N0 * kZC) buf[e] = pf[i];
+        }
+        lds_barrier();
+        if (ci + 1 < a.ccount) {
+            const float2* Tn = Tc + (int64_t)N0 * LL;
+#pragma unroll
+            for (int i = 0; i < PFZ; ++i) {
+                const int e = t + kZT * i;
+                if (e < N0 * kZC) pf[i] = Tn[(int64_t)(e / kZC) * LL + (e % kZC)];
+            }
+        }
+        col_fft_ip<-1, kZC, kZT, NB0>(buf, twz, N0);
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = t + kZT * i;
+            if (e < NEL) buf[e] = cscale(buf[e], kv[i]);
+        }
+        lds_barrier();
+        col_fft_ip<1, kZC, kZT, NB0>(buf, twz, L0);
+#pragma unroll
+        for (int i = 0; i < (N0 * kZC + kZT - 1) / kZT; ++i) {
+            const int e = t + kZT * i;
+            if (e < N0 * kZC) Tc[(int64_t)(e / kZC) * LL + (e % kZC)] = buf[e];
+        }
+        lds_barrier();
+    }
+}
+
+// 16-point DFT in registers: n = 4a + b, k = c + 4d,
+// X[c + 4d] = sum_b w16^{bc} w4^{bd} sum_a v[4a + b] w4^{ac}.
+template <int S> __device__ __forceinline__ void dft16(float2* v)
+{
+    // w16^m = exp(S 2 pi i m / 16), m = 0..9
+    const float C[10] = {1.f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f, 0.f,
+                         -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f, -1.f,
+                         -0.92387953251128674f};
+    const float Sn[10] = {0.f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f, 1.f,
+                          0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f, 0.f,
+                          -0.38268343236508977f};
+    float2 t[16];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        float2 q[4] = {v[b], v[4 + b], v[8 + b], v[12 + b]};
+        dft4<S>(q);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int m = b * c;
+            t[4 * b + c] = m == 0 ? q[c] : cmul(q[c], float2{C[m], (float)S * Sn[m]});
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float2 q[4] = {t[c], t[4 + c], t[8 + c], t[12 + c]};
+        dft4<S>(q);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) v[c + 4 * d] = q[d];
+    }
+}
+
+// z-axis pass for L0 = 256 (N0 = 128) as a 16 x 16 four-step FFT with the data in
+// registers: each of 256 threads owns one (column, group) pair of a 16-column bundle,
+// does 16-point DFTs in registers and meets the others through one LDS transpose per
+// direction; loads and stores go straight between HBM and registers.
+constexpr int kR16C = 16;   // columns per bundle
+constexpr int kR16T = 256;  // threads
+constexpr int kR16S1 = 17;  // LDS stride of the inner index (padded)
+constexpr int kR16S2 = 280; // LDS stride of the outer index (padded)
+
+__global__ void __launch_bounds__(kR16T) z_conv_r16_kernel(FusedArgs a, int LL, int pl, const float* __restrict__ K)
+{
+    __shared__ float2 lds[16 * kR16S2];
+    const int t = threadIdx.x;
+    const int c = t & 15;
+    const int g = t >> 4;
+    const int pos0 = blockIdx.x * kR16C;
+    float2 tw[16];  // w256^{g m}, forward sign
+#pragma unroll
+    for (int m = 0; m < 16; ++m) tw[m] = a.twL0[(g * m) & 255];
+    float kv[16];   // kernel at k = g + 16 k2 of this column
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) kv[k2] = K[(int64_t)(g + 16 * k2) * LL + pos0 + c];
+    const float2* T0 = a.T + (int64_t)pl * a.ccount * 128 * (int64_t)LL + pos0 + c;
+    float2 pf[8];
+#pragma unroll
+    for (int n1 = 0; n1 < 8; ++n1) pf[n1] = T0[(int64_t)(16 * n1 + g) * LL];
+
+    for (int ci = 0; ci < a.ccount; ++ci) {
+        float2* Tc = a.T + ((int64_t)pl * a.ccount + ci) * 128 * (int64_t)LL + pos0 + c;
+        float2 v[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 8; ++n1) v[n1] = pf[n1];
+#pragma unroll
+        for (int n1 = 8; n1 < 16; ++n1) v[n1] = float2{0.f, 0.f};
+        if (ci + 1 < a.ccount) {
+            const float2* Tn = Tc + (int64_t)128 * LL;
+#pragma unroll
+            for (int n1 = 0; n1 < 8; ++n1) pf[n1] = Tn[(int64_t)(16 * n1 + g) * LL];
+        }
+        // forward, step A (n2 = g): DFT over n1, twiddle w256^{g k1}, transpose
+        dft16<-1>(v);
+#pragma unroll
+        for (int k1 = 1; k1 < 16; ++k1) v[k1] = cmul(v[k1], tw[k1]);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) lds[k1 * kR16S2 + g * kR16S1 + c] = v[k1];
+        lds_barrier();
+        // forward, step B (k1 = g): DFT over n2 -> X[g + 16 k2]; times the kernel
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) v[n2] = lds[g * kR16S2 + n2 * kR16S1 + c];
+        dft16<-1>(v);
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) v[k2] = cscale(v[k2], kv[k2]);
+        // inverse, step B' (k1 = g): IDFT over k2 -> n1, twiddle w256^{-g n1}
+        dft16<1>(v);
+#pragma unroll
+        for (int n1 = 1; n1 < 16; ++n1) v[n1] = cmulc(tw[n1], v[n1]);
+        lds_barrier();  // all step-B reads are done before the transpose is overwritten
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1) lds[n1 * kR16S2 + g * kR16S1 + c] = v[n1];
+        lds_barrier();
+        // inverse, step A' (n1 = g): IDFT over k1 -> out[g + 16 n2], keep rows < 128
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) v[k1] = lds[g * kR16S2 + k1 * kR16S1 + c];
+        dft16<1>(v);
+#pragma unroll
+        for (int n2 = 0; n2 < 8; ++n2) Tc[(int64_t)(g + 16 * n2) * LL] = v[n2];
+        lds_barrier();
     }
 }
 
@@ -488,6 +760,13 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
     a.codeL0 = radix_code(L0, &a.npassL0);
     const size_t smem_xy = sizeof(float2) * (2 * (size_t)L * kW + L + 32);
     const size_t smem_z = sizeof(float2) * (2 * (size_t)L0 * kW + L0);
+    const size_t smem_zct = sizeof(float2) * ((size_t)L0 * kZC + L0);
+    decltype(&z_conv_ct_kernel<1>) zct = nullptr;
+    if (L0 == 64) zct = z_conv_ct_kernel<1>;
+    else if (L0 == 128) zct = z_conv_ct_kernel<2>;
+    else if (L0 == 192) zct = z_conv_ct_kernel<3>;
+    else if (L0 == 256) zct = z_conv_ct_kernel<4>;
+    if (zct) allow_lds(zct, smem_zct);
     auto fwd = NB == 1 ? fwd_xy_kernel<1> : NB == 2 ? fwd_xy_kernel<2> : NB == 3 ? fwd_xy_kernel<3> : fwd_xy_kernel<4>;
     auto inv = NB == 1 ? inv_xy_kernel<1> : NB == 2 ? inv_xy_kernel<2> : NB == 3 ? inv_xy_kernel<3> : inv_xy_kernel<4>;
     allow_lds(fwd, smem_xy);
@@ -504,7 +783,15 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
             check_launch("fwd_xy_kernel");
             for (int pl = 0; pl < np; ++pl) {
                 const float* K = kernels + (int64_t)kidx[p0 + pl] * L0 * (int64_t)L * L;
-                hipLaunchKernelGGL(z_conv_kernel, dim3((unsigned)(L * L / kW)), dim3(kZ), smem_z, s, a, L * L, pl, K);
+                if (L0 == 256) {
+                    hipLaunchKernelGGL(z_conv_r16_kernel, dim3((unsigned)(L * L / kR16C)), dim3(kR16T), 0, s, a,
+                                       L * L, pl, K);
+                } else if (zct) {
+                    hipLaunchKernelGGL(zct, dim3((unsigned)(L * L / kZC)), dim3(kZT), smem_zct, s, a, L * L, pl, K);
+                } else {
+                    hipLaunchKernelGGL(z_conv_kernel, dim3((unsigned)(L * L / kW)), dim3(kZ), smem_z, s, a, L * L, pl,
+                                       K);
+                }
                 check_launch("z_conv_kernel");
             }
             hipLaunchKernelGGL(inv, dim3(N0, np), dim3(kXY), smem_xy, s, a);

@@ -147,6 +147,7 @@ class SenseNormalToeplitz:
         self.n3 = list(smaps.shape[1:])
         _require(list(kernels.shape[1:]) == [2 * n for n in self.n3], "kernel must be [nk, 2N0, 2N1, 2N2]")
         coils = list(range(smaps.shape[0])) if coils is None else [int(c) for c in coils]
+        _require(len(coils) >= 1 and all(0 <= c < smaps.shape[0] for c in coils), "coil index out of range")
         self.coils = torch.tensor(coils, dtype=torch.int32, device=smaps.device)
         self.ncoil = len(coils)
         self.coil_batch = max(1, min(coil_batch, self.ncoil))
@@ -172,8 +173,8 @@ class SenseNormalToeplitz:
             out = torch.empty_like(x)
         kidx = [0] * P if kernel_index is None else list(kernel_index)
         C.call("hasty_sense_normal_toeplitz", x.data_ptr(), out.data_ptr(), self.smaps.data_ptr(),
-               self.coils.data_ptr(), self.ncoil, self.kernels.data_ptr(), self.layout, C.i32arr(kidx), C.ptr(sub),
-               P, C.i64arr(self.n3), self.workspace.data_ptr(), self.workspace.numel(), self.coil_batch,
+               self.coils.data_ptr(), self.ncoil, self.kernels.data_ptr(), self.kernels.shape[0], self.layout,
+               C.i32arr(kidx), C.ptr(sub), P, C.i64arr(self.n3), self.workspace.data_ptr(), self.workspace.numel(), self.coil_batch,
                C.stream_ptr())
         return out
 

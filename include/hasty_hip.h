@@ -97,9 +97,9 @@ size_t hasty_sense_normal_workspace_size(const int64_t* nmodes3, int coil_batch,
 int hasty_sense_normal_prepare(const float* natural, float* prepared, int64_t nk, const int64_t* nmodes3,
                                int* layout, void* stream);
 int hasty_sense_normal_toeplitz(const void* x, void* out, const void* smaps, const int32_t* coils, int ncoil,
-                                const float* kernels, int layout, const int32_t* kernel_index, const void* sub,
-                                int nproblems, const int64_t* nmodes3, void* workspace, size_t workspace_bytes,
-                                int coil_batch, void* stream);
+                                const float* kernels, int nkernels, int layout, const int32_t* kernel_index,
+                                const void* sub, int nproblems, const int64_t* nmodes3, void* workspace,
+                                size_t workspace_bytes, int coil_batch, void* stream);
 
 /* Library options: "sense_fused" (default 1) lets hasty_sense_normal_prepare choose the
  * fused layout. Returns 0, or 1 for an unknown name. */
