@@ -66,15 +66,20 @@ def build_hip(jobs: int = 8, verbose: bool = False) -> str:
             if verbose and out.strip():
                 print(out)
     tmp = HIP_LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-soname,libhasty_hip.so", "-o", tmp, *objs])
     os.replace(tmp, HIP_LIB)
     return HIP_LIB
 
 
 def build_torch(verbose: bool = False) -> str:
+    import ctypes
+
     import torch
     from torch.utils.cpp_extension import load
 
+    # the loader opens the fresh build from its build folder, where $ORIGIN does not
+    # reach libhasty_hip.so; a global preload (matched by soname) satisfies it
+    ctypes.CDLL(HIP_LIB, mode=ctypes.RTLD_GLOBAL)
     os.makedirs(LIB_DIR, exist_ok=True)
     bdir = os.path.join(BUILD_DIR, "torch_ext")
     os.makedirs(bdir, exist_ok=True)
@@ -85,7 +90,8 @@ def build_torch(verbose: bool = False) -> str:
         sources=srcs,
         extra_cflags=["-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1", "-I/opt/rocm/include",
                       "-I" + os.path.join(ROOT, "include"), "-I" + HOST_SRC],
-        extra_ldflags=["-L" + LIB_DIR, "-lhasty_hip", "-Wl,-rpath,$ORIGIN", "-L" + torch_lib, "-lc10_hip",
+        # "$$" is ninja's escape and "\$" the shell's: the linker sees -rpath,$ORIGIN
+        extra_ldflags=["-L" + LIB_DIR, "-lhasty_hip", "-Wl,-rpath,\\$$ORIGIN", "-L" + torch_lib, "-lc10_hip",
                        "-ltorch_hip"],
         build_directory=bdir,
         is_python_module=False,

@@ -1,0 +1,352 @@
+// TorchScript classes with the reference's module / class / method names:
+//   torch.classes.HastyNufft.{NufftOptions, Nufft, NufftNormal}        python/cpp/py_nufft.cpp:83-106
+//   torch.classes.HastySense.{Sense, SenseAdjoint, SenseNormal}        python/cpp/py_sense.cpp:83-96
+//   torch.classes.HastyBatchedSense.{BatchedSense, BatchedSenseAdjoint, BatchedSenseNormal}
+//                                                                      python/cpp/py_batched_sense.cpp:171-190
+//   torch.classes.HastySVT.{Random3DBlocksSVT, Normal3DBlocksSVT}      python/cpp/py_svt.cpp:64-73
+//   torch.classes.HastyInterface.FunctionLambda                        python/cpp/py_interface.cpp:37-42
+// Errors surface as RuntimeError carrying the operator's message (torch_util.cpp:181-200).
+#include <torch/custom_class.h>
+#include <torch/jit.h>
+#include <torch/script.h>
+
+#include <numeric>
+
+#include "hasty_ops.hpp"
+
+namespace hasty {
+namespace ffi {
+
+namespace {
+template <typename F> void catcher(F&& f)
+{
+    try {
+        f();
+    } catch (const c10::Error& e) {
+        throw std::runtime_error(e.what());
+    }
+}
+
+std::vector<std::vector<int64_t>> all_coils(int nouter, int ncoil)
+{
+    std::vector<int64_t> spec(ncoil);
+    std::iota(spec.begin(), spec.end(), 0);
+    return std::vector<std::vector<int64_t>>(nouter, spec);
+}
+
+using Streams = c10::optional<std::vector<c10::Stream>>;
+
+void check_streams(const Streams& streams)
+{
+    // one GPU per process: streams on other devices cannot be honoured
+    if (!streams.has_value()) return;
+    for (const auto& s : *streams)
+        if (s.device_type() != c10::DeviceType::CUDA)
+            throw std::runtime_error("streams must be GPU streams");
+}
+}  // namespace
+
+class FunctionLambda : public torch::CustomClassHolder {
+public:
+    FunctionLambda(const std::string& script, const std::string& entry, std::vector<at::Tensor> captures)
+        : _cunit(torch::jit::compile(script)), _entry(entry), _captures(std::move(captures))
+    {
+    }
+    void apply(std::vector<at::Tensor> inout) const { _cunit->run_method(_entry, inout, _captures); }
+
+private:
+    std::shared_ptr<torch::jit::CompilationUnit> _cunit;
+    std::string _entry;
+    std::vector<at::Tensor> _captures;
+};
+
+class NufftOptions : public torch::CustomClassHolder {
+public:
+    NufftOptions(int64_t type, const c10::optional<bool>& positive, const c10::optional<double>& tol)
+    {
+        if (type > 3 || type < 1) throw std::runtime_error("Not available type");
+        _opts = fft::NufftOptions((fft::NufftType)type, positive, tol);
+    }
+    const fft::NufftOptions& get() const { return _opts; }
+
+private:
+    fft::NufftOptions _opts;
+};
+
+namespace {
+void require_cuda_coords(const at::Tensor& coords)
+{
+    // the reference's CPU (finufft) operators are not part of this library
+    if (!coords.is_cuda()) throw std::runtime_error("Coordinates did not reside in cuda device");
+}
+}  // namespace
+
+class Nufft : public torch::CustomClassHolder {
+public:
+    Nufft(const at::Tensor& coords, const std::vector<int64_t>& nmodes, const NufftOptions& opts)
+    {
+        require_cuda_coords(coords);
+        _op = std::make_unique<fft::CUDANufft>(coords, nmodes, opts.get());
+    }
+    void apply(const at::Tensor& in, at::Tensor out) const
+    {
+        catcher([&] { _op->apply(in, out); });
+    }
+
+private:
+    std::unique_ptr<fft::CUDANufft> _op;
+};
+
+class NufftNormal : public torch::CustomClassHolder {
+public:
+    NufftNormal(const at::Tensor& coords, const std::vector<int64_t>& nmodes, const NufftOptions& fwd,
+                const NufftOptions& bwd)
+    {
+        require_cuda_coords(coords);
+        _op = std::make_unique<fft::CUDANufftNormal>(coords, nmodes, fwd.get(), bwd.get());
+    }
+    void apply(const at::Tensor& in, at::Tensor out, at::Tensor storage,
+               const c10::optional<c10::intrusive_ptr<FunctionLambda>>& func_between) const
+    {
+        at::optional<std::function<void(at::Tensor&)>> lambda;
+        if (func_between.has_value()) {
+            auto fl = *func_between;
+            lambda = std::function<void(at::Tensor&)>([fl](at::Tensor& t) { fl->apply({t}); });
+        }
+        catcher([&] { _op->apply(in, out, storage, lambda); });
+    }
+
+private:
+    std::unique_ptr<fft::CUDANufftNormal> _op;
+};
+
+class Sense : public torch::CustomClassHolder {
+public:
+    Sense(const at::Tensor& coords, const std::vector<int64_t>& nmodes)
+    {
+        require_cuda_coords(coords);
+        _op = std::make_unique<mri::CUDASense>(coords, nmodes);
+    }
+    void apply(const at::Tensor& in, at::Tensor out, const at::Tensor& smaps, const std::vector<int64_t>& coils,
+               const c10::optional<at::Tensor>& imspace_storage, const c10::optional<at::Tensor>& kspace_storage)
+    {
+        catcher([&] { _op->apply(in, out, smaps, coils, imspace_storage, kspace_storage, at::nullopt, at::nullopt); });
+    }
+
+private:
+    std::unique_ptr<mri::CUDASense> _op;
+};
+
+class SenseAdjoint : public torch::CustomClassHolder {
+public:
+    SenseAdjoint(const at::Tensor& coords, const std::vector<int64_t>& nmodes)
+    {
+        require_cuda_coords(coords);
+        _op = std::make_unique<mri::CUDASenseAdjoint>(coords, nmodes);
+    }
+    void apply(const at::Tensor& in, at::Tensor out, const at::Tensor& smaps, const std::vector<int64_t>& coils,
+               const c10::optional<at::Tensor>& imspace_storage, const c10::optional<at::Tensor>& kspace_storage)
+    {
+        catcher([&] { _op->apply(in, out, smaps, coils, imspace_storage, kspace_storage, at::nullopt, at::nullopt); });
+    }
+
+private:
+    std::unique_ptr<mri::CUDASenseAdjoint> _op;
+};
+
+class SenseNormal : public torch::CustomClassHolder {
+public:
+    SenseNormal(const at::Tensor& coords, const std::vector<int64_t>& nmodes)
+    {
+        require_cuda_coords(coords);
+        _op = std::make_unique<mri::CUDASenseNormal>(coords, nmodes);
+    }
+    void apply(const at::Tensor& in, at::Tensor out, const at::Tensor& smaps, const std::vector<int64_t>& coils,
+               const c10::optional<at::Tensor>& imspace_storage, const c10::optional<at::Tensor>& kspace_storage)
+    {
+        catcher([&] { _op->apply(in, out, smaps, coils, imspace_storage, kspace_storage); });
+    }
+
+private:
+    std::unique_ptr<mri::CUDASenseNormal> _op;
+};
+
+namespace {
+c10::optional<std::vector<at::Tensor>> opt_list(const c10::optional<std::vector<at::Tensor>>& l) { return l; }
+}  // namespace
+
+class BatchedSense : public torch::CustomClassHolder {
+public:
+    BatchedSense(const std::vector<at::Tensor>& coords, const at::Tensor& smaps,
+                 const c10::optional<std::vector<at::Tensor>>& kdata,
+                 const c10::optional<std::vector<at::Tensor>>& weights, const Streams& streams)
+    {
+        check_streams(streams);
+        _bs = std::make_unique<mri::BatchedSense>(coords, smaps, opt_list(kdata), opt_list(weights));
+    }
+    void apply(const at::Tensor& in, std::vector<at::Tensor> out,
+               const c10::optional<std::vector<std::vector<int64_t>>>& coils)
+    {
+        catcher([&] { _bs->apply(in, out, coils.has_value() ? *coils : all_coils(_bs->nouter_batches(), _bs->ncoils())); });
+    }
+
+private:
+    std::unique_ptr<mri::BatchedSense> _bs;
+};
+
+class BatchedSenseAdjoint : public torch::CustomClassHolder {
+public:
+    BatchedSenseAdjoint(const std::vector<at::Tensor>& coords, const at::Tensor& smaps,
+                        const c10::optional<std::vector<at::Tensor>>& kdata,
+                        const c10::optional<std::vector<at::Tensor>>& weights, const Streams& streams)
+    {
+        check_streams(streams);
+        _bs = std::make_unique<mri::BatchedSenseAdjoint>(coords, smaps, opt_list(kdata), opt_list(weights));
+    }
+    void apply(std::vector<at::Tensor> in, at::Tensor out,
+               const c10::optional<std::vector<std::vector<int64_t>>>& coils)
+    {
+        catcher([&] { _bs->apply(in, out, coils.has_value() ? *coils : all_coils(_bs->nouter_batches(), _bs->ncoils())); });
+    }
+
+private:
+    std::unique_ptr<mri::BatchedSenseAdjoint> _bs;
+};
+
+class BatchedSenseNormal : public torch::CustomClassHolder {
+public:
+    BatchedSenseNormal(const std::vector<at::Tensor>& coords, const at::Tensor& smaps,
+                       const c10::optional<std::vector<at::Tensor>>& kdata,
+                       const c10::optional<std::vector<at::Tensor>>& weights, const Streams& streams)
+    {
+        check_streams(streams);
+        _bs = std::make_unique<mri::BatchedSenseNormal>(coords, smaps, opt_list(kdata), opt_list(weights));
+    }
+    void apply(const at::Tensor& in, at::Tensor out, const c10::optional<std::vector<std::vector<int64_t>>>& coils)
+    {
+        catcher([&] { _bs->apply(in, out, coils.has_value() ? *coils : all_coils(_bs->nouter_batches(), _bs->ncoils())); });
+    }
+
+private:
+    std::unique_ptr<mri::BatchedSenseNormal> _bs;
+};
+
+class Random3DBlocksSVT : public torch::CustomClassHolder {
+public:
+    explicit Random3DBlocksSVT(const Streams& streams)
+    {
+        check_streams(streams);
+        _op = std::make_unique<svt::Random3DBlocksSVT>();
+    }
+    void apply(at::Tensor in, int64_t nblocks, const std::array<int64_t, 3>& block_shape, double thresh, bool soft)
+    {
+        catcher([&] { _op->apply(in, nblocks, block_shape, thresh, soft); });
+    }
+
+private:
+    std::unique_ptr<svt::Random3DBlocksSVT> _op;
+};
+
+class Normal3DBlocksSVT : public torch::CustomClassHolder {
+public:
+    explicit Normal3DBlocksSVT(const Streams& streams)
+    {
+        check_streams(streams);
+        _op = std::make_unique<svt::OldNormal3DBlocksSVT>();
+    }
+    void apply(at::Tensor in, const std::array<int64_t, 3>& block_strides, const std::array<int64_t, 3>& block_shape,
+               int64_t block_iter, double thresh, bool soft)
+    {
+        catcher([&] { _op->apply(in, block_strides, block_shape, block_iter, thresh, soft); });
+    }
+
+private:
+    std::unique_ptr<svt::OldNormal3DBlocksSVT> _op;
+};
+
+}  // namespace ffi
+}  // namespace hasty
+
+using namespace hasty::ffi;
+
+TORCH_LIBRARY(HastyInterface, m)
+{
+    m.class_<FunctionLambda>("FunctionLambda")
+        .def(torch::init<const std::string&, const std::string&, std::vector<at::Tensor>>())
+        .def("apply", &FunctionLambda::apply);
+    m.def("doc", []() -> std::string {
+        return "FunctionLambda(script, entry, captures).apply(inout): runs TorchScript function `entry` of `script` "
+               "as entry(inout, captures).";
+    });
+}
+
+TORCH_LIBRARY(HastyNufft, m)
+{
+    m.class_<NufftOptions>("NufftOptions")
+        .def(torch::init<int64_t, const c10::optional<bool>&, const c10::optional<double>&>());
+    m.class_<Nufft>("Nufft")
+        .def(torch::init([](const at::Tensor& coords, const std::vector<int64_t>& nmodes,
+                            const c10::intrusive_ptr<NufftOptions>& opts) {
+            return c10::make_intrusive<Nufft>(coords, nmodes, *opts);
+        }))
+        .def("apply", &Nufft::apply);
+    m.class_<NufftNormal>("NufftNormal")
+        .def(torch::init([](const at::Tensor& coords, const std::vector<int64_t>& nmodes,
+                            const c10::intrusive_ptr<NufftOptions>& fwd, const c10::intrusive_ptr<NufftOptions>& bwd) {
+            return c10::make_intrusive<NufftNormal>(coords, nmodes, *fwd, *bwd);
+        }))
+        .def("apply", &NufftNormal::apply);
+    m.def("doc", []() -> std::string {
+        return "Nufft(coords [ndim, M], nmodes [ntransf, N...], NufftOptions(type, positive, tol)).apply(in, out); "
+               "NufftNormal(coords, nmodes, fwd, bwd).apply(in, out, storage, func_between). Both scale by "
+               "1/sqrt(prod N).";
+    });
+}
+
+TORCH_LIBRARY(HastySense, m)
+{
+    m.class_<Sense>("Sense")
+        .def(torch::init<const at::Tensor&, const std::vector<int64_t>&>())
+        .def("apply", &Sense::apply);
+    m.class_<SenseAdjoint>("SenseAdjoint")
+        .def(torch::init<const at::Tensor&, const std::vector<int64_t>&>())
+        .def("apply", &SenseAdjoint::apply);
+    m.class_<SenseNormal>("SenseNormal")
+        .def(torch::init<const at::Tensor&, const std::vector<int64_t>&>())
+        .def("apply", &SenseNormal::apply);
+    m.def("doc", []() -> std::string {
+        return "Sense / SenseAdjoint / SenseNormal(coords, nmodes).apply(in, out, smaps, coils, imspace_storage, "
+               "kspace_storage), all tensors on the GPU.";
+    });
+}
+
+TORCH_LIBRARY(HastyBatchedSense, m)
+{
+    m.class_<BatchedSense>("BatchedSense")
+        .def(torch::init<const std::vector<at::Tensor>&, const at::Tensor&, const c10::optional<std::vector<at::Tensor>>&,
+                         const c10::optional<std::vector<at::Tensor>>&, const Streams&>())
+        .def("apply", &BatchedSense::apply);
+    m.class_<BatchedSenseAdjoint>("BatchedSenseAdjoint")
+        .def(torch::init<const std::vector<at::Tensor>&, const at::Tensor&, const c10::optional<std::vector<at::Tensor>>&,
+                         const c10::optional<std::vector<at::Tensor>>&, const Streams&>())
+        .def("apply", &BatchedSenseAdjoint::apply);
+    m.class_<BatchedSenseNormal>("BatchedSenseNormal")
+        .def(torch::init<const std::vector<at::Tensor>&, const at::Tensor&, const c10::optional<std::vector<at::Tensor>>&,
+                         const c10::optional<std::vector<at::Tensor>>&, const Streams&>())
+        .def("apply", &BatchedSenseNormal::apply);
+    m.def("doc", []() -> std::string {
+        return "BatchedSense{,Adjoint,Normal}(coords list, smaps, kdata list?, weights list?, streams?).apply(in, out, "
+               "coils?). Tensors may live on the CPU or the GPU; BatchedSenseNormal runs the Toeplitz-embedded "
+               "normal operator, one kernel per outer batch.";
+    });
+}
+
+TORCH_LIBRARY(HastySVT, m)
+{
+    m.class_<Random3DBlocksSVT>("Random3DBlocksSVT").def(torch::init<const Streams&>()).def("apply", &Random3DBlocksSVT::apply);
+    m.class_<Normal3DBlocksSVT>("Normal3DBlocksSVT").def(torch::init<const Streams&>()).def("apply", &Normal3DBlocksSVT::apply);
+    m.def("doc", []() -> std::string {
+        return "Random3DBlocksSVT(streams?).apply(in [F,E,X,Y,Z], nblocks, block_shape, thresh, soft); "
+               "Normal3DBlocksSVT(streams?).apply(in, block_strides, block_shape, block_iter, thresh, soft).";
+    });
+}

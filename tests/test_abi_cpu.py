@@ -1,0 +1,102 @@
+"""CPU checks of the boundary: the C-ABI library loads and exports every function that
+include/hasty_hip.h declares, its host-only size queries answer, and the TorchScript
+operator library registers the reference's module/class names and reports the
+reference's argument errors. No kernel is launched here."""
+import ctypes
+import os
+
+import pytest
+import torch
+
+from hastycompute_amd import _capi
+
+
+def test_header_symbols_exported():
+    names = _capi.header_functions()
+    assert len(names) >= 20
+    lib = ctypes.CDLL(_capi.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # every declared function has a ctypes signature and nothing extra is bound
+    assert sorted(_capi._SIGS) == names
+
+
+def test_host_queries():
+    L = _capi.lib()
+    assert L.hasty_version() >= 1
+    n3 = _capi.i64arr([128, 128, 128])
+    ws = L.hasty_sense_normal_workspace_size(n3, 16, 4)
+    assert ws >= 4 * 16 * 8 * (2 * 128) ** 3 // 8  # at least the per-coil half-grid staging
+    assert L.hasty_cg_workspace_size(4, 1000) > 0
+    assert L.hasty_svt_workspace_size(4096, 100) > 0
+
+
+def test_llr_shifts_host():
+    """hasty_llr_shifts is host code: it must reproduce the oracle's restatement."""
+    from oracle import oracle as O
+
+    import numpy as np
+
+    out = np.zeros((3, 5), dtype=np.int32)
+    _capi.check(_capi.lib().hasty_llr_shifts(5, _capi.i64arr([8, 8, 8]),
+                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    assert (out == O.llr_shifts(5, [8, 8, 8])).all()
+
+
+def test_invalid_arguments_rejected_on_host():
+    L = _capi.lib()
+    plan = ctypes.c_void_p()
+    rc = L.hasty_nufft_makeplan(3, 3, _capi.i64arr([8, 8, 8]), -1, 1, 1e-6, 0, ctypes.byref(plan))
+    assert rc != 0 and b"type" in L.hasty_last_error().lower()
+    rc = L.hasty_nufft_makeplan(2, 4, _capi.i64arr([8, 8, 8, 8]), -1, 1, 1e-6, 0, ctypes.byref(plan))
+    assert rc != 0
+
+
+@pytest.fixture(scope="module")
+def ffi():
+    from hastycompute_amd.ffi import hasty_interface, hasty_nufft, hasty_sense, hasty_svt
+
+    return hasty_nufft, hasty_sense, hasty_svt, hasty_interface
+
+
+def test_torchscript_modules_registered(ffi):
+    for mod, classes in {"HastyNufft": ["NufftOptions", "Nufft", "NufftNormal"],
+                         "HastySense": ["Sense", "SenseAdjoint", "SenseNormal"],
+                         "HastyBatchedSense": ["BatchedSense", "BatchedSenseAdjoint", "BatchedSenseNormal"],
+                         "HastySVT": ["Random3DBlocksSVT", "Normal3DBlocksSVT"],
+                         "HastyInterface": ["FunctionLambda"]}.items():
+        ns = getattr(torch.classes, mod)
+        for c in classes:
+            assert getattr(ns, c) is not None
+        assert isinstance(getattr(torch.ops, mod).doc(), str)
+
+
+def test_reference_errors(ffi):
+    hasty_nufft, hasty_sense, _, _ = ffi
+    with pytest.raises(RuntimeError, match="Not available type"):
+        hasty_nufft.NufftOptions(4)
+    co = torch.zeros(3, 10)
+    with pytest.raises(RuntimeError, match="cuda device"):
+        hasty_nufft.Nufft(co, [1, 8, 8, 8], hasty_nufft.NufftOptions.type2())
+    with pytest.raises(RuntimeError, match="cuda device"):
+        hasty_sense.SenseNormal(co, [1, 8, 8, 8])
+
+
+def test_function_lambda_runs(ffi):
+    _, _, _, hasty_interface = ffi
+    fl = hasty_interface.FunctionLambda(
+        "def apply(a: List[Tensor], b: List[Tensor]):\n    for i in range(len(a)):\n        a[i].mul_(b[0])\n",
+        "apply", [torch.full((4,), 3.0)])
+    x = torch.arange(4.0)
+    fl.apply([x])
+    assert torch.equal(x, torch.arange(4.0) * 3)
+
+
+def test_product_has_no_oracle_import():
+    """Nothing in the package imports the oracle (it is test infrastructure only)."""
+    pkg = os.path.dirname(os.path.abspath(_capi.__file__))
+    for root, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hpp", ".hip", ".h")):
+                text = open(os.path.join(root, f), errors="ignore").read()
+                assert "import oracle" not in text and "from oracle" not in text, f
