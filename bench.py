@@ -1,16 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the batched Toeplitz SENSE normal operator on one MI355X.
 
-Workload (the 4D-flow scan): 5 velocity encodings x 20 cardiac frames = 100 problems,
+Workload (a 4D-flow scan): 5 velocity encodings x 20 cardiac frames = 100 problems,
 image 128^3 complex64, 16 coils, oversampled grid 256^3; kooshball readouts of 128
 samples with per-frame spoke counts drawn from U{800..1500} in a seed-fixed golden-means
 order; one Toeplitz kernel per frame (shared by its 5 encodings). A step is one batched
 normal-operator apply over all 100 problems with inputs resident in HBM; the value is
 problems * steps / seconds (normal-op applies/s).
 
-Also reported: CG-SENSE recons/s (15 iterations per problem, batched device CG), the
-roofline of the apply, and the CPU baseline (the FP64 oracle's Toeplitz SENSE normal
-operator on host cores, one problem x one apply).
+Also reported:
+- per-kernel durations from HIP events recorded on the launch stream over the timed
+  region, and the roofline of the dominant kernel (algorithmic bytes per launch over
+  its average duration);
+- CG-SENSE recons/s for the batched form (15 iterations, all 100 problems in one
+  device-CG batch) and for the plain per-frame drop-in (the explicit NUFFT
+  SenseNormal per problem inside a host-synchronised CG, as the reference runs it),
+  each with p50/p99 latency, and the relative difference of their images;
+- the CPU baseline: the FP64 oracle's Toeplitz SENSE normal operator on host cores.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -18,7 +24,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import platform
 import sys
@@ -28,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def parse():
@@ -47,7 +53,12 @@ def parse():
     ap.add_argument("--generic", action="store_true", help="use the generic five-pass path")
     ap.add_argument("--cg-iters", type=int, default=15)
     ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--plain", type=int, default=3, help="problems solved by the plain per-frame path (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-applies", type=int, default=6)
+    ap.add_argument("--no-kernel-events", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (the known-size kernel for counter calibration)")
     ap.add_argument("--seed", type=int, default=2024)
     return ap.parse_args()
 
@@ -57,6 +68,21 @@ def normal_op_bytes(N: int, C: int) -> int:
     maps (8 C N^3), one complex64 kernel read (8 (2N)^3) and one oversampled-grid write
     + read per coil (16 C (2N)^3)."""
     return 16 * N ** 3 + 8 * C * N ** 3 + 8 * (2 * N) ** 3 + 16 * C * (2 * N) ** 3
+
+
+def kernel_bytes(name: str, N: int, C: int, cb: int, P: int) -> int:
+    """Algorithmic bytes the fused path's kernels move for P problems (all coils, coil
+    batches of cb). The grid between kernels is half-padded: (2N)^2 x N complex64 per
+    coil (32 N^3 B); the Toeplitz kernel is real float32 (2N)^3 (32 N^3 B)."""
+    n3 = N ** 3
+    nbat = -(-C // cb)
+    if name == "sense_fwd_xy":   # x per coil batch + coil maps + grid write
+        return P * (nbat * 8 * n3 + 8 * C * n3 + 32 * C * n3)
+    if name == "sense_z_conv":   # grid read + write + kernel read per coil batch
+        return P * (64 * C * n3 + nbat * 32 * n3)
+    if name == "sense_inv_xy":   # grid read + coil maps + out write (+ out read when accumulating)
+        return P * (32 * C * n3 + 8 * C * n3 + 8 * n3 + (nbat - 1) * 16 * n3)
+    return 0
 
 
 def make_workload(args, torch, native):
@@ -85,44 +111,121 @@ def make_workload(args, torch, native):
     x = torch.randn((P, args.n, args.n, args.n), dtype=torch.complex64, device=dev, generator=g)
     kidx = [p // args.encodings for p in range(P)]
     torch.cuda.synchronize()
-    return K, smaps, x, kidx, npts
+    return K, smaps, x, kidx, npts, spokes, offsets
 
 
 def cpu_baseline(args, K, smaps, x):
-    """The FP64 oracle's Toeplitz SENSE normal operator on host cores for one problem x
-    one apply at the benchmark size (kernel values come from the GPU build; the timing
-    does not depend on them)."""
+    """The FP64 oracle's Toeplitz SENSE normal operator on host cores, one problem x
+    args.cpu_applies applies at the benchmark size (kernel values from the GPU build;
+    the timing does not depend on them)."""
     import numpy as np
 
     from oracle import oracle as O
 
     threads = O.default_threads()
     N = args.n
-    # the oracle takes the reference's complex diagonal: D = kernel * N^d (see DESIGN.md)
+    # the oracle takes the reference's complex diagonal D = kernel * N^d
     diag = (K[0].double().cpu().numpy() * float(N) ** 3).astype(np.complex128)[None]
     S = smaps.cpu().numpy().astype(np.complex128)
     x0 = x[0:1].cpu().numpy().astype(np.complex128)
     t = time.perf_counter()
-    O.sense_normal_toeplitz(diag, x0, S, list(range(args.coils)), workers=threads)
+    for _ in range(args.cpu_applies):
+        O.sense_normal_toeplitz(diag, x0, S, list(range(args.coils)), workers=threads)
     dt = time.perf_counter() - t
-    return {"value": 1.0 / dt, "unit": "applies/s", "cores": threads, "kind": "port",
-            "sample": f"1 problem x 1 apply at {N}^3 x {args.coils} coils, FP64 oracle "
+    return {"value": args.cpu_applies / dt, "unit": "applies/s", "cores": threads, "kind": "port",
+            "sample": f"1 problem x {args.cpu_applies} applies at {N}^3 x {args.coils} coils, FP64 oracle "
                       f"(scipy.fft, {threads} threads) on {platform.processor() or platform.machine()}",
             "seconds": dt}
+
+
+def percentile(v, q):
+    import numpy as np
+
+    return float(np.percentile(np.asarray(v, dtype=np.float64), q))
+
+
+def plain_recons(args, torch, spokes, offsets, smaps, b_all, x_batched, toeplitz_apply):
+    """The plain per-frame drop-in: for each sampled problem, the explicit NUFFT
+    SenseNormal (HastySense.SenseNormal, default NUFFT options) inside the host CG of
+    ConjugateGradient::run (opalgs.cpp:50-96), one problem at a time. Also returns, per
+    problem, the relative difference of one apply (explicit vs Toeplitz operator) and
+    the relative residuals |T x - b| / |b| of both CG results under the Toeplitz
+    operator T."""
+    from hastycompute_amd import synthetic
+    from hastycompute_amd.ffi import hasty_sense
+
+    N = args.n
+    coils = list(range(args.coils))
+    lat, diffs, apply_diffs, res_plain, res_batched = [], [], [], [], []
+    for j in range(args.plain):
+        p = j * args.encodings  # encoding 0 of frames 0, 1, ...
+        f = p // args.encodings
+        co = synthetic.kooshball_torch(int(spokes[f]), args.readout, int(offsets[f]), smaps.device)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op = hasty_sense.SenseNormal(co, [1, N, N, N])
+        im = torch.empty((1, N, N, N), dtype=torch.complex64, device=smaps.device)
+        ks = torch.empty((1, co.shape[1]), dtype=torch.complex64, device=smaps.device)
+
+        def A(v):
+            out = torch.empty_like(v)
+            op.apply(v, out, smaps, coils, im, ks)
+            return out
+
+        b = b_all[p:p + 1]
+        x = torch.zeros_like(b)
+        r = b - A(x)
+        pv = r.clone()
+        rz = torch.vdot(r.flatten(), r.flatten()).real.item()
+        for _ in range(args.cg_iters):
+            Ap = A(pv)
+            pap = torch.vdot(pv.flatten(), Ap.flatten()).real.item()
+            if pap <= 0:
+                raise RuntimeError("A was not positive definite")
+            alpha = rz / pap
+            x.add_(pv, alpha=alpha)
+            r.sub_(Ap, alpha=alpha)
+            rz_new = torch.vdot(r.flatten(), r.flatten()).real.item()
+            pv.mul_(rz_new / rz).add_(r)
+            rz = rz_new
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+        nrm = torch.linalg.vector_norm
+        ref = x_batched[p:p + 1]
+        diffs.append(float(nrm(x - ref) / nrm(ref)))
+        ta, tr = toeplitz_apply(b, p), toeplitz_apply(x, p)
+        apply_diffs.append(float(nrm(A(b) - ta) / nrm(ta)))
+        res_plain.append(float(nrm(tr - b) / nrm(b)))
+        res_batched.append(float(nrm(toeplitz_apply(ref, p) - b) / nrm(b)))
+        del op, co
+    return lat, diffs, apply_diffs, res_plain, res_batched
+
+
+def copy_calibration(torch):
+    n = 1 << 27  # 2^27 complex64 = 1 GiB
+    a = torch.ones(n, dtype=torch.complex64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8}), flush=True)
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is described "
-                                   "in DESIGN.md, not launched)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (how the path would shard over "
+                                   "several GPUs is described, not launched)", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
 
     from hastycompute_amd import native
 
     torch.cuda.set_device(0)
-    K, smaps, x, kidx, npts = make_workload(args, torch, native)
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
+    K, smaps, x, kidx, npts, spokes, offsets = make_workload(args, torch, native)
     P = x.shape[0]
     op = native.SenseNormalToeplitz(smaps, K, coil_batch=args.coil_batch, problem_batch=args.problem_batch,
                                     fused=False if args.generic else None)
@@ -132,6 +235,9 @@ def main():
     for _ in range(args.warmup):
         op.apply(x, out=out, kernel_index=kidx)
     torch.cuda.synchronize()
+    events = not args.no_kernel_events
+    if events:
+        native.timing_enable(True)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
@@ -141,29 +247,39 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    ktimes = native.timing_read() if events else {}
+    if events:
+        native.timing_enable(False)
     ms_total = e0.elapsed_time(e1)
     ms_per_step = ms_total / args.steps
     applies_per_s = P * args.steps / (ms_total / 1e3)
     alg_bytes = normal_op_bytes(args.n, args.coils)
-    achieved = alg_bytes * P / (ms_per_step / 1e3) / 1e9
+    op_achieved = alg_bytes * P / (ms_per_step / 1e3) / 1e9
 
-    extra = {}
-    if not args.no_cg:
-        # CG-SENSE: b = A x_true, 15 iterations from zero, all problems in one batch
-        from hastycompute_amd.native import cg_solve
-
-        b = out  # any right-hand side of the right shape
-        x0 = torch.zeros_like(x)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        cg_solve(lambda v, o: op.apply(v, out=o, kernel_index=kidx), b, x0, args.cg_iters, check_pd=False)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t
-        extra["recons_per_s"] = P / dt
-        extra["cg_iters"] = args.cg_iters
+    kernels = {}
+    for name, (ms, cnt) in ktimes.items():
+        kb = kernel_bytes(name, args.n, args.coils, min(args.coil_batch, args.coils), P * args.steps)
+        kernels[name] = {"launches": cnt, "avg_ms": ms / cnt, "total_ms": ms,
+                         "alg_bytes_per_launch": kb / cnt if kb else None,
+                         "achieved_GBs": kb / (ms / 1e3) / 1e9 if kb else None}
+    roofline = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    if kernels:
+        dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+        d = kernels[dom]
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            tr = json.load(open(TRAFFIC_FILE))
+            key = f"{dom}@{args.n}x{args.coils}x{args.problem_batch}"
+            traffic = tr.get(key, {}).get("hbm_bytes_per_launch")
+        roofline.update({"kernel": dom, "achieved": d["achieved_GBs"], "frac": (d["achieved_GBs"] or 0) / HBM_PEAK_GBS,
+                         "traffic": traffic, "alg_bytes_per_launch": d["alg_bytes_per_launch"],
+                         "avg_launch_ms": d["avg_ms"], "timing": "HIP events on the launch stream over the timed region"})
+    else:
+        roofline.update({"kernel": "sense_normal (whole pipeline)", "achieved": op_achieved,
+                         "frac": op_achieved / HBM_PEAK_GBS, "traffic": None})
 
     line = {
-        "metric": "SENSE normal-op applies/s @128^3x16coil",
+        "metric": "SENSE normal-op applies/s @128^3x16coil; recons/s + p50/p99 latency vs plain",
         "value": applies_per_s,
         "unit": "applies/s",
         "n_gpus": 1,
@@ -173,20 +289,56 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex64 (fp32)",
+        "dtype": "c64 (fp32)",
         "data": "synthetic (seeded kooshball trajectories, Gaussian x linear-phase coil maps, random images)",
         "config": {"workload": "4D-flow batched normal op: 5 enc x 20 frames = 100 problems, 128^3, 16 coils, "
                                "grid 256^3, 800-1500 spokes x 128 per frame",
                    "problems": P, "n": args.n, "coils": args.coils, "coil_batch": args.coil_batch,
                    "problem_batch": args.problem_batch, "path": "fused" if op.layout == 1 else "generic",
                    "points_per_frame_mean": float(sum(npts) / len(npts))},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "sense_normal_apply (whole Toeplitz pipeline per problem)",
-                     "alg_bytes_per_problem": alg_bytes},
+        "roofline": roofline,
+        "op_roofline": {"achieved": op_achieved, "frac": op_achieved / HBM_PEAK_GBS, "alg_bytes_per_apply": alg_bytes,
+                        "what": "whole normal-op pipeline per problem (all kernels)"},
+        "kernels": kernels,
         "wall_s": wall,
     }
-    line.update(extra)
+
+    if not args.no_cg:
+        from hastycompute_amd.native import cg_solve
+
+        from hastycompute_amd import synthetic
+
+        # right-hand sides b = T x_true with x_true a phantom whose global phase varies
+        # over the problems (the velocity encodings and frames)
+        ph = synthetic.ellipsoid_phantom_torch(args.n, x.device, seed=args.seed)
+        phase = torch.exp(1j * torch.linspace(0.0, 3.0, P, device=x.device, dtype=torch.float64)).to(torch.complex64)
+        b = torch.empty_like(x)
+        op.apply((ph[None] * phase[:, None, None, None]).contiguous(), out=b, kernel_index=kidx)
+        del ph
+        x0 = torch.zeros_like(x)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        xb, _ = cg_solve(lambda v, o: op.apply(v, out=o, kernel_index=kidx), b, x0, args.cg_iters, check_pd=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        cg = {"iters": args.cg_iters, "batched_recons_per_s": P / dt,
+              "batched_latency_p50_s": dt, "batched_latency_p99_s": dt,
+              "batched_note": "all problems submitted together; each finishes with the batch"}
+        if args.plain > 0:
+            def t_apply(v, p):
+                o = torch.empty_like(v)
+                op.apply(v, out=o, kernel_index=[kidx[p]])
+                return o
+
+            lat, diffs, adiff, rp, rb = plain_recons(args, torch, spokes, offsets, smaps, b, xb, t_apply)
+            cg.update({"plain_recons_per_s": len(lat) / sum(lat), "plain_latency_p50_s": percentile(lat, 50),
+                       "plain_latency_p99_s": percentile(lat, 99), "plain_sampled_problems": len(lat),
+                       "speedup_recons_per_s": (P / dt) / (len(lat) / sum(lat)),
+                       "plain_vs_batched_apply_rel_diff_max": max(adiff),
+                       "plain_vs_batched_image_rel_diff_max": max(diffs),
+                       "residual_rel_plain_max": max(rp), "residual_rel_batched_max": max(rb)})
+        line["cg"] = cg
+        line["recons_per_s"] = cg["batched_recons_per_s"]
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args, K, smaps, x)
     print(json.dumps(line), flush=True)

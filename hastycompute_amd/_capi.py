@@ -42,6 +42,9 @@ _SIGS = {
     "hasty_sense_normal_toeplitz": ([c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_int, c_int, P_i32, c_vp, c_int, P_i64,
                                      c_vp, c_sz, c_int, c_vp], c_int),
     "hasty_set_option": ([ctypes.c_char_p, c_i64], c_int),
+    "hasty_timing_enable": ([c_int], c_int),
+    "hasty_timing_entries": ([], c_int),
+    "hasty_timing_read": ([c_int, ctypes.c_char_p, c_int, P_dbl, P_i64], c_int),
     "hasty_cg_workspace_size": ([c_i64, c_i64], c_sz),
     "hasty_cg_init": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "hasty_cg_iterate": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp], c_int),

@@ -293,6 +293,30 @@ int hasty_set_option(const char* name, int64_t value)
     return 1;
 }
 
+int hasty_timing_enable(int on)
+{
+    return guarded([&] { timing_enable(on != 0); });
+}
+
+int hasty_timing_entries(void)
+{
+    int n = -1;
+    if (guarded([&] { n = timing_entries(); }) != 0) return -1;
+    return n;
+}
+
+int hasty_timing_read(int index, char* name, int name_capacity, double* total_ms, int64_t* count)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(name && name_capacity > 0 && total_ms && count, "null argument");
+        std::string nm;
+        HASTY_REQUIRE(timing_read(index, &nm, total_ms, count), "timing index out of range");
+        const size_t n = std::min(nm.size(), (size_t)name_capacity - 1);
+        std::memcpy(name, nm.data(), n);
+        name[n] = '\0';
+    });
+}
+
 size_t hasty_cg_workspace_size(int64_t nproblems, int64_t n)
 {
     return sizeof(double) * cg_workspace_doubles(nproblems, n);

@@ -88,4 +88,26 @@ void* dev_alloc(size_t bytes);
 void dev_free(void* p);
 int current_device();
 
+// Per-kernel timing with HIP events recorded on the launch stream around each launch
+// (off by default; hasty_timing_enable). Bracket a launch with a KernelTimer in scope.
+bool timing_on();
+hipEvent_t timing_begin(hipStream_t s);
+void timing_end(const char* name, hipEvent_t start, hipStream_t s);  // name: a string literal
+void timing_enable(bool on);                                        // also clears the statistics
+int timing_entries();
+bool timing_read(int index, std::string* name, double* total_ms, int64_t* count);
+
+struct KernelTimer {
+    KernelTimer(const char* n, hipStream_t st) : name(n), s(st), ev(timing_on() ? timing_begin(st) : nullptr) {}
+    ~KernelTimer()
+    {
+        if (ev) timing_end(name, ev, s);
+    }
+    KernelTimer(const KernelTimer&) = delete;
+    KernelTimer& operator=(const KernelTimer&) = delete;
+    const char* name;
+    hipStream_t s;
+    hipEvent_t ev;
+};
+
 }  // namespace hasty_hip

@@ -376,6 +376,7 @@ template <typename R> void run_axis_fft(const AxisFft<R>& a, hipStream_t stream)
     const int64_t blocks = ceil_div(a.nlines, B);
     HASTY_REQUIRE(blocks < (1ll << 31), "too many FFT lines for one launch");
     const int mode = a.conv_real ? 1 : (a.conv_cplx ? 2 : 0);
+    KernelTimer t("fft_axis", stream);
     if (mode == 0)
         hipLaunchKernelGGL((axis_fft_kernel<R, 0>), dim3((unsigned)blocks), dim3(kFftThreads), smem, stream, args);
     else if (mode == 1)

@@ -443,6 +443,7 @@ void llr_blocks(float2* data, const LlrGeom& g, const int3* corners_dev, int64_t
     int64_t rps = ceil_div(ceil_div(R, nsplit), 16) * 16;
     {
         const size_t smem = sizeof(float) * 2 * 16 * Fp;
+        KernelTimer t("llr_gram", s);
         hipLaunchKernelGGL(gram_kernel, dim3((unsigned)nblk, (unsigned)nsplit), dim3(kThreads), smem, s,
                            (const float2*)data, g, corners_dev, Fp, rps, gpart);
         check_launch("gram_kernel");
@@ -450,6 +451,7 @@ void llr_blocks(float2* data, const LlrGeom& g, const int3* corners_dev, int64_t
     {
         const size_t smem = llr_eig_smem(g.F);
         allow_lds(eig_w_kernel, smem);
+        KernelTimer t("llr_eig", s);
         hipLaunchKernelGGL(eig_w_kernel, dim3((unsigned)nblk), dim3(kThreads), smem, s, (const float2*)gpart, nsplit,
                            (int)g.F, Fp, thresh, soft, rank, 15, W, eig_out);
         check_launch("eig_w_kernel");
@@ -459,6 +461,7 @@ void llr_blocks(float2* data, const LlrGeom& g, const int3* corners_dev, int64_t
         const int64_t nchunk = ceil_div(R, rows_per_wg);
         const size_t smem = sizeof(float) * (2 * (size_t)Fp * Fp + 2 * 16 * (size_t)(Fp + 1));
         allow_lds(apply_kernel, smem);
+        KernelTimer t("llr_apply", s);
         hipLaunchKernelGGL(apply_kernel, dim3((unsigned)nblk, (unsigned)nchunk), dim3(kThreads), smem, s, data, g,
                            corners_dev, Fp, rows_per_wg, (const float2*)W);
         check_launch("apply_kernel");

@@ -332,10 +332,12 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(SpreadArgs<R> a)
 template <typename R, int NS, int DIM> void launch_spread_ns(const SpreadArgs<R>& a, int64_t nsub, size_t smem, hipStream_t s)
 {
     allow_lds(spread_kernel<R, NS, DIM>, smem);
+    KernelTimer t("nufft_spread", s);
     hipLaunchKernelGGL((spread_kernel<R, NS, DIM>), dim3((unsigned)nsub), dim3(kThreads), smem, s, a);
 }
 template <typename R, int NS, int DIM> void launch_interp_ns(const SpreadArgs<R>& a, int64_t M, hipStream_t s)
 {
+    KernelTimer t("nufft_interp", s);
     hipLaunchKernelGGL((interp_kernel<R, NS, DIM>), dim3((unsigned)ceil_div(M, kThreads)), dim3(kThreads), 0, s, a);
 }
 

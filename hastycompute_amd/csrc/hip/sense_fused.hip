@@ -779,10 +779,14 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
             a.c0 = c0;
             a.ccount = std::min(cchunk, ncoil - c0);
             a.accumulate = c0 > 0 ? 1 : 0;
-            hipLaunchKernelGGL(fwd, dim3(N0, np), dim3(kXY), smem_xy, s, a);
-            check_launch("fwd_xy_kernel");
+            {
+                KernelTimer t("sense_fwd_xy", s);
+                hipLaunchKernelGGL(fwd, dim3(N0, np), dim3(kXY), smem_xy, s, a);
+                check_launch("fwd_xy_kernel");
+            }
             for (int pl = 0; pl < np; ++pl) {
                 const float* K = kernels + (int64_t)kidx[p0 + pl] * L0 * (int64_t)L * L;
+                KernelTimer t("sense_z_conv", s);
                 if (L0 == 256) {
                     hipLaunchKernelGGL(z_conv_r16_kernel, dim3((unsigned)(L * L / kR16C)), dim3(kR16T), 0, s, a,
                                        L * L, pl, K);
@@ -794,6 +798,7 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
                 }
                 check_launch("z_conv_kernel");
             }
+            KernelTimer t("sense_inv_xy", s);
             hipLaunchKernelGGL(inv, dim3(N0, np), dim3(kXY), smem_xy, s, a);
             check_launch("inv_xy_kernel");
         }

@@ -265,3 +265,24 @@ def svt_matrix(A: torch.Tensor, thresh: float, soft: bool) -> torch.Tensor:
     C.call("hasty_svt_matrix", A.data_ptr(), R, F, float(thresh), int(bool(soft)), ws.data_ptr(), ws.numel(),
            C.stream_ptr())
     return A
+
+
+def timing_enable(on: bool = True) -> None:
+    """Start (and clear) or stop per-kernel event timing inside the library."""
+    C.call("hasty_timing_enable", 1 if on else 0)
+
+
+def timing_read() -> dict:
+    """{kernel name: (total_ms, launches)} of the launches timed since timing_enable."""
+    L = C.lib()
+    n = L.hasty_timing_entries()
+    if n < 0:
+        C.check(1)
+    out = {}
+    buf = ctypes.create_string_buffer(128)
+    for i in range(n):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int64()
+        C.call("hasty_timing_read", i, buf, 128, ctypes.byref(ms), ctypes.byref(cnt))
+        out[buf.value.decode()] = (ms.value, cnt.value)
+    return out

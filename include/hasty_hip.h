@@ -105,6 +105,16 @@ int hasty_sense_normal_toeplitz(const void* x, void* out, const void* smaps, con
  * fused layout. Returns 0, or 1 for an unknown name. */
 int hasty_set_option(const char* name, int64_t value);
 
+/* Kernel timing (no counterpart in the reference; a measurement hook for benchmarks).
+ * While enabled, each instrumented launch is bracketed by two HIP events on its own
+ * stream, and the durations are summed per kernel name. hasty_timing_enable clears the
+ * statistics; hasty_timing_entries waits for the recorded events and returns the number
+ * of kernel names (or -1 on error); hasty_timing_read copies entry `index` (names in
+ * sorted order, NUL-terminated, truncated to name_capacity). */
+int hasty_timing_enable(int on);
+int hasty_timing_entries(void);
+int hasty_timing_read(int index, char* name, int name_capacity, double* total_ms, int64_t* count);
+
 /* ---------------------------------------------------------------------------------
  * Device-side batched conjugate gradient, the loop body of ConjugateGradient::run
  * (lib/op/opalgs.cpp:50-96) for P problems of n complex64 unknowns with FP64 scalars
