@@ -608,6 +608,236 @@ template <int S> __device__ __forceinline__ void dft16(float2* v)
     }
 }
 
+// 12-point DFT in registers: n = 4 n1 + n2, k = k1 + 3 k2,
+// X[k1 + 3 k2] = sum_{n2} w12^{n2 k1} w4^{n2 k2} sum_{n1} v[4 n1 + n2] w3^{n1 k1}.
+template <int S> __device__ __forceinline__ void dft12(float2* v)
+{
+    // w12^m = exp(S 2 pi i m / 12), m = 0..6
+    const float C[7] = {1.f, 0.86602540378443865f, 0.5f, 0.f, -0.5f, -0.86602540378443865f, -1.f};
+    const float Sn[7] = {0.f, 0.5f, 0.86602540378443865f, 1.f, 0.86602540378443865f, 0.5f, 0.f};
+    float2 t[12];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        float2 q[3] = {v[n2], v[4 + n2], v[8 + n2]};
+        dft3<S>(q);
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1) {
+            const int mm = n2 * k1;
+            t[3 * n2 + k1] = mm == 0 ? q[k1] : cmul(q[k1], float2{C[mm], (float)S * Sn[mm]});
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+        float2 q[4] = {t[k1], t[3 + k1], t[6 + k1], t[9 + k1]};
+        dft4<S>(q);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 3 * k2] = q[k2];
+    }
+}
+
+// Natural-order DFT of length R in {4, 8, 12, 16} in registers.
+template <int R, int S> __device__ __forceinline__ void dftn(float2* v)
+{
+    if constexpr (R == 4) dft4<S>(v);
+    else if constexpr (R == 8) dft8<S>(v);
+    else if constexpr (R == 12) dft12<S>(v);
+    else dft16<S>(v);
+}
+
+// Plane kernels with the L-point y transforms as register four-step FFTs,
+// L = 16 * R2 (R2 = L / 16 = 4 NB): row y = y1 + 16 y2 and frequency ky = k1 + R2 k2, so
+//   X[k1 + R2 k2] = sum_{y1} w16^{y1 k2} [ wL^{y1 k1} sum_{y2} x[y1 + 16 y2] wR2^{y2 k1} ].
+// Each step is a DFT in registers; the two steps meet through one LDS transpose. Two
+// cosets of 32 x-frequencies go through a workgroup together; each of the 512 threads
+// owns one (column, y1 or k1) position of both cosets, and in the x step and the
+// output step the rows yl + 16 k of lane b. 512 threads leave 256 VGPRs per thread for
+// the plane (x S products, or the coil sum and conj(S)) plus two cosets of transforms.
+// Compared with the Stockham passes in LDS this moves each element through LDS about
+// three times instead of about ten.
+constexpr int kXYW = 512;
+
+template <int NB> __global__ void __launch_bounds__(kXYW) fwd_xy_w_kernel(FusedArgs a)
+{
+    constexpr int N = 32 * NB, L = 64 * NB, P = 2 * NB, R2 = 4 * NB;
+    constexpr int CW = L * kW;  // one coset block: L rows x 32 columns
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* buf = reinterpret_cast<float2*>(smem_raw);  // [2][CW]
+    float2* twL = buf + 2 * CW;
+    float2* tw32 = twL + L;
+    const int t = threadIdx.x;
+    const int b = t & 31;   // x step: lane of the half-wave (x index b + 32 j)
+    const int yl = t >> 5;  // x step: rows yl + 16 k
+    const int m = t & 31;   // y steps: column (x frequency P m + s)
+    const int r = t >> 5;   // y steps: y1 (step 1) or k1 (step 2)
+    const int z = blockIdx.x;
+    const int pl = blockIdx.y;
+    const int p = a.p0 + pl;
+    for (int i = t; i < L; i += kXYW) twL[i] = a.twL[i];
+    if (t < 32) tw32[t] = a.tw32[t];
+    const int64_t plane = (int64_t)N * N;
+    const float2* xp = a.x + ((int64_t)p * a.N0 + z) * plane;
+    lds_barrier();
+
+    for (int ci = 0; ci < a.ccount; ++ci) {
+        const int coil = a.coils[a.c0 + ci];
+        const float2* sp = a.smaps + ((int64_t)coil * a.N0 + z) * plane;
+        float2 v[2 * NB][NB];
+#pragma unroll
+        for (int k = 0; k < 2 * NB; ++k)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int64_t idx = (int64_t)(yl + 16 * k) * N + b + 32 * j;
+                v[k][j] = cmul(xp[idx], sp[idx]);
+            }
+        float2* Tp = a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L;
+        for (int s0 = 0; s0 < P; s0 += 2) {
+            // x step: for coset s the 32 frequencies P m + s of each row, written as rows
+            // y < N of the coset block (column m)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                const int s = s0 + cc;
+                const float2 w1 = twL[(s * b) % L];
+                const float2 w32 = twL[(s * 32) % L];
+#pragma unroll
+                for (int k = 0; k < 2 * NB; ++k) {
+                    float2 acc = v[k][NB - 1];
+#pragma unroll
+                    for (int j = NB - 2; j >= 0; --j) acc = cadd(cmul(acc, w32), v[k][j]);
+                    acc = cmul(acc, w1);
+                    acc = dft32_fwd(acc, b, tw32);
+                    buf[cc * CW + (yl + 16 * k) * kW + brev5(b)] = acc;
+                }
+            }
+            lds_barrier();
+            // y step 1 (thread: column m, y1 = r, both cosets): DFT over y2 (rows >= N
+            // are zero), twiddle wL^{y1 k1}. A thread reads rows r + 16 y2 and writes rows
+            // 16 k1 + r of its own column: the same rows (mod 16), touched by no other
+            // thread, so the transform is in place without a barrier.
+#pragma unroll 1
+            for (int cs = 0; cs < 2; ++cs) {
+                float2 u[16];
+#pragma unroll
+                for (int y2 = 0; y2 < R2; ++y2)
+                    u[y2] = y2 < R2 / 2 ? buf[cs * CW + (r + 16 * y2) * kW + m] : float2{0.f, 0.f};
+                dftn<R2, -1>(u);
+#pragma unroll
+                for (int k1 = 1; k1 < R2; ++k1) u[k1] = cmul(u[k1], twL[(r * k1) % L]);
+#pragma unroll
+                for (int k1 = 0; k1 < R2; ++k1) buf[cs * CW + (k1 * 16 + r) * kW + m] = u[k1];
+            }
+            lds_barrier();
+            // y step 2 (thread: column m, k1 = r < R2): DFT over y1 -> k2,
+            // ky = k1 + R2 k2, straight to HBM
+            if (r < R2) {
+#pragma unroll 1
+                for (int cs = 0; cs < 2; ++cs) {
+                    float2 u[16];
+#pragma unroll
+                    for (int y1 = 0; y1 < 16; ++y1) u[y1] = buf[cs * CW + (r * 16 + y1) * kW + m];
+                    dft16<-1>(u);
+                    float2* dst = Tp + (int64_t)(s0 + cs) * CW + m;
+#pragma unroll
+                    for (int k2 = 0; k2 < 16; ++k2) dst[(r + R2 * k2) * kW] = u[k2];
+                }
+            }
+            lds_barrier();
+        }
+    }
+}
+
+template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedArgs a)
+{
+    constexpr int N = 32 * NB, L = 64 * NB, P = 2 * NB, R2 = 4 * NB;
+    constexpr int CW = L * kW;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* buf = reinterpret_cast<float2*>(smem_raw);  // [2][16 y1][R2 k1][32 m]
+    float2* twL = buf + 2 * CW;
+    float2* tw32 = twL + L;
+    const int t = threadIdx.x;
+    const int b = t & 31;   // step 2: lane (x index b + 32 j after the inverse DFT)
+    const int yl = t >> 5;  // step 2: pixel rows yl + 16 k, i.e. y1 = yl, y2 = k
+    const int m = t & 31;   // step 1: column
+    const int r = t >> 5;   // step 1: k1 (< R2)
+    const int z = blockIdx.x;
+    const int pl = blockIdx.y;
+    const int p = a.p0 + pl;
+    for (int i = t; i < L; i += kXYW) twL[i] = a.twL[i];
+    if (t < 32) tw32[t] = a.tw32[t];
+    const int64_t plane = (int64_t)N * N;
+    float2 acc[2 * NB][NB];
+#pragma unroll
+    for (int k = 0; k < 2 * NB; ++k)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[k][j] = float2{0.f, 0.f};
+    lds_barrier();
+
+    for (int ci = 0; ci < a.ccount; ++ci) {
+        const int coil = a.coils[a.c0 + ci];
+        const float2* sp = a.smaps + ((int64_t)coil * a.N0 + z) * plane;
+        float2 sv[2 * NB][NB];
+#pragma unroll
+        for (int k = 0; k < 2 * NB; ++k)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const float2 q = sp[(yl + 16 * k) * N + b + 32 * j];
+                sv[k][j] = float2{q.x, -q.y};
+            }
+        const float2* Tp = a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L;
+        for (int s0 = 0; s0 < P; s0 += 2) {
+            // step 1 (thread: column m, k1 = r, both cosets): inverse DFT over k2 of
+            // X[k1 + R2 k2] -> y1, twiddle wL^{-y1 k1}, into LDS as [y1][k1][m]
+            if (r < R2) {
+#pragma unroll 1
+                for (int cs = 0; cs < 2; ++cs) {
+                    float2 u[16];
+                    const float2* src = Tp + (int64_t)(s0 + cs) * CW + m;
+#pragma unroll
+                    for (int k2 = 0; k2 < 16; ++k2) u[k2] = src[(r + R2 * k2) * kW];
+                    dft16<1>(u);
+#pragma unroll
+                    for (int q = 1; q < 16; ++q) u[q] = cmulc(twL[(q * r) % L], u[q]);
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) buf[cs * CW + (q * R2 + r) * kW + m] = u[q];
+                }
+            }
+            lds_barrier();
+            // step 2 (thread: rows yl + 16 k, lane b): inverse DFT over k1 -> y2 = k for
+            // column m = brev5(b); inverse 32-point DFT across lanes; coset twiddle,
+            // conj(S) multiply, coil sum
+#pragma unroll 1
+            for (int cc = 0; cc < 2; ++cc) {
+                const int s = s0 + cc;
+                float2 u[R2];
+#pragma unroll
+                for (int k1 = 0; k1 < R2; ++k1) u[k1] = buf[cc * CW + (yl * R2 + k1) * kW + brev5(b)];
+                dftn<R2, 1>(u);
+                float2 w[NB];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) w[j] = twid<1>(twL, (s * (b + 32 * j)) % L);
+#pragma unroll
+                for (int k = 0; k < 2 * NB; ++k) {
+                    const float2 val = dft32_inv_bitrev_in(u[k], b, tw32);
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) acc[k][j] = cadd(acc[k][j], cmul(sv[k][j], cmul(w[j], val)));
+                }
+            }
+            lds_barrier();
+        }
+    }
+    float2* op = a.out + ((int64_t)p * a.N0 + z) * plane;
+    const float2* sb = a.sub ? a.sub + ((int64_t)p * a.N0 + z) * plane : nullptr;
+#pragma unroll
+    for (int k = 0; k < 2 * NB; ++k)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int64_t idx = (int64_t)(yl + 16 * k) * N + b + 32 * j;
+            float2 rr = acc[k][j];
+            if (a.accumulate) rr = cadd(rr, op[idx]);
+            else if (sb) rr = csub(rr, sb[idx]);
+            op[idx] = rr;
+        }
+}
+
 // z-axis pass for L0 = 256 (N0 = 128) as a 16 x 16 four-step FFT with the data in
 // registers: each of 256 threads owns one (column, group) pair of a 16-column bundle,
 // does 16-point DFTs in registers and meets the others through one LDS transpose per
@@ -710,6 +940,12 @@ __global__ void permute_kernel_kernel(const float* in, float* out, int L0, int L
 
 }  // namespace
 
+namespace {
+int g_xy_variant = 1;
+}
+
+void sense_fused_set_xy_variant(int v) { g_xy_variant = v; }
+
 bool sense_fused_supported(const int64_t* n3)
 {
     return n3[1] == n3[2] && n3[1] % 32 == 0 && n3[1] >= 32 && n3[1] <= 128 && fft_size_supported((int)(2 * n3[0])) &&
@@ -769,6 +1005,12 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
     if (zct) allow_lds(zct, smem_zct);
     auto fwd = NB == 1 ? fwd_xy_kernel<1> : NB == 2 ? fwd_xy_kernel<2> : NB == 3 ? fwd_xy_kernel<3> : fwd_xy_kernel<4>;
     auto inv = NB == 1 ? inv_xy_kernel<1> : NB == 2 ? inv_xy_kernel<2> : NB == 3 ? inv_xy_kernel<3> : inv_xy_kernel<4>;
+    int nxy = kXY;
+    if (g_xy_variant == 1) {
+        fwd = NB == 1 ? fwd_xy_w_kernel<1> : NB == 2 ? fwd_xy_w_kernel<2> : NB == 3 ? fwd_xy_w_kernel<3> : fwd_xy_w_kernel<4>;
+        inv = NB == 1 ? inv_xy_w_kernel<1> : NB == 2 ? inv_xy_w_kernel<2> : NB == 3 ? inv_xy_w_kernel<3> : inv_xy_w_kernel<4>;
+        nxy = kXYW;
+    }
     allow_lds(fwd, smem_xy);
     allow_lds(inv, smem_xy);
     allow_lds(z_conv_kernel, smem_z);
@@ -781,7 +1023,7 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
             a.accumulate = c0 > 0 ? 1 : 0;
             {
                 KernelTimer t("sense_fwd_xy", s);
-                hipLaunchKernelGGL(fwd, dim3(N0, np), dim3(kXY), smem_xy, s, a);
+                hipLaunchKernelGGL(fwd, dim3(N0, np), dim3(nxy), smem_xy, s, a);
                 check_launch("fwd_xy_kernel");
             }
             for (int pl = 0; pl < np; ++pl) {
@@ -799,7 +1041,7 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
                 check_launch("z_conv_kernel");
             }
             KernelTimer t("sense_inv_xy", s);
-            hipLaunchKernelGGL(inv, dim3(N0, np), dim3(kXY), smem_xy, s, a);
+            hipLaunchKernelGGL(inv, dim3(N0, np), dim3(nxy), smem_xy, s, a);
             check_launch("inv_xy_kernel");
         }
     }

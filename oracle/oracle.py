@@ -396,6 +396,70 @@ def conjugate_gradient(A: Callable[[np.ndarray], np.ndarray], b: np.ndarray, x: 
     return x
 
 
+def power_iteration(A: Callable[[np.ndarray], np.ndarray], x0: np.ndarray, iters: int) -> float:
+    """MaxEig (python/python/hastypy/base/opalg.py:647-674; op::PowerIteration,
+    lib/op/opalgs.cpp:29-44): per iteration y = A x, lambda = |y|_2, x = y / lambda."""
+    x = np.array(x0, dtype=np.complex128)
+    lam = float("inf")
+    for _ in range(iters):
+        y = A(x)
+        lam = float(np.linalg.norm(y))
+        x = y / lam
+    return lam
+
+
+def gradient_descent(gradf: Callable[[np.ndarray], np.ndarray], x0: np.ndarray,
+                     prox: Optional[Callable[[np.ndarray], np.ndarray]], alpha: float, iters: int,
+                     accelerate: bool = True) -> np.ndarray:
+    """GradientDescent.run (python/python/hastypy/base/solvers.py:85-162): per iteration
+    x_old = x; (accelerated: x = z); x = x - alpha grad f(x); x = prox(x); accelerated:
+    t' = (1 + sqrt(1 + 4 t^2)) / 2, z = x + ((t - 1) / t') (x - x_old). The prox takes
+    no step size: ProximalOperator.__call__ drops it (proximal.py:39-40)."""
+    x = np.array(x0, dtype=np.complex128)
+    z = x.copy()
+    t = 1.0
+    for _ in range(iters):
+        x_old = x.copy()
+        if accelerate:
+            x = z
+        x = x - alpha * gradf(x)
+        if prox is not None:
+            x = prox(x)
+        if accelerate:
+            t_old = t
+            t = (1.0 + math.sqrt(1.0 + 4.0 * t_old * t_old)) / 2.0
+            z = x + ((t_old - 1.0) / t) * (x - x_old)
+    return x
+
+
+def five_point_llr(coords_list, smaps, kdata_list, weights_list, x0, stepsize: float, iters: int,
+                   block_strides, block_shape, block_iter: int, thresh: float, soft: bool,
+                   accelerate: bool = True) -> np.ndarray:
+    """FivePointLLR.run with solver 'GD' (python/python/hastypy/base/recon.py:87-205):
+    x0 [F, 5, N...]; the gradient of problem p = 5 f + e is sense_normal_apply with
+    kdata/weights (BatchedSenseNormal on the [F*5, 1, N...] view); the prox is
+    normal_3d_blocks_svt on the [F, 5, N...] view."""
+    F = x0.shape[0]
+    sp = x0.shape[2:]
+    coils = list(range(smaps.shape[0]))
+
+    def grad(xg):
+        out = np.empty_like(xg)
+        for p in range(F * 5):
+            w = None if weights_list is None else weights_list[p]
+            kd = None if kdata_list is None else kdata_list[p][0]
+            out[p] = sense_normal_apply(coords_list[p], xg[p], smaps, coils, kdata=kd, weights=w)
+        return out
+
+    def prox(xg):
+        return normal_3d_blocks_svt(xg.reshape((F, 5) + sp), block_strides, block_shape, block_iter, thresh,
+                                    soft).reshape((F * 5, 1) + sp)
+
+    x = gradient_descent(grad, np.asarray(x0, dtype=np.complex128).reshape((F * 5, 1) + sp), prox, stepsize,
+                         iters, accelerate)
+    return x.reshape((F, 5) + sp)
+
+
 # ---------------------------------------------------------------------------------------
 # SVT / LLR (lib/op/svt.cpp, lib/op/svt.ixx)
 # ---------------------------------------------------------------------------------------
