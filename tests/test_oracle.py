@@ -218,3 +218,40 @@ def test_normal_blocks_svt_sequential_semantics():
     out = O.normal_3d_blocks_svt(x, (4, 4, 4), (4, 4, 4), 1, 1e6, True)
     assert np.abs(out[:, :, :4, :4, :4]).max() == 0.0
     assert np.array_equal(out[:, :, 4:, :, :], x[:, :, 4:, :, :])
+
+
+def test_power_iteration_converges_to_largest_eigenvalue():
+    """MaxEig: on a Hermitian PSD matrix with a spectral gap the estimate approaches the
+    largest eigenvalue (numpy eigvalsh) from below; a slip (returning |x|, normalising
+    before applying A) fails."""
+    Q, _ = np.linalg.qr(rnd((12, 12)))
+    lam = np.array([5.0, 2.0] + [1.0] * 10)
+    A = (Q * lam) @ Q.conj().T
+    x0 = rnd(12)
+    est = O.power_iteration(lambda v: A @ v, x0, 40)
+    assert abs(est - 5.0) < 1e-9
+    assert O.power_iteration(lambda v: A @ v, x0, 3) <= 5.0 + 1e-12
+
+
+def test_gradient_descent_lasso_orthonormal_closed_form():
+    """GD / FISTA with a soft-threshold prox on 0.5|Q x - b|^2 + lam |x|_1, Q unitary,
+    step 1: the minimiser is soft(Q^H b, lam) and both variants reach it and stay."""
+    Q, _ = np.linalg.qr(rnd((10, 10)))
+    b = rnd(10)
+    lam = 0.7
+
+    def soft(v, t=lam):
+        return np.exp(1j * np.angle(v)) * np.maximum(np.abs(v) - t, 0.0)
+
+    grad = lambda v: Q.conj().T @ (Q @ v - b)  # noqa: E731
+    xs = soft(Q.conj().T @ b)
+    for acc in (False, True):
+        x = O.gradient_descent(grad, rnd(10), soft, 1.0, 5, accelerate=acc)
+        assert np.abs(x - xs).max() < 1e-12
+    # without a prox, FISTA on an ill-conditioned least-squares problem converges to lstsq
+    r2 = np.random.default_rng(7)
+    A = (r2.standard_normal((30, 8)) + 1j * r2.standard_normal((30, 8))) @ np.diag(np.linspace(1.0, 0.5, 8))
+    bb = r2.standard_normal(30) + 1j * r2.standard_normal(30)
+    L = np.linalg.eigvalsh(A.conj().T @ A).max()
+    x = O.gradient_descent(lambda v: A.conj().T @ (A @ v - bb), np.zeros(8), None, 1.0 / L, 2000, accelerate=True)
+    assert np.abs(x - np.linalg.lstsq(A, bb, rcond=None)[0]).max() < 1e-8
