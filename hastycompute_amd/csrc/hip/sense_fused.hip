@@ -95,6 +95,83 @@ __device__ __forceinline__ float2 dft32_inv_bitrev_in(float2 v, int b, const flo
     return v;
 }
 
+// Value of lane (lane ^ H) within the half-wave, without LDS traffic where the DPP
+// forms reach: quad_perm for H = 1, 2, row_ror:8 for H = 8 (xor 8 inside a 16-lane row),
+// ds_swizzle (bitmask mode, xor_mask = H) for H = 4, 16.
+template <int H> __device__ __forceinline__ float xlane(float v)
+{
+    const int iv = __builtin_bit_cast(int, v);
+    int r;
+    if constexpr (H == 1) r = __builtin_amdgcn_update_dpp(0, iv, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    else if constexpr (H == 2) r = __builtin_amdgcn_update_dpp(0, iv, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    else if constexpr (H == 8) r = __builtin_amdgcn_update_dpp(0, iv, 0x128, 0xF, 0xF, false); // row_ror:8
+    else r = __builtin_amdgcn_ds_swizzle(iv, 0x1F | (H << 10));
+    return __builtin_bit_cast(float, r);
+}
+template <int H> __device__ __forceinline__ float2 xlane2(float2 v) { return float2{xlane<H>(v.x), xlane<H>(v.y)}; }
+
+// Per-lane stage twiddles of the 32-point lane DFTs (index i: stage with h = 16 >> i):
+//  fwd[i]: exp(-2 pi i (b mod h) / 2h) on the upper lanes (b & h), 1 on the lower ones;
+//  inv[i]: exp(+2 pi i (b mod h') / 2h') with h' = 1 << i (both lanes of a pair).
+struct LaneTw32 {
+    float2 fwd[5];
+    float2 inv[5];
+};
+__device__ __forceinline__ LaneTw32 lane_tw32(int b, const float2* tw32)
+{
+    LaneTw32 t;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const int h = 16 >> i;
+        t.fwd[i] = (b & h) ? tw32[(b & (h - 1)) * (16 / h)] : float2{1.f, 0.f};
+        const int hi = 1 << i;
+        const float2 w = tw32[(b & (hi - 1)) * (16 / hi)];
+        t.inv[i] = float2{w.x, -w.y};
+    }
+    return t;
+}
+
+// dft32_fwd without branches or twiddle loads: t = o + v (lower) or o - v (upper), then
+// times the lane's stage twiddle.
+__device__ __forceinline__ float2 dft32_fwd_r(float2 v, int b, const float2 (&tw)[5])
+{
+#define HASTY_DIF_STAGE(I, H)                                        \
+    {                                                                \
+        const float2 o = xlane2<H>(v);                               \
+        const bool up = (b & (H)) != 0;                              \
+        const float2 t = float2{up ? o.x - v.x : o.x + v.x, up ? o.y - v.y : o.y + v.y}; \
+        v = cmul(t, tw[I]);                                          \
+    }
+    HASTY_DIF_STAGE(0, 16)
+    HASTY_DIF_STAGE(1, 8)
+    HASTY_DIF_STAGE(2, 4)
+    HASTY_DIF_STAGE(3, 2)
+    HASTY_DIF_STAGE(4, 1)
+#undef HASTY_DIF_STAGE
+    return v;
+}
+
+// dft32_inv_bitrev_in without branches or twiddle loads: m = w * (upper ? v : o);
+// v = (upper ? o : v) -/+ m.
+__device__ __forceinline__ float2 dft32_inv_r(float2 v, int b, const float2 (&tw)[5])
+{
+#define HASTY_DIT_STAGE(I, H)                                        \
+    {                                                                \
+        const float2 o = xlane2<H>(v);                               \
+        const bool up = (b & (H)) != 0;                              \
+        const float2 m = cmul(tw[I], up ? v : o);                    \
+        const float2 q = up ? o : v;                                 \
+        v = float2{up ? q.x - m.x : q.x + m.x, up ? q.y - m.y : q.y + m.y}; \
+    }
+    HASTY_DIT_STAGE(0, 1)
+    HASTY_DIT_STAGE(1, 2)
+    HASTY_DIT_STAGE(2, 4)
+    HASTY_DIT_STAGE(3, 8)
+    HASTY_DIT_STAGE(4, 16)
+#undef HASTY_DIT_STAGE
+    return v;
+}
+
 template <int RAD, int S> __device__ __forceinline__ void dftR(float2* v);
 
 template <int S> __device__ __forceinline__ float2 mis(float2 z)
@@ -676,6 +753,7 @@ template <int NB> __global__ void __launch_bounds__(kXYW) fwd_xy_w_kernel(FusedA
     if (t < 32) tw32[t] = a.tw32[t];
     const int64_t plane = (int64_t)N * N;
     const float2* xp = a.x + ((int64_t)p * a.N0 + z) * plane;
+    const LaneTw32 lt = lane_tw32(b, a.tw32);
     lds_barrier();
 
     for (int ci = 0; ci < a.ccount; ++ci) {
@@ -704,7 +782,7 @@ template <int NB> __global__ void __launch_bounds__(kXYW) fwd_xy_w_kernel(FusedA
 #pragma unroll
                     for (int j = NB - 2; j >= 0; --j) acc = cadd(cmul(acc, w32), v[k][j]);
                     acc = cmul(acc, w1);
-                    acc = dft32_fwd(acc, b, tw32);
+                    acc = dft32_fwd_r(acc, b, lt.fwd);
                     buf[cc * CW + (yl + 16 * k) * kW + brev5(b)] = acc;
                 }
             }
@@ -764,6 +842,7 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
     for (int i = t; i < L; i += kXYW) twL[i] = a.twL[i];
     if (t < 32) tw32[t] = a.tw32[t];
     const int64_t plane = (int64_t)N * N;
+    const LaneTw32 lt = lane_tw32(b, a.tw32);
     float2 acc[2 * NB][NB];
 #pragma unroll
     for (int k = 0; k < 2 * NB; ++k)
@@ -816,7 +895,7 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
                 for (int j = 0; j < NB; ++j) w[j] = twid<1>(twL, (s * (b + 32 * j)) % L);
 #pragma unroll
                 for (int k = 0; k < 2 * NB; ++k) {
-                    const float2 val = dft32_inv_bitrev_in(u[k], b, tw32);
+                    const float2 val = dft32_inv_r(u[k], b, lt.inv);
 #pragma unroll
                     for (int j = 0; j < NB; ++j) acc[k][j] = cadd(acc[k][j], cmul(sv[k][j], cmul(w[j], val)));
                 }
