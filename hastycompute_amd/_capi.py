@@ -41,6 +41,8 @@ _SIGS = {
     "hasty_sense_normal_prepare": ([c_vp, c_vp, c_i64, P_i64, ctypes.POINTER(c_int), c_vp], c_int),
     "hasty_sense_normal_toeplitz": ([c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_int, c_int, P_i32, c_vp, c_int, P_i64,
                                      c_vp, c_sz, c_int, c_vp], c_int),
+    "hasty_sense_normal_toeplitz_ex": ([c_vp, c_vp, c_vp, c_i64, c_vp, c_int, c_int, P_i32, c_vp, c_vp, c_int, c_int,
+                                        P_i32, c_vp, c_int, P_i64, c_vp, c_sz, c_int, c_vp], c_int),
     "hasty_set_option": ([ctypes.c_char_p, c_i64], c_int),
     "hasty_timing_enable": ([c_int], c_int),
     "hasty_timing_entries": ([], c_int),

@@ -266,21 +266,38 @@ int hasty_sense_normal_prepare(const float* natural, float* prepared, int64_t nk
     });
 }
 
-int hasty_sense_normal_toeplitz(const void* x, void* out, const void* smaps, const int32_t* coils, int ncoil,
-                                const float* kernels, int nkernels, int layout, const int32_t* kernel_index,
-                                const void* sub, int nproblems, const int64_t* nmodes3, void* workspace,
-                                size_t workspace_bytes, int coil_batch, void* stream)
+int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, int64_t nsmaps, const int32_t* coils,
+                                   int ncoil, int coil_problem_stride, const int32_t* problem_ncoil_host,
+                                   const int32_t* problem_ncoil_dev, const float* kernels, int nkernels, int layout,
+                                   const int32_t* kernel_index, const void* sub, int nproblems,
+                                   const int64_t* nmodes3, void* workspace, size_t workspace_bytes, int coil_batch,
+                                   void* stream)
 {
     return guarded([&] {
         HASTY_REQUIRE(x && out && smaps && coils && kernels && kernel_index && nmodes3 && workspace, "null argument");
         HASTY_REQUIRE(nproblems >= 1, "nproblems must be >= 1");
+        HASTY_REQUIRE(coil_problem_stride >= 0, "coil_problem_stride must be >= 0");
+        HASTY_REQUIRE(nsmaps < 0 || (int64_t)(nproblems - 1) * coil_problem_stride < nsmaps,
+                      "coil_problem_stride reaches past the coil maps");
+        HASTY_REQUIRE(!problem_ncoil_dev || problem_ncoil_host, "give the host copy of the per-problem coil counts");
         for (int p = 0; p < nproblems; ++p)
             HASTY_REQUIRE(kernel_index[p] >= 0 && kernel_index[p] < nkernels, "kernel index out of range");
         for (int d = 0; d < 3; ++d) HASTY_REQUIRE(nmodes3[d] >= 1, "nmodes must be >= 1");
         sense_normal_toeplitz((const float2*)x, (float2*)out, (const float2*)smaps, coils, ncoil, kernels, layout,
                               kernel_index, (const float2*)sub, nproblems, nmodes3, workspace, workspace_bytes,
-                              coil_batch, as_stream(stream));
+                              coil_batch, as_stream(stream), coil_problem_stride, problem_ncoil_host,
+                              problem_ncoil_dev);
     });
+}
+
+int hasty_sense_normal_toeplitz(const void* x, void* out, const void* smaps, const int32_t* coils, int ncoil,
+                                const float* kernels, int nkernels, int layout, const int32_t* kernel_index,
+                                const void* sub, int nproblems, const int64_t* nmodes3, void* workspace,
+                                size_t workspace_bytes, int coil_batch, void* stream)
+{
+    return hasty_sense_normal_toeplitz_ex(x, out, smaps, -1, coils, ncoil, 0, nullptr, nullptr, kernels, nkernels,
+                                          layout, kernel_index, sub, nproblems, nmodes3, workspace, workspace_bytes,
+                                          coil_batch, stream);
 }
 
 int hasty_set_option(const char* name, int64_t value)

@@ -415,9 +415,20 @@ struct FusedArgs {
     int p0;                // first problem of the chunk
     int c0, ccount;        // coil range of this launch
     int accumulate;        // out += (later coil chunks)
+    int coil_pstride;      // problem p uses coil-map rows coils[i] + p * coil_pstride
+    const int* pcoils;     // optional [P] device: problem p uses its first pcoils[p] coils
     uint32_t codeL, codeL0;
     int npassL, npassL0;
 };
+
+// Coils problem p runs through in this launch (the coil chunk [c0, c0 + ccount) cut to
+// the problem's own count). The grid workspace keeps the stride of ccount coils.
+__device__ __forceinline__ int coil_count(const FusedArgs& a, int p)
+{
+    if (!a.pcoils) return a.ccount;
+    const int n = a.pcoils[p] - a.c0;
+    return n < 0 ? 0 : (n < a.ccount ? n : a.ccount);
+}
 
 template <int NB> __global__ void __launch_bounds__(kXY) fwd_xy_kernel(FusedArgs a)
 {
@@ -440,8 +451,9 @@ template <int NB> __global__ void __launch_bounds__(kXY) fwd_xy_kernel(FusedArgs
     const float2* xp = a.x + ((int64_t)p * a.N0 + z) * plane;
     lds_barrier();
 
-    for (int ci = 0; ci < a.ccount; ++ci) {
-        const int coil = a.coils[a.c0 + ci];
+    const int ncl = coil_count(a, a.p0 + pl);
+    for (int ci = 0; ci < ncl; ++ci) {
+        const int coil = a.coils[a.c0 + ci] + p * a.coil_pstride;
         const float2* sp = a.smaps + ((int64_t)coil * a.N0 + z) * plane;
         float2 v[NB][NB];
 #pragma unroll
@@ -497,8 +509,9 @@ template <int NB> __global__ void __launch_bounds__(kXY) inv_xy_kernel(FusedArgs
         for (int j = 0; j < NB; ++j) acc[k][j] = float2{0.f, 0.f};
     lds_barrier();
 
-    for (int ci = 0; ci < a.ccount; ++ci) {
-        const int coil = a.coils[a.c0 + ci];
+    const int ncl = coil_count(a, a.p0 + pl);
+    for (int ci = 0; ci < ncl; ++ci) {
+        const int coil = a.coils[a.c0 + ci] + p * a.coil_pstride;
         const float2* sp = a.smaps + ((int64_t)coil * a.N0 + z) * plane;
         float2 sv[NB][NB];
 #pragma unroll
@@ -567,7 +580,8 @@ __global__ void __launch_bounds__(kZ) z_conv_kernel(FusedArgs a, int LL, int pl,
         kv[i] = e < nel ? K[(int64_t)(e / kW) * LL + pos0 + (e % kW)] : 0.f;
     }
     lds_barrier();
-    for (int ci = 0; ci < a.ccount; ++ci) {
+    const int ncl = coil_count(a, a.p0 + pl);
+    for (int ci = 0; ci < ncl; ++ci) {
         float2* Tc = a.T + ((int64_t)pl * a.ccount + ci) * a.N0 * (int64_t)LL + pos0;
         for (int e = t; e < a.N0 * kW; e += kZ) bufA[e] = Tc[(int64_t)(e / kW) * LL + (e % kW)];
         lds_barrier();
@@ -621,7 +635,8 @@ __global__ void __launch_bounds__(kZT) z_conv_ct_kernel(FusedArgs a, int LL, int
             if (e < N0 * kZC) pf[i] = T0[(int64_t)(e / kZC) * LL + (e % kZC)];
         }
     }
-    for (int ci = 0; ci < a.ccount; ++ci) {
+    const int ncl = coil_count(a, a.p0 + pl);
+    for (int ci = 0; ci < ncl; ++ci) {
         float2* Tc = a.T + ((int64_t)pl * a.ccount + ci) * N0 * (int64_t)LL + pos0;
 #pragma unroll
         for (int i = 0; i < PFZ; ++i) {
@@ -629,7 +644,7 @@ __global__ void __launch_bounds__(kZT) z_conv_ct_kernel(FusedArgs a, int LL, int
             if (e < N0 * kZC) buf[e] = pf[i];
         }
         lds_barrier();
-        if (ci + 1 < a.ccount) {
+        if (ci + 1 < ncl) {
             const float2* Tn = Tc + (int64_t)N0 * LL;
 #pragma unroll
             for (int i = 0; i < PFZ; ++i) {
@@ -756,8 +771,9 @@ template <int NB> __global__ void __launch_bounds__(kXYW) fwd_xy_w_kernel(FusedA
     const LaneTw32 lt = lane_tw32(b, a.tw32);
     lds_barrier();
 
-    for (int ci = 0; ci < a.ccount; ++ci) {
-        const int coil = a.coils[a.c0 + ci];
+    const int ncl = coil_count(a, a.p0 + pl);
+    for (int ci = 0; ci < ncl; ++ci) {
+        const int coil = a.coils[a.c0 + ci] + p * a.coil_pstride;
         const float2* sp = a.smaps + ((int64_t)coil * a.N0 + z) * plane;
         float2 v[2 * NB][NB];
 #pragma unroll
@@ -850,8 +866,9 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
         for (int j = 0; j < NB; ++j) acc[k][j] = float2{0.f, 0.f};
     lds_barrier();
 
-    for (int ci = 0; ci < a.ccount; ++ci) {
-        const int coil = a.coils[a.c0 + ci];
+    const int ncl = coil_count(a, a.p0 + pl);
+    for (int ci = 0; ci < ncl; ++ci) {
+        const int coil = a.coils[a.c0 + ci] + p * a.coil_pstride;
         const float2* sp = a.smaps + ((int64_t)coil * a.N0 + z) * plane;
         float2 sv[2 * NB][NB];
 #pragma unroll
@@ -944,14 +961,15 @@ __global__ void __launch_bounds__(kR16T) z_conv_r16_kernel(FusedArgs a, int LL, 
 #pragma unroll
     for (int n1 = 0; n1 < 8; ++n1) pf[n1] = T0[(int64_t)(16 * n1 + g) * LL];
 
-    for (int ci = 0; ci < a.ccount; ++ci) {
+    const int ncl = coil_count(a, a.p0 + pl);
+    for (int ci = 0; ci < ncl; ++ci) {
         float2* Tc = a.T + ((int64_t)pl * a.ccount + ci) * 128 * (int64_t)LL + pos0 + c;
         float2 v[16];
 #pragma unroll
         for (int n1 = 0; n1 < 8; ++n1) v[n1] = pf[n1];
 #pragma unroll
         for (int n1 = 8; n1 < 16; ++n1) v[n1] = float2{0.f, 0.f};
-        if (ci + 1 < a.ccount) {
+        if (ci + 1 < ncl) {
             const float2* Tn = Tc + (int64_t)128 * LL;
 #pragma unroll
             for (int n1 = 0; n1 < 8; ++n1) pf[n1] = Tn[(int64_t)(16 * n1 + g) * LL];
@@ -1049,7 +1067,8 @@ size_t sense_fused_workspace_bytes(const int64_t* n3, int pchunk, int ccount)
 // kernels: permuted layout (sense_fused_permute_kernel); kidx: host int [P].
 void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const int* coils_dev, int ncoil,
                        const float* kernels, const int* kidx, const float2* sub, int P, const int64_t* n3,
-                       void* ws, size_t ws_bytes, int pchunk, int cchunk, hipStream_t s)
+                       void* ws, size_t ws_bytes, int pchunk, int cchunk, hipStream_t s, int coil_pstride,
+                       const int* pcoils_dev)
 {
     HASTY_REQUIRE(sense_fused_supported(n3), "fused SENSE normal operator needs N1 == N2 in {32, 64, 96, 128}");
     const int N0 = (int)n3[0], N = (int)n3[1];
@@ -1064,6 +1083,8 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
     a.sub = sub;
     a.smaps = smaps;
     a.coils = coils_dev;
+    a.coil_pstride = coil_pstride;
+    a.pcoils = pcoils_dev;
     a.kernels = kernels;
     a.T = reinterpret_cast<float2*>(ws);
     a.twL = twiddles<float>(L);

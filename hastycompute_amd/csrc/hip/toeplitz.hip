@@ -304,15 +304,21 @@ size_t sense_normal_workspace_bytes(const int64_t* n3, int coil_batch, int probl
 
 void sense_normal_toeplitz(const float2* x, float2* out, const float2* smaps, const int* coils_dev, int ncoil,
                            const float* kernels, int layout, const int* kidx, const float2* sub, int P,
-                           const int64_t* n3, void* workspace, size_t ws_bytes, int coil_batch, hipStream_t s)
+                           const int64_t* n3, void* workspace, size_t ws_bytes, int coil_batch, hipStream_t s,
+                           int coil_pstride, const int* pcoils_host, const int* pcoils_dev)
 {
     HASTY_REQUIRE(ncoil >= 1 && coil_batch >= 1, "need at least one coil");
+    HASTY_REQUIRE(coil_pstride >= 0, "coil stride must be >= 0");
+    if (pcoils_host)
+        for (int p = 0; p < P; ++p)
+            HASTY_REQUIRE(pcoils_host[p] >= 1 && pcoils_host[p] <= ncoil, "per-problem coil count out of range");
     if (layout == 1) {
+        HASTY_REQUIRE(!pcoils_host || pcoils_dev, "the fused layout needs the device copy of the coil counts");
         const int cb = std::min(coil_batch, ncoil);
         const size_t per = sense_fused_workspace_bytes(n3, 1, cb);
         const int pchunk = (int)std::max<size_t>(1, std::min<size_t>(64, ws_bytes / per));
         sense_fused_apply(x, out, smaps, coils_dev, ncoil, kernels, kidx, sub, P, n3, workspace, ws_bytes, pchunk, cb,
-                          s);
+                          s, coil_pstride, pcoils_host ? pcoils_dev : nullptr);
         return;
     }
     HASTY_REQUIRE(layout == 0, "unknown kernel layout");
@@ -329,14 +335,16 @@ void sense_normal_toeplitz(const float2* x, float2* out, const float2* smaps, co
         const float2* xp = x + p * nvox;
         float2* op = out + p * nvox;
         const float* kp = kernels + (int64_t)kidx[p] * kvox;
-        for (int c0 = 0; c0 < ncoil; c0 += cb) {
-            const int nc = std::min(cb, ncoil - c0);
+        const float2* smaps_p = smaps + (int64_t)p * coil_pstride * nvox;
+        const int ncoil_p = pcoils_host ? pcoils_host[p] : ncoil;
+        for (int c0 = 0; c0 < ncoil_p; c0 += cb) {
+            const int nc = std::min(cb, ncoil_p - c0);
             // K1: x * S_c, pad-FFT along dim 2 -> T0 [c][N0][N1][L2]
             {
                 AxisFft<float> a;
                 a.in = xp;
                 a.ain = LineAddr{N0 * N1, 0, N2, 1, nullptr};
-                a.inmul = smaps;
+                a.inmul = smaps_p;
                 a.amul = LineAddr{N0 * N1, nvox, N2, 1, coils_dev + c0};
                 a.out = T0;
                 a.aout = LineAddr{1, L2, 0, 1, nullptr};
@@ -397,7 +405,7 @@ void sense_normal_toeplitz(const float2* x, float2* out, const float2* smaps, co
                 a.ain = LineAddr{1, L2, 0, 1, nullptr};
                 a.out = op;
                 a.aout = LineAddr{1, N2, 0, 1, nullptr};
-                a.outmul = smaps;
+                a.outmul = smaps_p;
                 a.aomul = LineAddr{N0 * N1, nvox, N2, 1, coils_dev + c0 + c};
                 a.outmul_conj = 1;
                 a.accumulate = (c0 + c) > 0 ? 1 : 0;

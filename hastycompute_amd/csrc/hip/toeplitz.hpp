@@ -36,9 +36,13 @@ int sense_kernel_layout(const int64_t* n3, bool allow_fused);
 void sense_prepare_kernels(const float* natural, float* prepared, int64_t nk, const int64_t* n3, int layout,
                            hipStream_t s);
 size_t sense_normal_workspace_bytes(const int64_t* n3, int coil_batch, int problem_batch);
+// Problem p uses coil-map rows coils[i] + p * coil_pstride (0: all problems share them),
+// for i < pcoils[p] when the per-problem counts are given (host copy for the host-driven
+// generic path, device copy for the fused kernels), else for i < ncoil.
 void sense_normal_toeplitz(const float2* x, float2* out, const float2* smaps, const int* coils_dev, int ncoil,
                            const float* kernels, int layout, const int* kidx_host, const float2* sub, int P,
-                           const int64_t* n3, void* workspace, size_t ws_bytes, int coil_batch, hipStream_t s);
+                           const int64_t* n3, void* workspace, size_t ws_bytes, int coil_batch, hipStream_t s,
+                           int coil_pstride = 0, const int* pcoils_host = nullptr, const int* pcoils_dev = nullptr);
 
 // sense_fused.hip
 bool sense_fused_supported(const int64_t* n3);
@@ -48,6 +52,7 @@ void sense_fused_permute_kernel(const float* natural, float* permuted, const int
 size_t sense_fused_workspace_bytes(const int64_t* n3, int pchunk, int ccount);
 void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const int* coils_dev, int ncoil,
                        const float* kernels, const int* kidx, const float2* sub, int P, const int64_t* n3, void* ws,
-                       size_t ws_bytes, int pchunk, int cchunk, hipStream_t s);
+                       size_t ws_bytes, int pchunk, int cchunk, hipStream_t s, int coil_pstride = 0,
+                       const int* pcoils_dev = nullptr);
 
 }  // namespace hasty_hip

@@ -100,6 +100,20 @@ int hasty_sense_normal_toeplitz(const void* x, void* out, const void* smaps, con
                                 const float* kernels, int nkernels, int layout, const int32_t* kernel_index,
                                 const void* sub, int nproblems, const int64_t* nmodes3, void* workspace,
                                 size_t workspace_bytes, int coil_batch, void* stream);
+/* The same for problems that do not share coil maps or coil counts (the batched
+ * reconstruction server): problem p uses smaps rows coils[i] + p * coil_problem_stride
+ * (a pool of nsmaps rows; nsmaps < 0 skips the range check) for i < problem_ncoil[p]
+ * (1 <= problem_ncoil[p] <= ncoil; given as a HOST array, plus the same values in a
+ * DEVICE array for the fused layout; both NULL: every problem uses all ncoil).
+ * hasty_sense_normal_toeplitz is this call with stride 0 and no counts. The reference
+ * runs one operator per problem with its own maps (batch_sense.cpp:383-450); this lets
+ * problems with different maps and coil counts share launches. */
+int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, int64_t nsmaps, const int32_t* coils,
+                                   int ncoil, int coil_problem_stride, const int32_t* problem_ncoil_host,
+                                   const int32_t* problem_ncoil_dev, const float* kernels, int nkernels, int layout,
+                                   const int32_t* kernel_index, const void* sub, int nproblems,
+                                   const int64_t* nmodes3, void* workspace, size_t workspace_bytes, int coil_batch,
+                                   void* stream);
 
 /* Library options: "sense_fused" (default 1) lets hasty_sense_normal_prepare choose the
  * fused layout. Returns 0, or 1 for an unknown name. */

@@ -100,3 +100,15 @@ def test_product_has_no_oracle_import():
             if f.endswith((".py", ".cpp", ".hpp", ".hip", ".h")):
                 text = open(os.path.join(root, f), errors="ignore").read()
                 assert "import oracle" not in text and "from oracle" not in text, f
+
+
+def test_serving_request_stream_host_logic():
+    """The Poisson request stream is seeded and the footprints follow the stated sizes."""
+    from hastycompute_amd import serving
+
+    a = serving.poisson_requests(20, 2.0, seed=3)
+    b = serving.poisson_requests(20, 2.0, seed=3)
+    assert [(r.n, r.ncoils, r.nspokes, r.arrival) for r in a] == [(r.n, r.ncoils, r.nspokes, r.arrival) for r in b]
+    assert all(r.n in (96, 128, 160) and 8 <= r.ncoils <= 32 and 1000 <= r.nspokes <= 10000 for r in a)
+    assert all(x.arrival < y.arrival for x, y in zip(a, a[1:]))
+    assert serving.resident_bytes(128, 16) == 8 * 16 * 128 ** 3 + 4 * 256 ** 3 + 40 * 128 ** 3
