@@ -30,6 +30,16 @@ template <typename T, typename S> __host__ __device__ __forceinline__ T cscale(T
     return T{a.x * s, a.y * s};
 }
 template <typename T> __host__ __device__ __forceinline__ T cconj(T a) { return T{a.x, -a.y}; }
+// acc + a * b
+template <typename T> __host__ __device__ __forceinline__ T cfma(T acc, T a, T b)
+{
+    return T{acc.x + a.x * b.x - a.y * b.y, acc.y + a.x * b.y + a.y * b.x};
+}
+// acc + conj(a) * b
+template <typename T> __host__ __device__ __forceinline__ T cfma_conj(T acc, T a, T b)
+{
+    return T{acc.x + a.x * b.x + a.y * b.y, acc.y + a.x * b.y - a.y * b.x};
+}
 
 // Error type thrown inside the library; every extern "C" entry point converts it to a
 // nonzero return code and a message readable through hasty_last_error().

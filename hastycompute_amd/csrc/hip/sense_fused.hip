@@ -110,10 +110,17 @@ template <int H> __device__ __forceinline__ float xlane(float v)
 }
 template <int H> __device__ __forceinline__ float2 xlane2(float2 v) { return float2{xlane<H>(v.x), xlane<H>(v.y)}; }
 
-// Per-lane stage twiddles of the 32-point lane DFTs (index i: stage with h = 16 >> i):
-//  fwd[i]: exp(-2 pi i (b mod h) / 2h) on the upper lanes (b & h), 1 on the lower ones;
-//  inv[i]: exp(+2 pi i (b mod h') / 2h') with h' = 1 << i (both lanes of a pair).
+// Per-lane constants of the 32-point lane DFTs. A butterfly stage over lane pairs
+// (b, b ^ h) is written without selects: each lane holds a sign, +1 on the lower lane
+// of its pair and -1 on the upper one, and a twiddle that is 1 where the stage applies
+// none, so every lane runs the same multiply-adds.
+//  sgn[l]: sign of the stage with h = 1 << l;
+//  fwd[i]: exp(-2 pi i (b mod h) / 2h) on the upper lanes (b & h), 1 on the lower ones,
+//          for the forward stage with h = 16 >> i;
+//  inv[i]: exp(+2 pi i (b mod h) / 2h) on the upper lanes, 1 on the lower ones, for the
+//          inverse stage with h = 1 << i.
 struct LaneTw32 {
+    float sgn[5];
     float2 fwd[5];
     float2 inv[5];
 };
@@ -122,52 +129,60 @@ __device__ __forceinline__ LaneTw32 lane_tw32(int b, const float2* tw32)
     LaneTw32 t;
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
+        t.sgn[i] = (b & (1 << i)) ? -1.f : 1.f;
         const int h = 16 >> i;
         t.fwd[i] = (b & h) ? tw32[(b & (h - 1)) * (16 / h)] : float2{1.f, 0.f};
         const int hi = 1 << i;
         const float2 w = tw32[(b & (hi - 1)) * (16 / hi)];
-        t.inv[i] = float2{w.x, -w.y};
+        t.inv[i] = (b & hi) ? float2{w.x, -w.y} : float2{1.f, 0.f};
     }
     return t;
 }
 
-// dft32_fwd without branches or twiddle loads: t = o + v (lower) or o - v (upper), then
-// times the lane's stage twiddle.
-__device__ __forceinline__ float2 dft32_fwd_r(float2 v, int b, const float2 (&tw)[5])
+__device__ __forceinline__ float2 sfma(float s, float2 v, float2 o)
 {
-#define HASTY_DIF_STAGE(I, H)                                        \
-    {                                                                \
-        const float2 o = xlane2<H>(v);                               \
-        const bool up = (b & (H)) != 0;                              \
-        const float2 t = float2{up ? o.x - v.x : o.x + v.x, up ? o.y - v.y : o.y + v.y}; \
-        v = cmul(t, tw[I]);                                          \
-    }
-    HASTY_DIF_STAGE(0, 16)
-    HASTY_DIF_STAGE(1, 8)
-    HASTY_DIF_STAGE(2, 4)
-    HASTY_DIF_STAGE(3, 2)
-    HASTY_DIF_STAGE(4, 1)
-#undef HASTY_DIF_STAGE
-    return v;
+    return float2{fmaf(s, v.x, o.x), fmaf(s, v.y, o.y)};
 }
 
-// dft32_inv_bitrev_in without branches or twiddle loads: m = w * (upper ? v : o);
-// v = (upper ? o : v) -/+ m.
-__device__ __forceinline__ float2 dft32_inv_r(float2 v, int b, const float2 (&tw)[5])
+// Decimation-in-frequency 32-point DFT across the lanes of a half-wave, natural order in,
+// bit-reversed order out. Stage h: t = o + v (lower lane) or o - v (upper lane), then
+// times the lane's twiddle. The last stage (h = 1) has twiddle 1 on every lane.
+__device__ __forceinline__ float2 dft32_fwd_r(float2 v, const LaneTw32& lt)
 {
-#define HASTY_DIT_STAGE(I, H)                                        \
+#define HASTY_DIF_STAGE(I, L)                                        \
     {                                                                \
-        const float2 o = xlane2<H>(v);                               \
-        const bool up = (b & (H)) != 0;                              \
-        const float2 m = cmul(tw[I], up ? v : o);                    \
-        const float2 q = up ? o : v;                                 \
-        v = float2{up ? q.x - m.x : q.x + m.x, up ? q.y - m.y : q.y + m.y}; \
+        const float2 o = xlane2<(1 << L)>(v);                        \
+        v = cmul(sfma(lt.sgn[L], v, o), lt.fwd[I]);                  \
     }
-    HASTY_DIT_STAGE(0, 1)
-    HASTY_DIT_STAGE(1, 2)
-    HASTY_DIT_STAGE(2, 4)
-    HASTY_DIT_STAGE(3, 8)
-    HASTY_DIT_STAGE(4, 16)
+    HASTY_DIF_STAGE(0, 4)
+    HASTY_DIF_STAGE(1, 3)
+    HASTY_DIF_STAGE(2, 2)
+    HASTY_DIF_STAGE(3, 1)
+#undef HASTY_DIF_STAGE
+    const float2 o = xlane2<1>(v);
+    return sfma(lt.sgn[0], v, o);
+}
+
+// Decimation-in-time 32-point inverse DFT across a half-wave, bit-reversed order in,
+// natural order out. Stage h: each lane first scales its value by its twiddle (1 on the
+// lower lane), then v = y + o (lower) or o - y (upper), which is v + w o on the lower
+// lane and v_lower - w v on the upper one. The first stage (h = 1) has twiddle 1.
+__device__ __forceinline__ float2 dft32_inv_r(float2 v, const LaneTw32& lt)
+{
+    {
+        const float2 o = xlane2<1>(v);
+        v = sfma(lt.sgn[0], v, o);
+    }
+#define HASTY_DIT_STAGE(L)                                           \
+    {                                                                \
+        const float2 y = cmul(v, lt.inv[L]);                         \
+        const float2 o = xlane2<(1 << L)>(y);                        \
+        v = sfma(lt.sgn[L], y, o);                                   \
+    }
+    HASTY_DIT_STAGE(1)
+    HASTY_DIT_STAGE(2)
+    HASTY_DIT_STAGE(3)
+    HASTY_DIT_STAGE(4)
 #undef HASTY_DIT_STAGE
     return v;
 }
@@ -798,7 +813,7 @@ template <int NB> __global__ void __launch_bounds__(kXYW) fwd_xy_w_kernel(FusedA
 #pragma unroll
                     for (int j = NB - 2; j >= 0; --j) acc = cadd(cmul(acc, w32), v[k][j]);
                     acc = cmul(acc, w1);
-                    acc = dft32_fwd_r(acc, b, lt.fwd);
+                    acc = dft32_fwd_r(acc, lt);
                     buf[cc * CW + (yl + 16 * k) * kW + brev5(b)] = acc;
                 }
             }
@@ -870,14 +885,12 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
     for (int ci = 0; ci < ncl; ++ci) {
         const int coil = a.coils[a.c0 + ci] + p * a.coil_pstride;
         const float2* sp = a.smaps + ((int64_t)coil * a.N0 + z) * plane;
-        float2 sv[2 * NB][NB];
+        // this coil's image, summed over the cosets; times conj(S) once at the end
+        float2 ic[2 * NB][NB];
 #pragma unroll
         for (int k = 0; k < 2 * NB; ++k)
 #pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                const float2 q = sp[(yl + 16 * k) * N + b + 32 * j];
-                sv[k][j] = float2{q.x, -q.y};
-            }
+            for (int j = 0; j < NB; ++j) ic[k][j] = float2{0.f, 0.f};
         const float2* Tp = a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L;
         for (int s0 = 0; s0 < P; s0 += 2) {
             // step 1 (thread: column m, k1 = r, both cosets): inverse DFT over k2 of
@@ -912,13 +925,18 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
                 for (int j = 0; j < NB; ++j) w[j] = twid<1>(twL, (s * (b + 32 * j)) % L);
 #pragma unroll
                 for (int k = 0; k < 2 * NB; ++k) {
-                    const float2 val = dft32_inv_r(u[k], b, lt.inv);
+                    const float2 val = dft32_inv_r(u[k], lt);
 #pragma unroll
-                    for (int j = 0; j < NB; ++j) acc[k][j] = cadd(acc[k][j], cmul(sv[k][j], cmul(w[j], val)));
+                    for (int j = 0; j < NB; ++j) ic[k][j] = cfma(ic[k][j], w[j], val);
                 }
             }
             lds_barrier();
         }
+#pragma unroll
+        for (int k = 0; k < 2 * NB; ++k)
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                acc[k][j] = cfma_conj(acc[k][j], sp[(yl + 16 * k) * N + b + 32 * j], ic[k][j]);
     }
     float2* op = a.out + ((int64_t)p * a.N0 + z) * plane;
     const float2* sb = a.sub ? a.sub + ((int64_t)p * a.N0 + z) * plane : nullptr;
@@ -1106,7 +1124,7 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
     auto fwd = NB == 1 ? fwd_xy_kernel<1> : NB == 2 ? fwd_xy_kernel<2> : NB == 3 ? fwd_xy_kernel<3> : fwd_xy_kernel<4>;
     auto inv = NB == 1 ? inv_xy_kernel<1> : NB == 2 ? inv_xy_kernel<2> : NB == 3 ? inv_xy_kernel<3> : inv_xy_kernel<4>;
     int nxy = kXY;
-    if (g_xy_variant == 1) {
+    if (g_xy_variant >= 1) {
         fwd = NB == 1 ? fwd_xy_w_kernel<1> : NB == 2 ? fwd_xy_w_kernel<2> : NB == 3 ? fwd_xy_w_kernel<3> : fwd_xy_w_kernel<4>;
         inv = NB == 1 ? inv_xy_w_kernel<1> : NB == 2 ? inv_xy_w_kernel<2> : NB == 3 ? inv_xy_w_kernel<3> : inv_xy_w_kernel<4>;
         nxy = kXYW;

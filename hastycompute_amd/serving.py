@@ -25,6 +25,8 @@ import argparse
 import ctypes
 import json
 import math
+import queue as queue_mod
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import List, Optional
@@ -224,10 +226,34 @@ class ReconServer:
             self.groups[n] = _Group(n, self.cmax, self.device, self.coil_batch, self.problem_batch)
         return self.groups[n]
 
-    def run(self, requests: List[ReconRequest], realtime: bool = True, log=None) -> dict:
+    def _setup_worker(self, inq: "queue_mod.Queue", outq: "queue_mod.Queue"):
+        """Request setup on its own stream (default priority) so admissions overlap the
+        in-flight batches; results go back with an event the main stream waits on."""
+        torch.cuda.set_device(self.device)
+        stream = torch.cuda.Stream(device=self.device)
+        while True:
+            req = inq.get()
+            if req is None:
+                return
+            try:
+                with torch.cuda.stream(stream):
+                    data = req.prepared if req.prepared is not None else self.setup(req)
+                    ev = torch.cuda.Event()
+                    ev.record(stream)
+                outq.put((req, data, ev))
+            except Exception as e:  # surfaced by the main loop
+                outq.put((req, e, None))
+
+    def run(self, requests: List[ReconRequest], realtime: bool = True, log=None,
+            async_setup: Optional[bool] = None) -> dict:
+        """Serve `requests`. realtime: arrivals follow the wall clock (latency in seconds);
+        otherwise one arrival joins per tick (latency in ticks). async_setup (default: with
+        realtime) runs request setup on a worker thread and stream."""
         for r in requests:
             if r.ncoils > self.cmax:
                 raise RuntimeError(f"request {r.rid}: {r.ncoils} coils > server limit {self.cmax}")
+        if async_setup is None:
+            async_setup = realtime
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         queue: List[ReconRequest] = []
         finished: List[ReconRequest] = []
@@ -236,44 +262,79 @@ class ReconServer:
         vclock = [0.0]  # without real time: arrivals are taken in order, one per tick at most
         now = clock if realtime else (lambda: vclock[0])
         ticks, batch_sum, max_seen = 0, 0, 0
+        in_setup = 0
         torch.cuda.reset_peak_memory_stats(self.device)
-        while len(finished) < len(requests):
-            t = now()
-            while pending and (pending[0].arrival <= t or not realtime):
-                queue.append(pending.pop(0))
-                if not realtime:
-                    break
-            active = sum(g.active for g in self.groups.values())
-            while queue and active < self.max_active:
-                req = queue[0]
-                used = torch.cuda.memory_allocated(self.device)
-                if used + self.footprint(req) > self.hbm_budget and active > 0:
-                    break  # wait for memory to free up
-                queue.pop(0)
-                req.t_admit = now()
-                self.admissions.append((req.rid, used, self.footprint(req), active))
-                smaps, kern, b = req.prepared if req.prepared is not None else self.setup(req)
-                self._group(req.n).add(req, smaps, kern, b)
-                del smaps, kern, b
-                active += 1
-                if log:
-                    log(f"t={now():.2f}s admit {req.rid} N={req.n} C={req.ncoils} M={req.samples} active={active}")
-            if active == 0:
-                if realtime and pending:
-                    time.sleep(max(0.0, min(0.05, pending[0].arrival - now())))
-                continue
-            max_seen = max(max_seen, active)
-            for g in self.groups.values():
-                batch_sum += g.active
-                g.step()
-            ticks += 1
-            if not realtime:
-                vclock[0] += 1.0
-            for g in self.groups.values():
-                for req in g.pop_finished(now):
-                    finished.append(req)
+        main_stream = torch.cuda.current_stream(self.device)
+        if async_setup:
+            inq, outq = queue_mod.Queue(), queue_mod.Queue()
+            worker = threading.Thread(target=self._setup_worker, args=(inq, outq), daemon=True)
+            worker.start()
+
+        def join_group(req, data):
+            smaps, kern, b = data
+            self._group(req.n).add(req, smaps, kern, b)
+
+        try:
+            while len(finished) < len(requests):
+                t = now()
+                while pending and (pending[0].arrival <= t or not realtime):
+                    queue.append(pending.pop(0))
+                    if not realtime:
+                        break
+                if async_setup:  # finished setups join their groups
+                    while True:
+                        try:
+                            req, data, ev = outq.get_nowait()
+                        except queue_mod.Empty:
+                            break
+                        in_setup -= 1
+                        if isinstance(data, Exception):
+                            raise data
+                        main_stream.wait_event(ev)
+                        for tns in data:
+                            tns.record_stream(main_stream)
+                        join_group(req, data)
+                        del data
+                active = sum(g.active for g in self.groups.values()) + in_setup
+                while queue and active < self.max_active:
+                    req = queue[0]
+                    used = torch.cuda.memory_allocated(self.device)
+                    if used + self.footprint(req) > self.hbm_budget and active > 0:
+                        break  # wait for memory to free up
+                    queue.pop(0)
+                    req.t_admit = now()
+                    self.admissions.append((req.rid, used, self.footprint(req), active))
+                    if async_setup:
+                        inq.put(req)
+                        in_setup += 1
+                    else:
+                        join_group(req, req.prepared if req.prepared is not None else self.setup(req))
+                    active += 1
                     if log:
-                        log(f"t={now():.2f}s done {req.rid} latency={req.latency:.3f}s")
+                        log(f"t={now():.2f}s admit {req.rid} N={req.n} C={req.ncoils} M={req.samples} "
+                            f"active={active}")
+                if sum(g.active for g in self.groups.values()) == 0:
+                    if async_setup and in_setup:
+                        time.sleep(0.002)
+                    elif realtime and pending:
+                        time.sleep(max(0.0, min(0.05, pending[0].arrival - now())))
+                    continue
+                max_seen = max(max_seen, active)
+                for g in self.groups.values():
+                    batch_sum += g.active
+                    g.step()
+                ticks += 1
+                if not realtime:
+                    vclock[0] += 1.0
+                for g in self.groups.values():
+                    for req in g.pop_finished(now):
+                        finished.append(req)
+                        if log:
+                            log(f"t={now():.2f}s done {req.rid} latency={req.latency:.3f}s")
+        finally:
+            if async_setup:
+                inq.put(None)
+                worker.join()
         torch.cuda.synchronize(self.device)
         makespan = now() - min(r.arrival for r in requests) if realtime else float(ticks)
         lat = np.array([r.latency for r in finished])
