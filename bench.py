@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--coil-batch", type=int, default=16)
     ap.add_argument("--problem-batch", type=int, default=4)
     ap.add_argument("--generic", action="store_true", help="use the generic five-pass path")
+    ap.add_argument("--xy-variant", type=int, default=None,
+                    help="plane kernels of the fused path (library option sense_xy_variant)")
     ap.add_argument("--cg-iters", type=int, default=15)
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--plain", type=int, default=3, help="problems solved by the plain per-frame path (0: skip)")
@@ -313,6 +315,9 @@ def main():
     if args.mode == "serve":
         serve_main(args, torch)
         return
+    if args.xy_variant is not None:
+        from hastycompute_amd import _capi
+        _capi.call("hasty_set_option", b"sense_xy_variant", args.xy_variant)
     K, smaps, x, kidx, npts, spokes, offsets = make_workload(args, torch, native)
     P = x.shape[0]
     op = native.SenseNormalToeplitz(smaps, K, coil_batch=args.coil_batch, problem_batch=args.problem_batch,

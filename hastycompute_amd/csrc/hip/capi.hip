@@ -306,7 +306,7 @@ int hasty_set_option(const char* name, int64_t value)
         g_opt_sense_fused = (int)value;
         return 0;
     }
-    if (name && std::strcmp(name, "sense_xy_variant") == 0 && value >= 0 && value <= 1) {
+    if (name && std::strcmp(name, "sense_xy_variant") == 0 && value >= 0 && value <= 3) {
         sense_fused_set_xy_variant((int)value);
         return 0;
     }

@@ -952,6 +952,436 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
         }
 }
 
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// Copy one coset block (CW complex64, contiguous in HBM) into LDS with direct-to-LDS
+// loads (global_load_lds_dwordx4): instruction i of wave w moves 1 KB, lane l the 16 B at
+// byte (i * 512 + w * 64 + l) * 16 of both the block and the LDS buffer. Completion is
+// counted by vmcnt; a wait on it and a barrier make the data visible to every wave.
+template <int CW> __device__ __forceinline__ void coset_to_lds(const float2* __restrict__ src, float2* dst)
+{
+    static_assert((CW * 8) % (kXYW * 16) == 0, "coset block must be a whole number of 8 KB steps");
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const char* g = reinterpret_cast<const char*>(src);
+    char* d = reinterpret_cast<char*>(dst);
+#pragma unroll
+    for (int i = 0; i < (CW * 8) / (kXYW * 16); ++i) {
+        const int chunk = (i * kXYW + w * 64) * 16;
+        __builtin_amdgcn_global_load_lds(g + chunk + l * 16, (lds_void_t*)(d + chunk), 16, 0, 0);
+    }
+}
+
+// inv_xy_w_kernel with the HBM loads of the next coset in flight during the current
+// coset's work: one coset per step, its raw block staged in LDS by direct-to-LDS loads
+// issued right after the step-1 transform has consumed the previous one.
+template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_d_kernel(FusedArgs a)
+{
+    constexpr int N = 32 * NB, L = 64 * NB, P = 2 * NB, R2 = 4 * NB;
+    constexpr int CW = L * kW;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* stage = reinterpret_cast<float2*>(smem_raw);  // raw coset block [ky][m]
+    float2* buf = stage + CW;                               // [y1][k1][m]
+    float2* twL = buf + CW;
+    const int t = threadIdx.x;
+    const int b = t & 31;
+    const int yl = t >> 5;
+    const int m = t & 31;
+    const int r = t >> 5;
+    const int z = blockIdx.x;
+    const int pl = blockIdx.y;
+    const int p = a.p0 + pl;
+    for (int i = t; i < L; i += kXYW) twL[i] = a.twL[i];
+    const int64_t plane = (int64_t)N * N;
+    const LaneTw32 lt = lane_tw32(b, a.tw32);
+    float2 acc[2 * NB][NB];
+#pragma unroll
+    for (int k = 0; k < 2 * NB; ++k)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[k][j] = float2{0.f, 0.f};
+    const int ncl = coil_count(a, p);
+    auto block_of = [&](int ci, int s) {
+        return a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L + (int64_t)s * CW;
+    };
+    if (ncl > 0) coset_to_lds<CW>(block_of(0, 0), stage);
+
+    for (int ci = 0; ci < ncl; ++ci) {
+        const int coil = a.coils[a.c0 + ci] + p * a.coil_pstride;
+        const float2* sp = a.smaps + ((int64_t)coil * a.N0 + z) * plane;
+        float2 ic[2 * NB][NB];
+#pragma unroll
+        for (int k = 0; k < 2 * NB; ++k)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) ic[k][j] = float2{0.f, 0.f};
+        for (int s = 0; s < P; ++s) {
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's stage loads have landed
+            lds_barrier();                  // ... and every other wave's
+            // step 1 (thread: column m, k1 = r): inverse DFT over k2 -> y1, twiddle
+            if (r < R2) {
+                float2 u[16];
+#pragma unroll
+                for (int k2 = 0; k2 < 16; ++k2) u[k2] = stage[(r + R2 * k2) * kW + m];
+                dft16<1>(u);
+#pragma unroll
+                for (int q = 1; q < 16; ++q) u[q] = cmulc(twL[(q * r) % L], u[q]);
+#pragma unroll
+                for (int q = 0; q < 16; ++q) buf[(q * R2 + r) * kW + m] = u[q];
+            }
+            lds_barrier();  // stage consumed, buf complete
+            {
+                const int nci = s + 1 < P ? ci : ci + 1;
+                const int ns = s + 1 < P ? s + 1 : 0;
+                if (nci < ncl) coset_to_lds<CW>(block_of(nci, ns), stage);
+            }
+            // step 2 (thread: rows yl + 16 k, lane b)
+            float2 u[R2];
+#pragma unroll
+            for (int k1 = 0; k1 < R2; ++k1) u[k1] = buf[(yl * R2 + k1) * kW + brev5(b)];
+            dftn<R2, 1>(u);
+            float2 w[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) w[j] = twid<1>(twL, (s * (b + 32 * j)) % L);
+#pragma unroll
+            for (int k = 0; k < 2 * NB; ++k) {
+                const float2 val = dft32_inv_r(u[k], lt);
+#pragma unroll
+                for (int j = 0; j < NB; ++j) ic[k][j] = cfma(ic[k][j], w[j], val);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 2 * NB; ++k)
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                acc[k][j] = cfma_conj(acc[k][j], sp[(yl + 16 * k) * N + b + 32 * j], ic[k][j]);
+    }
+    float2* op = a.out + ((int64_t)p * a.N0 + z) * plane;
+    const float2* sb = a.sub ? a.sub + ((int64_t)p * a.N0 + z) * plane : nullptr;
+#pragma unroll
+    for (int k = 0; k < 2 * NB; ++k)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int64_t idx = (int64_t)(yl + 16 * k) * N + b + 32 * j;
+            float2 rr = acc[k][j];
+            if (a.accumulate) rr = cadd(rr, op[idx]);
+            else if (sb) rr = csub(rr, sb[idx]);
+            op[idx] = rr;
+        }
+}
+
+// 32-point DFT in registers, radix 2 in time over two 16-point DFTs.
+template <int S> __device__ __forceinline__ void dft32(float2* v)
+{
+    // w32^m = exp(S 2 pi i m / 32), m = 0..15
+    const float C[16] = {1.f, 0.98078528040323044f, 0.92387953251128674f, 0.83146961230254524f,
+                         0.70710678118654752f, 0.55557023301960222f, 0.38268343236508977f, 0.19509032201612827f,
+                         0.f, -0.19509032201612827f, -0.38268343236508977f, -0.55557023301960222f,
+                         -0.70710678118654752f, -0.83146961230254524f, -0.92387953251128674f, -0.98078528040323044f};
+    float2 e[16], o[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        e[k] = v[2 * k];
+        o[k] = v[2 * k + 1];
+    }
+    dft16<S>(e);
+    dft16<S>(o);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        // sin(2 pi k / 32) = cos(2 pi |8 - k| / 32)
+        const float sn = k <= 8 ? C[8 - k] : C[k - 8];
+        const float2 w = k == 0 ? o[0] : cmul(o[k], float2{C[k], (float)S * sn});
+        v[k] = cadd(e[k], w);
+        v[k + 16] = csub(e[k], w);
+    }
+}
+
+// Plane kernels for N1 = N2 = 128 (L = 256) with every transform in registers. Thread t
+// of 512 has q = t & 15 and g = t >> 4 and owns the rows y = g + 32 i (i < 4).
+//  x: x = q + 16 x2 (x2 < 8 nonzero of 16), kx = k1 + 16 k2:
+//       X[k1 + 16 k2] = sum_{x1} w16^{x1 k2} [ w256^{x1 k1} sum_{x2} v[x1 + 16 x2] w16^{x2 k1} ]
+//     the inner DFT runs in the thread that holds x1 = q, the outer one in the thread
+//     that holds k1 = q after a 16 x 16 transpose inside the 16 lanes of the row (LDS
+//     private to the wavefront, no workgroup barrier).
+//  y: y = g + 32 i (i < 4 nonzero of 8), ky = ka + 8 kb:
+//       Y[ka + 8 kb] = sum_g w32^{g kb} [ w256^{g ka} sum_i v[g + 32 i] w8^{i ka} ]
+//     the inner 8-point DFT in the row thread, the outer 32-point one in a thread that
+//     owns one (column, ka) pair after an exchange through LDS, 64 columns at a time.
+// The plane is stored at position ((h * 32 + kb) * 8 + ka) * 64 + q + 16 c for
+// kx = q + 16 (4 h + c): a wavefront's 64 lanes write 512 contiguous bytes.
+constexpr int kQT = 512;   // threads
+constexpr int kXS = 272;   // x transpose tile of one 16-lane row: 16 x 17 (padded)
+constexpr int kYS = 528;   // y exchange: stride of g, 8 ka x 64 columns + 16 (bank shift)
+constexpr size_t kQLds = sizeof(float2) * (32 * kYS + 272 + 256);
+
+__host__ __device__ __forceinline__ int64_t r16_pos(int ky, int kx)
+{
+    const int q = kx & 15, k2 = kx >> 4, h = k2 >> 2, c = k2 & 3;
+    const int ka = ky & 7, kb = ky >> 3;
+    return ((int64_t)(h * 32 + kb) * 8 + ka) * 64 + q + 16 * c;
+}
+
+// Buffer loads and stores on one plane: a descriptor in SGPRs (base, size in bytes;
+// accesses past the size read zero and write nothing), a per-lane 32-bit byte offset
+// and a uniform one, so a thread needs one VGPR of address for all its accesses.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, float2 v, uint32_t voff, uint32_t soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+
+// Order of LDS operations inside one wavefront (the hardware keeps it; this keeps the
+// compiler from moving memory operations across).
+// Both also keep the instruction scheduler from moving work across (it would hoist loads
+// of later steps and run out of registers).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_sched_barrier(0);
+}
+__device__ __forceinline__ void phase_barrier()
+{
+    __builtin_amdgcn_sched_barrier(0);
+    lds_barrier();
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+__global__ void __launch_bounds__(kQT) fwd_xy_r16_kernel(FusedArgs a)
+{
+    constexpr int N = 128, L = 256;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* yb = reinterpret_cast<float2*>(smem_raw);  // [32 g][kYS]
+    float2* tq = yb + 32 * kYS;  // [16 q][17]: w256^{q k} (row stride 17: no bank conflicts)
+    float2* tgt = tq + 272;      // [32 g][8]: w256^{g ka}
+    const int t = threadIdx.x;
+    const int q = t & 15, g = t >> 4;
+    // x transpose tile of this 16-lane row (shares memory with yb; a workgroup barrier
+    // separates the two uses)
+    float2* xs = yb + g * kXS;
+    const int z = blockIdx.x;
+    const int pl = blockIdx.y;
+    const int p = a.p0 + pl;
+    if (t < 256) {
+        tq[(t >> 4) * 17 + (t & 15)] = a.twL[((t >> 4) * (t & 15)) & (L - 1)];
+        tgt[t] = a.twL[((t >> 3) * (t & 7)) & (L - 1)];
+    }
+    const int64_t plane = (int64_t)N * N;
+    const auto xr = plane_rsrc(a.x + ((int64_t)p * a.N0 + z) * plane, sizeof(float2) * N * N);
+    // byte offsets: pixel (g + 32 i, q + 16 x2) at xoff + i * 32768 + x2 * 128;
+    // T position ((h * 32 + kb) * 8 + ka) * 64 + col at toff + (h * 32 + kb) * 4096
+    const uint32_t xoff = (uint32_t)(g * N + q) * 8u;
+    const uint32_t toff = (uint32_t)t * 8u;
+    phase_barrier();
+
+    const int ncl = coil_count(a, p);
+    auto smap_rsrc = [&](int ci) {
+        const int coil = a.coils[a.c0 + ci] + p * a.coil_pstride;
+        return plane_rsrc(a.smaps + ((int64_t)coil * a.N0 + z) * plane, sizeof(float2) * N * N);
+    };
+    // x and S of the next row to transform, loaded one step ahead (the first row of the
+    // next coil during the last y step of this one)
+    float2 nx[8], ns[8];
+    auto load_row = [&](const __amdgpu_buffer_rsrc_t& sr, int i) {
+#pragma unroll
+        for (int x2 = 0; x2 < 8; ++x2) {
+            const uint32_t so = (uint32_t)(i * 32 * N + 16 * x2) * 8u;
+            nx[x2] = bload(xr, xoff, so);
+            ns[x2] = bload(sr, xoff, so);
+        }
+    };
+    if (ncl > 0) load_row(smap_rsrc(0), 0);
+    for (int ci = 0; ci < ncl; ++ci) {
+        const auto sr = smap_rsrc(ci);
+        const auto tr = plane_rsrc(a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L,
+                                   sizeof(float2) * L * L);
+        float2 X[4][16];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float2 u[16];
+#pragma unroll
+            for (int x2 = 0; x2 < 8; ++x2) u[x2] = cmul(nx[x2], ns[x2]);
+            if (i < 3) load_row(sr, i + 1);
+#pragma unroll
+            for (int x2 = 8; x2 < 16; ++x2) u[x2] = float2{0.f, 0.f};
+            dft16<-1>(u);
+#pragma unroll
+            for (int k1 = 1; k1 < 16; ++k1) u[k1] = cmul(u[k1], tq[q * 17 + k1]);
+            wave_lds_sync();
+#pragma unroll
+            for (int k1 = 0; k1 < 16; ++k1) xs[k1 * 17 + q] = u[k1];
+            wave_lds_sync();
+#pragma unroll
+            for (int x1 = 0; x1 < 16; ++x1) X[i][x1] = xs[q * 17 + x1];
+            dft16<-1>(X[i]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        phase_barrier();
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            if (h == 3 && ci + 1 < ncl) load_row(smap_rsrc(ci + 1), 0);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                float2 u[8];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) u[i] = X[i][4 * h + c];
+#pragma unroll
+                for (int i = 4; i < 8; ++i) u[i] = float2{0.f, 0.f};
+                dft8<-1>(u);
+#pragma unroll
+                for (int ka = 0; ka < 8; ++ka)
+                    yb[g * kYS + ka * 64 + 16 * c + q] = ka ? cmul(u[ka], tgt[g * 8 + ka]) : u[0];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            phase_barrier();
+            {
+                const int col = t & 63, ka = t >> 6;
+                float2 u[32];
+#pragma unroll
+                for (int gg = 0; gg < 32; ++gg) u[gg] = yb[gg * kYS + ka * 64 + col];
+                dft32<-1>(u);
+#pragma unroll
+                for (int kb = 0; kb < 32; ++kb) bstore(tr, u[kb], toff, (uint32_t)(h * 32 + kb) * 4096u);
+            }
+            phase_barrier();
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kQT) inv_xy_r16_kernel(FusedArgs a)
+{
+    constexpr int N = 128, L = 256;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* yb = reinterpret_cast<float2*>(smem_raw);
+    float2* tq = yb + 32 * kYS;  // [16 q][17]: w256^{q k} (row stride 17: no bank conflicts)
+    float2* tgt = tq + 272;      // [32 g][8]: w256^{g ka}
+    const int t = threadIdx.x;
+    const int q = t & 15, g = t >> 4;
+    float2* xs = yb + g * kXS;
+    const int z = blockIdx.x;
+    const int pl = blockIdx.y;
+    const int p = a.p0 + pl;
+    if (t < 256) {
+        tq[(t >> 4) * 17 + (t & 15)] = a.twL[((t >> 4) * (t & 15)) & (L - 1)];
+        tgt[t] = a.twL[((t >> 3) * (t & 7)) & (L - 1)];
+    }
+    const int64_t plane = (int64_t)N * N;
+    float2 acc[4][8];  // pixels (g + 32 i, q + 16 x2)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int x2 = 0; x2 < 8; ++x2) acc[i][x2] = float2{0.f, 0.f};
+    phase_barrier();
+
+    const uint32_t xoff = (uint32_t)(g * N + q) * 8u;
+    const uint32_t toff = (uint32_t)t * 8u;
+    const int ncl = coil_count(a, p);
+    for (int ci = 0; ci < ncl; ++ci) {
+        const int coil = a.coils[a.c0 + ci] + p * a.coil_pstride;
+        const auto sr = plane_rsrc(a.smaps + ((int64_t)coil * a.N0 + z) * plane, sizeof(float2) * N * N);
+        const auto tr = plane_rsrc(a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L,
+                                   sizeof(float2) * L * L);
+        float2 X[4][16];
+        float2 sv[8];  // conj(S) row of the pixels the thread accumulates next
+        auto load_srow = [&](float2 (&d)[8], int i) {
+#pragma unroll
+            for (int x2 = 0; x2 < 8; ++x2) d[x2] = bload(sr, xoff, (uint32_t)(i * 32 * N + 16 * x2) * 8u);
+        };
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            {
+                // inverse DFT over kb -> g for one (column, ka) pair per thread, as two
+                // 16-point DFTs over the even and the odd kb (half the registers of one
+                // 32-point DFT): yb row g' < 16 holds E[g'], row 16 + g' holds w32^{+g'} O[g'],
+                // and g = g' + 16 s' is E[g'] + (-1)^{s'} w32^{g'} O[g'], formed by the reader.
+                const int col = t & 63, ka = t >> 6;
+                const float C[16] = {1.f, 0.98078528040323044f, 0.92387953251128674f, 0.83146961230254524f,
+                                     0.70710678118654752f, 0.55557023301960222f, 0.38268343236508977f,
+                                     0.19509032201612827f, 0.f, -0.19509032201612827f, -0.38268343236508977f,
+                                     -0.55557023301960222f, -0.70710678118654752f, -0.83146961230254524f,
+                                     -0.92387953251128674f, -0.98078528040323044f};
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    float2 u[16];
+#pragma unroll
+                    for (int m = 0; m < 16; ++m) u[m] = bload(tr, toff, (uint32_t)(h * 32 + 2 * m + e) * 4096u);
+                    dft16<1>(u);
+#pragma unroll
+                    for (int gp = 0; gp < 16; ++gp) {
+                        const float sn = gp <= 8 ? C[8 - gp] : C[gp - 8];
+                        const float2 v = (e && gp) ? cmul(u[gp], float2{C[gp], sn}) : u[gp];
+                        yb[(16 * e + gp) * kYS + ka * 64 + col] = v;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            phase_barrier();
+            if (h == 3) load_srow(sv, 0);
+            // twiddle w256^{-g ka}, inverse DFT over ka -> i, rows i < 4 kept
+            const float sg = g < 16 ? 1.f : -1.f;
+            const int gp = g & 15;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                float2 u[8];
+#pragma unroll
+                for (int ka = 0; ka < 8; ++ka) {
+                    const int o = ka * 64 + 16 * c + q;
+                    u[ka] = sfma(sg, yb[(16 + gp) * kYS + o], yb[gp * kYS + o]);
+                    if (ka) u[ka] = cmulc(tgt[g * 8 + ka], u[ka]);
+                }
+                dft8<1>(u);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) X[i][4 * h + c] = u[i];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            phase_barrier();
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            // inverse DFT over k2 -> x1 (thread k1 = q), transpose, twiddle w256^{-x1 k1},
+            // inverse DFT over k1 -> x2 (thread x1 = q), pixels x2 < 8 kept
+            float2 sn[8];
+            if (i < 3) load_srow(sn, i + 1);
+            dft16<1>(X[i]);
+            wave_lds_sync();
+#pragma unroll
+            for (int x1 = 0; x1 < 16; ++x1) xs[x1 * 17 + q] = X[i][x1];
+            wave_lds_sync();
+            float2 u[16];
+            u[0] = xs[q * 17];
+#pragma unroll
+            for (int k1 = 1; k1 < 16; ++k1) u[k1] = cmulc(tq[q * 17 + k1], xs[q * 17 + k1]);
+            dft16<1>(u);
+#pragma unroll
+            for (int x2 = 0; x2 < 8; ++x2) acc[i][x2] = cfma_conj(acc[i][x2], sv[x2], u[x2]);
+            if (i < 3) {
+#pragma unroll
+                for (int x2 = 0; x2 < 8; ++x2) sv[x2] = sn[x2];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        phase_barrier();
+    }
+    float2* op = a.out + ((int64_t)p * a.N0 + z) * plane;
+    const float2* sb = a.sub ? a.sub + ((int64_t)p * a.N0 + z) * plane : nullptr;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int x2 = 0; x2 < 8; ++x2) {
+            const int64_t idx = (int64_t)(g + 32 * i) * N + q + 16 * x2;
+            float2 rr = acc[i][x2];
+            if (a.accumulate) rr = cadd(rr, op[idx]);
+            else if (sb) rr = csub(rr, sb[idx]);
+            op[idx] = rr;
+        }
+}
+
 // z-axis pass for L0 = 256 (N0 = 128) as a 16 x 16 four-step FFT with the data in
 // registers: each of 256 threads owns one (column, group) pair of a 16-column bundle,
 // does 16-point DFTs in registers and meets the others through one LDS transpose per
@@ -1039,8 +1469,9 @@ uint32_t radix_code(int L, int* npass)
     return code;
 }
 
-// Permute a natural-order kernel [L0][L][L] (kx fastest) into the fused (s, ky, m) order.
-__global__ void permute_kernel_kernel(const float* in, float* out, int L0, int L)
+// Permute a natural-order kernel [L0][L][L] (kx fastest) into the plane order of the
+// fused kernels: (s, ky, m) for the coset kernels, r16_pos for the L = 256 register ones.
+__global__ void permute_kernel_kernel(const float* in, float* out, int L0, int L, int r16)
 {
     const int64_t LL = (int64_t)L * L;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1048,6 +1479,10 @@ __global__ void permute_kernel_kernel(const float* in, float* out, int L0, int L
     const int64_t kz = i / LL;
     const int64_t r = i % LL;
     const int ky = (int)(r / L), kx = (int)(r % L);
+    if (r16) {
+        out[kz * LL + r16_pos(ky, kx)] = in[i];
+        return;
+    }
     const int P = L / kW;
     const int s = kx % P, m = kx / P;
     out[kz * LL + ((int64_t)s * L + ky) * kW + m] = in[i];
@@ -1056,8 +1491,14 @@ __global__ void permute_kernel_kernel(const float* in, float* out, int L0, int L
 }  // namespace
 
 namespace {
-int g_xy_variant = 1;
-}
+// plane kernels: 0 LDS Stockham, 1 coset kernels with wavefront x DFTs, 2 (default) the
+// register kernels where N1 = N2 = 128 and the coset kernels elsewhere, 3 coset kernels
+// with the inverse fed by direct-to-LDS loads. The kernel permutation follows the
+// variant, so it is set before kernels are prepared.
+int g_xy_variant = 2;
+
+bool use_r16(int N) { return g_xy_variant == 2 && N == 128; }
+}  // namespace
 
 void sense_fused_set_xy_variant(int v) { g_xy_variant = v; }
 
@@ -1072,7 +1513,7 @@ void sense_fused_permute_kernel(const float* natural, float* permuted, const int
     const int L0 = (int)(2 * n3[0]), L = (int)(2 * n3[1]);
     const int64_t n = (int64_t)L0 * L * L;
     hipLaunchKernelGGL(permute_kernel_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, natural, permuted, L0,
-                       L);
+                       L, use_r16((int)n3[1]) ? 1 : 0);
     check_launch("permute_kernel_kernel");
 }
 
@@ -1112,7 +1553,7 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
     a.L0 = L0;
     a.codeL = radix_code(L, &a.npassL);
     a.codeL0 = radix_code(L0, &a.npassL0);
-    const size_t smem_xy = sizeof(float2) * (2 * (size_t)L * kW + L + 32);
+    size_t smem_xy = sizeof(float2) * (2 * (size_t)L * kW + L + 32);
     const size_t smem_z = sizeof(float2) * (2 * (size_t)L0 * kW + L0);
     const size_t smem_zct = sizeof(float2) * ((size_t)L0 * kZC + L0);
     decltype(&z_conv_ct_kernel<1>) zct = nullptr;
@@ -1128,6 +1569,14 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
         fwd = NB == 1 ? fwd_xy_w_kernel<1> : NB == 2 ? fwd_xy_w_kernel<2> : NB == 3 ? fwd_xy_w_kernel<3> : fwd_xy_w_kernel<4>;
         inv = NB == 1 ? inv_xy_w_kernel<1> : NB == 2 ? inv_xy_w_kernel<2> : NB == 3 ? inv_xy_w_kernel<3> : inv_xy_w_kernel<4>;
         nxy = kXYW;
+    }
+    if (g_xy_variant == 3)
+        inv = NB == 1 ? inv_xy_d_kernel<1> : NB == 2 ? inv_xy_d_kernel<2> : NB == 3 ? inv_xy_d_kernel<3> : inv_xy_d_kernel<4>;
+    if (use_r16(N)) {
+        fwd = fwd_xy_r16_kernel;
+        inv = inv_xy_r16_kernel;
+        nxy = kQT;
+        smem_xy = kQLds;
     }
     allow_lds(fwd, smem_xy);
     allow_lds(inv, smem_xy);

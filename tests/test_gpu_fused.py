@@ -36,7 +36,8 @@ def maps(rng, C, shape):
     return m / np.sqrt(np.sum(np.abs(m) ** 2, axis=0, keepdims=True))
 
 
-@pytest.mark.parametrize("N0,Nn,C,cb,P", [(32, 32, 3, 2, 2), (40, 64, 4, 4, 3), (24, 96, 2, 1, 1), (20, 32, 5, 3, 4)])
+@pytest.mark.parametrize("N0,Nn,C,cb,P", [(32, 32, 3, 2, 2), (40, 64, 4, 4, 3), (24, 96, 2, 1, 1), (20, 32, 5, 3, 4),
+                                         (8, 128, 3, 2, 3)])
 def test_fused_vs_oracle(N, N0, Nn, C, cb, P):
     rng = np.random.default_rng(N0 + Nn + C)
     shape = (N0, Nn, Nn)
