@@ -1126,13 +1126,16 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, u
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
 }
-__device__ __forceinline__ float2 bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+// ST: streaming data read or written once (non-temporal: kept out of the caches' way of
+// data that is used again, such as the image plane re-read for every coil).
+template <bool ST = false> __device__ __forceinline__ float2 bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, ST ? 2 : 0));
 }
+template <bool ST = false>
 __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, float2 v, uint32_t voff, uint32_t soff)
 {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, ST ? 2 : 0);
 }
 
 // Order of LDS operations inside one wavefront (the hardware keeps it; this keeps the
@@ -1193,7 +1196,7 @@ __global__ void __launch_bounds__(kQT) fwd_xy_r16_kernel(FusedArgs a)
         for (int x2 = 0; x2 < 8; ++x2) {
             const uint32_t so = (uint32_t)(i * 32 * N + 16 * x2) * 8u;
             nx[x2] = bload(xr, xoff, so);
-            ns[x2] = bload(sr, xoff, so);
+            ns[x2] = bload<true>(sr, xoff, so);
         }
     };
     if (ncl > 0) load_row(smap_rsrc(0), 0);
@@ -1247,7 +1250,7 @@ __global__ void __launch_bounds__(kQT) fwd_xy_r16_kernel(FusedArgs a)
                 for (int gg = 0; gg < 32; ++gg) u[gg] = yb[gg * kYS + ka * 64 + col];
                 dft32<-1>(u);
 #pragma unroll
-                for (int kb = 0; kb < 32; ++kb) bstore(tr, u[kb], toff, (uint32_t)(h * 32 + kb) * 4096u);
+                for (int kb = 0; kb < 32; ++kb) bstore<true>(tr, u[kb], toff, (uint32_t)(h * 32 + kb) * 4096u);
             }
             phase_barrier();
         }
@@ -1291,7 +1294,7 @@ __global__ void __launch_bounds__(kQT) inv_xy_r16_kernel(FusedArgs a)
         float2 sv[8];  // conj(S) row of the pixels the thread accumulates next
         auto load_srow = [&](float2 (&d)[8], int i) {
 #pragma unroll
-            for (int x2 = 0; x2 < 8; ++x2) d[x2] = bload(sr, xoff, (uint32_t)(i * 32 * N + 16 * x2) * 8u);
+            for (int x2 = 0; x2 < 8; ++x2) d[x2] = bload<true>(sr, xoff, (uint32_t)(i * 32 * N + 16 * x2) * 8u);
         };
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
@@ -1310,7 +1313,7 @@ __global__ void __launch_bounds__(kQT) inv_xy_r16_kernel(FusedArgs a)
                 for (int e = 0; e < 2; ++e) {
                     float2 u[16];
 #pragma unroll
-                    for (int m = 0; m < 16; ++m) u[m] = bload(tr, toff, (uint32_t)(h * 32 + 2 * m + e) * 4096u);
+                    for (int m = 0; m < 16; ++m) u[m] = bload<true>(tr, toff, (uint32_t)(h * 32 + 2 * m + e) * 4096u);
                     dft16<1>(u);
 #pragma unroll
                     for (int gp = 0; gp < 16; ++gp) {
@@ -1404,23 +1407,32 @@ __global__ void __launch_bounds__(kR16T) z_conv_r16_kernel(FusedArgs a, int LL, 
     float kv[16];   // kernel at k = g + 16 k2 of this column
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) kv[k2] = K[(int64_t)(g + 16 * k2) * LL + pos0 + c];
-    const float2* T0 = a.T + (int64_t)pl * a.ccount * 128 * (int64_t)LL + pos0 + c;
+    // one coil's half grid [128 z][LL] as a buffer: element (z, pos0 + c) of z = 16 n1 + g
+    // at byte voff + n1 * 16 * LL * 8
+    auto grid_rsrc = [&](int ci) {
+        return plane_rsrc(a.T + ((int64_t)pl * a.ccount + ci) * 128 * (int64_t)LL, (uint32_t)(128 * 8 * (int64_t)LL));
+    };
+    const uint32_t voff = (uint32_t)(((int64_t)g * LL + pos0 + c) * 8);
+    const uint32_t zstep = (uint32_t)(16 * 8 * (int64_t)LL);
     float2 pf[8];
+    {
+        const auto r0 = grid_rsrc(0);
 #pragma unroll
-    for (int n1 = 0; n1 < 8; ++n1) pf[n1] = T0[(int64_t)(16 * n1 + g) * LL];
+        for (int n1 = 0; n1 < 8; ++n1) pf[n1] = bload<true>(r0, voff, n1 * zstep);
+    }
 
     const int ncl = coil_count(a, a.p0 + pl);
     for (int ci = 0; ci < ncl; ++ci) {
-        float2* Tc = a.T + ((int64_t)pl * a.ccount + ci) * 128 * (int64_t)LL + pos0 + c;
+        const auto rc = grid_rsrc(ci);
         float2 v[16];
 #pragma unroll
         for (int n1 = 0; n1 < 8; ++n1) v[n1] = pf[n1];
 #pragma unroll
         for (int n1 = 8; n1 < 16; ++n1) v[n1] = float2{0.f, 0.f};
         if (ci + 1 < ncl) {
-            const float2* Tn = Tc + (int64_t)128 * LL;
+            const auto rn = grid_rsrc(ci + 1);
 #pragma unroll
-            for (int n1 = 0; n1 < 8; ++n1) pf[n1] = Tn[(int64_t)(16 * n1 + g) * LL];
+            for (int n1 = 0; n1 < 8; ++n1) pf[n1] = bload<true>(rn, voff, n1 * zstep);
         }
         // forward, step A (n2 = g): DFT over n1, twiddle w256^{g k1}, transpose
         dft16<-1>(v);
@@ -1448,7 +1460,7 @@ __global__ void __launch_bounds__(kR16T) z_conv_r16_kernel(FusedArgs a, int LL, 
         for (int k1 = 0; k1 < 16; ++k1) v[k1] = lds[g * kR16S2 + k1 * kR16S1 + c];
         dft16<1>(v);
 #pragma unroll
-        for (int n2 = 0; n2 < 8; ++n2) Tc[(int64_t)(g + 16 * n2) * LL] = v[n2];
+        for (int n2 = 0; n2 < 8; ++n2) bstore<true>(rc, v[n2], voff, n2 * zstep);
         lds_barrier();
     }
 }
