@@ -1214,6 +1214,7 @@ __global__ void __launch_bounds__(kQT) fwd_xy_r16_kernel(FusedArgs a)
 #pragma unroll
             for (int x2 = 8; x2 < 16; ++x2) u[x2] = float2{0.f, 0.f};
             dft16<-1>(u);
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int k1 = 1; k1 < 16; ++k1) u[k1] = cmul(u[k1], tq[q * 17 + k1]);
             wave_lds_sync();
@@ -1228,7 +1229,6 @@ __global__ void __launch_bounds__(kQT) fwd_xy_r16_kernel(FusedArgs a)
         phase_barrier();
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
-            if (h == 3 && ci + 1 < ncl) load_row(smap_rsrc(ci + 1), 0);
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 float2 u[8];
@@ -1243,6 +1243,7 @@ __global__ void __launch_bounds__(kQT) fwd_xy_r16_kernel(FusedArgs a)
                 __builtin_amdgcn_sched_barrier(0);
             }
             phase_barrier();
+            if (h == 3 && ci + 1 < ncl) load_row(smap_rsrc(ci + 1), 0);
             {
                 const int col = t & 63, ka = t >> 6;
                 float2 u[32];
