@@ -18,7 +18,9 @@ import re
 import sys
 
 # kernel symbol -> name used by the library's event timing
-KERNEL_NAMES = {"fwd_xy_kernel": "sense_fwd_xy", "inv_xy_kernel": "sense_inv_xy", "z_conv_r16_kernel": "sense_z_conv",
+KERNEL_NAMES = {"fwd_xy_kernel": "sense_fwd_xy", "inv_xy_kernel": "sense_inv_xy", "fwd_xy_w_kernel": "sense_fwd_xy",
+                "inv_xy_w_kernel": "sense_inv_xy", "inv_xy_d_kernel": "sense_inv_xy", "fwd_xy_r16_kernel": "sense_fwd_xy",
+                "inv_xy_r16_kernel": "sense_inv_xy", "z_conv_r16_kernel": "sense_z_conv",
                 "z_conv_ct_kernel": "sense_z_conv", "z_conv_kernel": "sense_z_conv", "spread_kernel": "nufft_spread",
                 "interp_kernel": "nufft_interp", "axis_fft_kernel": "fft_axis", "gram_kernel": "llr_gram",
                 "eig_w_kernel": "llr_eig", "apply_kernel": "llr_apply"}
