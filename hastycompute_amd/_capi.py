@@ -50,6 +50,14 @@ _SIGS = {
     "hasty_cg_workspace_size": ([c_i64, c_i64], c_sz),
     "hasty_cg_init": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "hasty_cg_iterate": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp], c_int),
+    "hasty_cg_init_precond": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
+    "hasty_cg_iterate_xr": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp], c_int),
+    "hasty_cg_iterate_pz": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
+    "hasty_circulant_precond_build": ([c_vp, c_i64, c_vp, c_int, c_i64, c_int, P_i64, c_vp, c_dbl, c_int, c_vp, c_dbl,
+                                       c_vp], c_int),
+    "hasty_circulant_precond_workspace_size": ([c_i64, c_int, P_i64, c_int, c_int], c_sz),
+    "hasty_circulant_precond_apply": ([c_vp, c_vp, c_vp, c_i64, c_i64, P_i32, c_int, P_i64, c_int, c_int, c_vp, c_sz,
+                                       c_vp], c_int),
     "hasty_llr_shifts": ([c_i64, P_i64, P_i32], c_int),
     "hasty_llr_workspace_size": ([P_i64, P_i64, P_i64], c_sz),
     "hasty_llr_normal_blocks": ([c_vp, P_i64, P_i64, P_i64, c_i64, P_i32, c_dbl, c_int, c_vp, c_sz, c_vp], c_int),

@@ -363,6 +363,64 @@ int hasty_cg_iterate(int64_t nproblems, int64_t n, void* x, void* r, void* p, co
     });
 }
 
+int hasty_cg_init_precond(int64_t nproblems, int64_t n, const void* r, const void* z, void* p, double* state,
+                          void* workspace, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(r && z && p && state && workspace, "null argument");
+        cg_init_precond(nproblems, n, (const float2*)r, (const float2*)z, (float2*)p, state, (double*)workspace,
+                        as_stream(stream));
+    });
+}
+
+int hasty_cg_iterate_xr(int64_t nproblems, int64_t n, void* x, void* r, const void* p, const void* Ap, double* state,
+                        void* workspace, double tol, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(x && r && p && Ap && state && workspace, "null argument");
+        cg_iterate_xr(nproblems, n, (float2*)x, (float2*)r, (const float2*)p, (const float2*)Ap, state,
+                      (double*)workspace, tol, as_stream(stream));
+    });
+}
+
+int hasty_cg_iterate_pz(int64_t nproblems, int64_t n, const void* r, const void* z, void* p, double* state,
+                        void* workspace, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(r && z && p && state && workspace, "null argument");
+        cg_iterate_pz(nproblems, n, (const float2*)r, (const float2*)z, (float2*)p, state, (double*)workspace,
+                      as_stream(stream));
+    });
+}
+
+int hasty_circulant_precond_build(const void* smaps, int64_t ncoil, const void* coords, int coord_fp64, int64_t M,
+                                  int ndim, const int64_t* nmodes, const float* weights, double lambda, int centered,
+                                  void* diag, double tol, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(smaps && nmodes && diag && (coords || M == 0), "null argument");
+        circulant_precond_build((const float2*)smaps, ncoil, coords, coord_fp64, M, ndim, nmodes, weights, lambda,
+                                centered, (float2*)diag, tol, as_stream(stream));
+    });
+}
+
+size_t hasty_circulant_precond_workspace_size(int64_t ndiag, int ndim, const int64_t* nmodes, int centered, int adjoint)
+{
+    if (!nmodes || ndim < 1 || ndim > 3) return 0;
+    return circulant_precond_workspace_bytes(ndiag, ndim, nmodes, centered, adjoint);
+}
+
+int hasty_circulant_precond_apply(const void* in, void* out, const void* diag, int64_t ndiag, int64_t nproblems,
+                                  const int32_t* diag_index, int ndim, const int64_t* nmodes, int centered,
+                                  int adjoint, void* workspace, size_t workspace_bytes, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(in && out && diag && nmodes, "null argument");
+        circulant_precond_apply((const float2*)in, (float2*)out, (const float2*)diag, ndiag, nproblems, diag_index,
+                                ndim, nmodes, centered, adjoint, workspace, workspace_bytes, as_stream(stream));
+    });
+}
+
 int hasty_llr_shifts(int64_t block_iter, const int64_t* block_shape, int32_t* shifts)
 {
     return guarded([&] {

@@ -172,6 +172,23 @@ __global__ void __launch_bounds__(kThreads) cg_update_p_kernel(float2* pv, const
     }
 }
 
+// rzold = partial sum (Re <r, z>); p = z (preconditioned start)
+__global__ void __launch_bounds__(kThreads) cg_rz_finalize(const double* partial, int nblk, double* state)
+{
+    __shared__ double sh[kThreads];
+    const int p = blockIdx.x;
+    const double s = sum_partials(partial + (int64_t)p * nblk, nblk, sh);
+    if (threadIdx.x == 0) state[(int64_t)p * kCgState] = s;
+}
+
+__global__ void __launch_bounds__(kThreads) copy_kernel(float2* dst, const float2* src, int64_t n)
+{
+    const int p = blockIdx.y;
+    const int64_t off = (int64_t)p * n;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+        dst[off + i] = src[off + i];
+}
+
 int blocks_for(int64_t n)
 {
     int64_t b = ceil_div(n, kThreads * 4);
@@ -219,6 +236,49 @@ void cg_iterate(int64_t P, int64_t n, float2* x, float2* r, float2* p, const flo
     check_launch("cg_beta_kernel");
     hipLaunchKernelGGL(cg_update_p_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, p, (const float2*)r, n,
                        (const double*)state);
+    check_launch("cg_update_p_kernel");
+}
+
+// Preconditioned variant (the _P branch of ConjugateGradient::run, opalgs.cpp:56-90):
+// cg_init, then z = P r by the caller, then cg_init_precond; per iteration Ap = A p,
+// cg_iterate_xr, z = P r, cg_iterate_pz.
+void cg_init_precond(int64_t P, int64_t n, const float2* r, const float2* z, float2* p, double* state, double* ws,
+                     hipStream_t s)
+{
+    HASTY_REQUIRE(P >= 1 && P < 65536 && n >= 1, "invalid CG problem sizes");
+    const int nb = blocks_for(n);
+    hipLaunchKernelGGL(dot_re_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, r, z, n, ws);
+    check_launch("dot_re_kernel");
+    hipLaunchKernelGGL(cg_rz_finalize, dim3((unsigned)P), dim3(kThreads), 0, s, (const double*)ws, nb, state);
+    check_launch("cg_rz_finalize");
+    hipLaunchKernelGGL(copy_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, p, z, n);
+    check_launch("copy_kernel");
+}
+
+void cg_iterate_xr(int64_t P, int64_t n, float2* x, float2* r, const float2* p, const float2* Ap, double* state,
+                   double* ws, double tol, hipStream_t s)
+{
+    HASTY_REQUIRE(P >= 1 && P < 65536 && n >= 1, "invalid CG problem sizes");
+    const int nb = blocks_for(n);
+    hipLaunchKernelGGL(dot_re_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, p, Ap, n, ws);
+    check_launch("dot_re_kernel");
+    hipLaunchKernelGGL(cg_alpha_kernel, dim3((unsigned)P), dim3(kThreads), 0, s, (const double*)ws, nb, state, tol);
+    check_launch("cg_alpha_kernel");
+    hipLaunchKernelGGL(cg_update_xr_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, x, r, p, Ap, n,
+                       (const double*)state, ws);
+    check_launch("cg_update_xr_kernel");
+}
+
+void cg_iterate_pz(int64_t P, int64_t n, const float2* r, const float2* z, float2* p, double* state, double* ws,
+                   hipStream_t s)
+{
+    HASTY_REQUIRE(P >= 1 && P < 65536 && n >= 1, "invalid CG problem sizes");
+    const int nb = blocks_for(n);
+    hipLaunchKernelGGL(dot_re_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, r, z, n, ws);
+    check_launch("dot_re_kernel");
+    hipLaunchKernelGGL(cg_beta_kernel, dim3((unsigned)P), dim3(kThreads), 0, s, (const double*)ws, nb, state);
+    check_launch("cg_beta_kernel");
+    hipLaunchKernelGGL(cg_update_p_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, p, z, n, (const double*)state);
     check_launch("cg_update_p_kernel");
 }
 

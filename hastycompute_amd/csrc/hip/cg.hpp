@@ -10,5 +10,11 @@ void cg_init(int64_t P, int64_t n, const float2* b, const float2* Ax, float2* r,
              hipStream_t s);
 void cg_iterate(int64_t P, int64_t n, float2* x, float2* r, float2* p, const float2* Ap, double* state, double* ws,
                 double tol, hipStream_t s);
+void cg_init_precond(int64_t P, int64_t n, const float2* r, const float2* z, float2* p, double* state, double* ws,
+                     hipStream_t s);
+void cg_iterate_xr(int64_t P, int64_t n, float2* x, float2* r, const float2* p, const float2* Ap, double* state,
+                   double* ws, double tol, hipStream_t s);
+void cg_iterate_pz(int64_t P, int64_t n, const float2* r, const float2* z, float2* p, double* state, double* ws,
+                   hipStream_t s);
 
 }  // namespace hasty_hip

@@ -44,6 +44,15 @@ void sense_normal_toeplitz(const float2* x, float2* out, const float2* smaps, co
                            const int64_t* n3, void* workspace, size_t ws_bytes, int coil_batch, hipStream_t s,
                            int coil_pstride = 0, const int* pcoils_host = nullptr, const int* pcoils_dev = nullptr);
 
+// precond.hip
+void circulant_precond_build(const float2* smaps, int64_t ncoil, const void* coords, int coord_fp64, int64_t M,
+                             int ndim, const int64_t* nmodes, const float* weights, double lambda, int centered,
+                             float2* diag, double tol, hipStream_t s);
+size_t circulant_precond_workspace_bytes(int64_t ndiag, int ndim, const int64_t* nmodes, int centered, int adjoint);
+void circulant_precond_apply(const float2* in, float2* out, const float2* diag, int64_t ndiag, int64_t P,
+                             const int32_t* dindex, int ndim, const int64_t* nmodes, int centered, int adjoint,
+                             void* ws, size_t ws_bytes, hipStream_t s);
+
 // sense_fused.hip
 bool sense_fused_supported(const int64_t* n3);
 // 1 (default): plane kernels with register four-step y transforms; 0: LDS Stockham passes.

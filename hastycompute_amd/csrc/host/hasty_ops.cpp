@@ -684,6 +684,77 @@ void BatchedSenseNormal::apply(const at::Tensor& in, at::Tensor& out, const std:
     if (!o.is_same(out)) out.copy_(o);
 }
 
+// ---------------------------------------------------------------------------------
+// CirculantPreconditionerOp (lib/mri/precond.cpp:11-117)
+// ---------------------------------------------------------------------------------
+
+std::unique_ptr<CirculantPreconditionerOp> CirculantPreconditionerOp::Create(at::Tensor diag, bool centered,
+                                                                             const std::optional<std::string>& norm,
+                                                                             bool adjointify)
+{
+    return std::unique_ptr<CirculantPreconditionerOp>(
+        new CirculantPreconditionerOp(std::move(diag), centered, norm, adjointify));
+}
+
+CirculantPreconditionerOp::CirculantPreconditionerOp(at::Tensor diag, bool centered,
+                                                     const std::optional<std::string>& norm, bool adjointify)
+    : _diag(std::move(diag)), _centered(centered), _norm(norm), _adjointify(adjointify)
+{
+    if (_norm.has_value() && *_norm != "ortho" && *_norm != "forward" && *_norm != "backward")
+        throw std::runtime_error("Invalid normalization mode: " + *_norm);
+}
+
+at::Tensor CirculantPreconditionerOp::build_diagonal(at::Tensor smaps, at::Tensor coord,
+                                                     const at::optional<at::Tensor>& weights,
+                                                     const at::optional<double>& lambda)
+{
+    if (!smaps.is_cuda() || !coord.is_cuda()) throw std::runtime_error("Coordinates did not reside in cuda device");
+    if (smaps.dim() < 2 || smaps.dim() > 4) throw std::runtime_error("smaps must be [C, N...] with 1 to 3 image dims");
+    const int ndim = (int)smaps.dim() - 1;
+    if (coord.dim() != 2 || coord.size(0) != ndim) throw std::runtime_error("coord must be [ndim, M]");
+    at::Tensor S = smaps.to(at::kComplexFloat).contiguous();
+    at::Tensor co = coord.scalar_type() == at::kDouble ? coord.contiguous() : coord.to(at::kFloat).contiguous();
+    at::Tensor w;
+    if (weights.has_value()) w = weights->to(at::kFloat).contiguous();
+    std::vector<int64_t> nm(S.sizes().begin() + 1, S.sizes().end());
+    at::Tensor d = at::empty(nm, S.options());
+    check(hasty_circulant_precond_build(S.data_ptr(), S.size(0), co.data_ptr(), co.scalar_type() == at::kDouble ? 1 : 0,
+                                        co.size(1), ndim, nm.data(), w.defined() ? w.data_ptr<float>() : nullptr,
+                                        lambda.has_value() ? *lambda : 0.0, 1, d.data_ptr(), 1e-10, detail::stream()));
+    return d;
+}
+
+at::Tensor CirculantPreconditionerOp::apply(const at::Tensor& in) const
+{
+    const int ndim = (int)_diag.dim();
+    if (ndim < 1 || ndim > 3 || in.dim() < ndim) throw std::runtime_error("input does not match the diagonal");
+    for (int d = 0; d < ndim; ++d)
+        if (in.size(in.dim() - ndim + d) != _diag.size(d)) throw std::runtime_error("input does not match the diagonal");
+    at::Tensor x = in.to(at::kComplexFloat).contiguous();
+    at::Tensor dg = _diag.to(at::kComplexFloat).contiguous();
+    std::vector<int64_t> nm(dg.sizes().begin(), dg.sizes().end());
+    const int64_t nb = x.numel() / dg.numel();
+    std::vector<int32_t> idx((size_t)nb, 0);
+    at::Tensor out = at::empty_like(x);
+    const size_t wsb = hasty_circulant_precond_workspace_size(1, ndim, nm.data(), _centered ? 1 : 0, _adjointify ? 1 : 0);
+    at::Tensor ws = at::empty({(int64_t)std::max<size_t>(wsb, 1)}, x.options().dtype(at::kByte));
+    check(hasty_circulant_precond_apply(x.data_ptr(), out.data_ptr(), dg.data_ptr(), 1, nb, idx.data(), ndim,
+                                        nm.data(), _centered ? 1 : 0, _adjointify ? 1 : 0, ws.data_ptr(), wsb,
+                                        detail::stream()));
+    return out.to(in.scalar_type());
+}
+
+std::shared_ptr<CirculantPreconditionerOp> CirculantPreconditionerOp::to_device(at::Stream stream) const
+{
+    return std::shared_ptr<CirculantPreconditionerOp>(
+        Create(_diag.to(stream.device()), _centered, _norm, _adjointify).release());
+}
+
+std::shared_ptr<CirculantPreconditionerOp> CirculantPreconditionerOp::adjoint() const
+{
+    return std::shared_ptr<CirculantPreconditionerOp>(Create(_diag, _centered, _norm, !_adjointify).release());
+}
+
 }  // namespace mri
 
 namespace svt {

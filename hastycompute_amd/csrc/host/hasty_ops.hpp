@@ -16,6 +16,7 @@
 #include <functional>
 #include <memory>
 #include <optional>
+#include <string>
 #include <vector>
 
 #include "hasty_hip.h"
@@ -284,6 +285,35 @@ private:
     std::unique_ptr<CUDASenseNormalToeplitz> _toep;
     at::Tensor _rhs;  // [F, E, N...] = A^H W y (when kdata is given)
     std::vector<int64_t> _rhs_coils;
+};
+
+// lib/mri/precond.ixx:13-38, precond.cpp:11-117: P = ifft(diag * fft(x)) (fftc / ifftc when
+// centered), the adjointified form fft(conj(diag) * ifft(x)). apply transforms the last
+// diag.dim() axes; leading axes are a batch. The norm does not change the product of the
+// forward and inverse transform and is kept only for the interface.
+class CirculantPreconditionerOp {
+public:
+    static std::unique_ptr<CirculantPreconditionerOp> Create(at::Tensor diag, bool centered,
+                                                             const std::optional<std::string>& norm, bool adjointify);
+    // smaps [C, N...] complex64, coord [ndim, M] float32/float64 CUDA, weights [M] float32;
+    // returns the complex64 diagonal [N...] in centred order (hasty_circulant_precond_build).
+    static at::Tensor build_diagonal(at::Tensor smaps, at::Tensor coord,
+                                     const at::optional<at::Tensor>& weights = at::nullopt,
+                                     const at::optional<double>& lambda = 0.0);
+    at::Tensor apply(const at::Tensor& in) const;
+    std::shared_ptr<CirculantPreconditionerOp> to_device(at::Stream stream) const;
+    std::shared_ptr<CirculantPreconditionerOp> adjoint() const;
+    const at::Tensor& diag() const { return _diag; }
+
+protected:
+    CirculantPreconditionerOp(at::Tensor diag, bool centered, const std::optional<std::string>& norm,
+                              bool adjointify);
+
+private:
+    at::Tensor _diag;
+    bool _centered;
+    std::optional<std::string> _norm;
+    bool _adjointify;
 };
 
 }  // namespace mri

@@ -196,10 +196,25 @@ class DeviceCG:
         C.call("hasty_cg_iterate", self.P, self.n, x.data_ptr(), r.data_ptr(), p.data_ptr(), Ap.data_ptr(),
                self.state.data_ptr(), self.ws.data_ptr(), float(tol), C.stream_ptr())
 
+    # preconditioned loop: init, z = P r, init_precond; per iteration iterate_xr, z = P r, iterate_pz
+    def init_precond(self, r, z, p):
+        C.call("hasty_cg_init_precond", self.P, self.n, r.data_ptr(), z.data_ptr(), p.data_ptr(),
+               self.state.data_ptr(), self.ws.data_ptr(), C.stream_ptr())
 
-def cg_solve(apply_A, b: torch.Tensor, x0: torch.Tensor, iters: int, tol: float = 0.0, check_pd: bool = True):
+    def iterate_xr(self, x, r, p, Ap, tol: float = 0.0):
+        C.call("hasty_cg_iterate_xr", self.P, self.n, x.data_ptr(), r.data_ptr(), p.data_ptr(), Ap.data_ptr(),
+               self.state.data_ptr(), self.ws.data_ptr(), float(tol), C.stream_ptr())
+
+    def iterate_pz(self, r, z, p):
+        C.call("hasty_cg_iterate_pz", self.P, self.n, r.data_ptr(), z.data_ptr(), p.data_ptr(),
+               self.state.data_ptr(), self.ws.data_ptr(), C.stream_ptr())
+
+
+def cg_solve(apply_A, b: torch.Tensor, x0: torch.Tensor, iters: int, tol: float = 0.0, check_pd: bool = True,
+             apply_P=None):
     """ConjugateGradient::run (lib/op/opalgs.cpp:50-96) for a batch [P, ...] with the
-    device CG; apply_A(v, out) writes A v into out. Returns (x, state)."""
+    device CG; apply_A(v, out) writes A v into out; apply_P(v, out), if given, writes the
+    preconditioned P v (the _P branch). Returns (x, state)."""
     P = b.shape[0]
     n = b[0].numel()
     x = x0.clone()
@@ -211,14 +226,72 @@ def cg_solve(apply_A, b: torch.Tensor, x0: torch.Tensor, iters: int, tol: float 
         return x, cg.state
     apply_A(x, Ap)
     cg.init(b, Ap, r, p)
-    for _ in range(iters):
-        apply_A(p, Ap)
-        cg.iterate(x, r, p, Ap, tol)
+    if apply_P is None:
+        for _ in range(iters):
+            apply_A(p, Ap)
+            cg.iterate(x, r, p, Ap, tol)
+    else:
+        z = torch.empty_like(b)
+        apply_P(r, z)
+        cg.init_precond(r, z, p)
+        for _ in range(iters):
+            apply_A(p, Ap)
+            cg.iterate_xr(x, r, p, Ap, tol)
+            apply_P(r, z)
+            cg.iterate_pz(r, z, p)
     if check_pd:
         st = cg.state.cpu()
         if bool((st[:, 4] != 0).any()):
             raise RuntimeError("A was not positive definite")
     return x, cg.state
+
+
+def circulant_precond_diagonal(smaps: torch.Tensor, coords: torch.Tensor, weights: Optional[torch.Tensor] = None,
+                               lamda: float = 0.0, centered: bool = True, tol: float = 1e-10) -> torch.Tensor:
+    """CirculantPreconditionerOp::build_diagonal (lib/mri/precond.cpp:65-117): complex64
+    [N...] diagonal, centred order (the reference's) or FFT order; see
+    hasty_circulant_precond_build."""
+    _cuda(smaps, "smaps")
+    _cuda(coords, "coords")
+    _require(smaps.dtype == torch.complex64 and smaps.is_contiguous(), "smaps must be contiguous complex64")
+    nm = list(smaps.shape[1:])
+    if weights is not None:
+        _cuda(weights, "weights")
+        _require(weights.dtype == torch.float32, "weights must be float32")
+        weights = weights.contiguous()
+    coords = coords.contiguous()
+    d = torch.empty(nm, dtype=torch.complex64, device=smaps.device)
+    C.call("hasty_circulant_precond_build", smaps.data_ptr(), int(smaps.shape[0]), coords.data_ptr(),
+           int(coords.dtype == torch.float64), int(coords.shape[1]), len(nm), C.i64arr(nm), C.ptr(weights),
+           float(lamda), int(bool(centered)), d.data_ptr(), float(tol), C.stream_ptr())
+    return d
+
+
+def circulant_precond_apply(x: torch.Tensor, diag: torch.Tensor, out: Optional[torch.Tensor] = None,
+                            diag_index: Optional[Sequence[int]] = None, centered: bool = True,
+                            adjoint: bool = False) -> torch.Tensor:
+    """CirculantPreconditionerOp::apply (precond.cpp:27-50) on x [P, N...] with diag [N...]
+    or [nd, N...] (diag_index[p] picks problem p's diagonal; default p)."""
+    _cuda(x, "x")
+    _require(x.dtype == torch.complex64 and x.is_contiguous(), "x must be contiguous complex64")
+    nm = list(x.shape[1:])
+    d = diag if diag.dim() == len(nm) + 1 else diag.unsqueeze(0)
+    _require(list(d.shape[1:]) == nm, "diag shape must match the image shape")
+    d = d.contiguous()
+    nd = int(d.shape[0])
+    if out is None:
+        out = torch.empty_like(x)
+    idx = None
+    if diag_index is not None:
+        idx = (ctypes.c_int32 * len(diag_index))(*[int(i) for i in diag_index])
+    elif nd == 1 and x.shape[0] > 1:
+        idx = (ctypes.c_int32 * x.shape[0])(*([0] * x.shape[0]))
+    wsb = C.lib().hasty_circulant_precond_workspace_size(nd, len(nm), C.i64arr(nm), int(bool(centered)),
+                                                          int(bool(adjoint)))
+    ws = torch.empty(max(int(wsb), 1), dtype=torch.uint8, device=x.device)
+    C.call("hasty_circulant_precond_apply", x.data_ptr(), out.data_ptr(), d.data_ptr(), nd, int(x.shape[0]), idx,
+           len(nm), C.i64arr(nm), int(bool(centered)), int(bool(adjoint)), ws.data_ptr(), int(wsb), C.stream_ptr())
+    return out
 
 
 def llr_shifts(block_iter: int, block_shape: Sequence[int]):

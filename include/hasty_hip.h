@@ -145,6 +145,41 @@ int hasty_cg_init(int64_t nproblems, int64_t n, const void* b, const void* Ax, v
                   void* workspace, void* stream);
 int hasty_cg_iterate(int64_t nproblems, int64_t n, void* x, void* r, void* p, const void* Ap, double* state,
                      void* workspace, double tol, void* stream);
+/* The preconditioned loop of ConjugateGradient::run (the _P branch, opalgs.cpp:56-90),
+ * z = P r computed by the caller between the calls:
+ *   hasty_cg_init(b, A x0) -> r; z = P r; hasty_cg_init_precond(r, z) -> p = z, rzold = <r, z>;
+ *   loop { Ap = A p; hasty_cg_iterate_xr(...) -> x, r; z = P r; hasty_cg_iterate_pz(r, z) -> p }.
+ * Stopping and the positive-definiteness flag as in hasty_cg_iterate (resid = sqrt(<r, z>)). */
+int hasty_cg_init_precond(int64_t nproblems, int64_t n, const void* r, const void* z, void* p, double* state,
+                          void* workspace, void* stream);
+int hasty_cg_iterate_xr(int64_t nproblems, int64_t n, void* x, void* r, const void* p, const void* Ap, double* state,
+                        void* workspace, double tol, void* stream);
+int hasty_cg_iterate_pz(int64_t nproblems, int64_t n, const void* r, const void* z, void* p, double* state,
+                        void* workspace, void* stream);
+
+/* ---------------------------------------------------------------------------------
+ * Circulant preconditioner.
+ * ------------------------------------------------------------------------------- */
+/* Replaces CirculantPreconditionerOp::build_diagonal (lib/mri/precond.cpp:65-117, after
+ * sigpy.mri.circulant_precond): diag = 1 / p, p the eigenvalues of the circulant matrix
+ * closest in Frobenius norm to sum_c S_c^H N^H diag(sqrt(w)) N S_c, plus lambda (p == 0
+ * -> 1). smaps [ncoil][N...] complex64, coords [ndim][M] (float32, or float64 if
+ * coord_fp64), weights [M] or NULL (= 1); N = nmodes (slowest first, 2N 2,3,5-smooth).
+ * FP64 on the GPU (type-1 NUFFT of the PSF at `tol`); diag: complex64 [N...], in the
+ * reference's centred order (frequency n - N/2 at index n) if centered, else FFT order. */
+int hasty_circulant_precond_build(const void* smaps, int64_t ncoil, const void* coords, int coord_fp64, int64_t M,
+                                  int ndim, const int64_t* nmodes, const float* weights, double lambda, int centered,
+                                  void* diag, double tol, void* stream);
+/* Replaces CirculantPreconditionerOp::apply (precond.cpp:27-50) for a batch:
+ * out[p] = ifftn(d * fftn(in[p])), or with adjoint fftn(conj(d) * ifftn(in[p])), with
+ * d = diag[diag_index[p]] (HOST int32 [nproblems], or NULL: diag p). The operator is a
+ * circular convolution, so `centered` (fftc / ifftc in the reference) only says in
+ * which order diag is stored. in / out [nproblems][N...] (out may equal in), diag
+ * [ndiag][N...]; workspace of hasty_circulant_precond_workspace_size bytes. */
+size_t hasty_circulant_precond_workspace_size(int64_t ndiag, int ndim, const int64_t* nmodes, int centered, int adjoint);
+int hasty_circulant_precond_apply(const void* in, void* out, const void* diag, int64_t ndiag, int64_t nproblems,
+                                  const int32_t* diag_index, int ndim, const int64_t* nmodes, int centered,
+                                  int adjoint, void* workspace, size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------
  * Locally-low-rank SVT.

@@ -357,6 +357,69 @@ def sense_normal_toeplitz(diag, x, smaps, coils, workers: Optional[int] = None):
 
 
 # ---------------------------------------------------------------------------------------
+# Circulant preconditioner (lib/mri/precond.cpp:27-117)
+# ---------------------------------------------------------------------------------------
+
+def fftc(x: np.ndarray, norm: Optional[str] = "ortho") -> np.ndarray:
+    """hasty::fft::fftc (lib/fft/fft.cpp:7-18): fftshift(fftn(ifftshift(x))) over all
+    axes, norm "ortho" by default (fft.ixx:12-13)."""
+    return np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(x), norm=norm))
+
+
+def ifftc(x: np.ndarray, norm: Optional[str] = "ortho") -> np.ndarray:
+    """hasty::fft::ifftc (fft.cpp:20-32)."""
+    return np.fft.fftshift(np.fft.ifftn(np.fft.ifftshift(x), norm=norm))
+
+
+def circulant_precond_diagonal(smaps: np.ndarray, coords: np.ndarray, weights: Optional[np.ndarray] = None,
+                               lam: float = 0.0, exact: bool = False, tol: float = 1e-12) -> np.ndarray:
+    """CirculantPreconditionerOp::build_diagonal (precond.cpp:65-117, "based on sigpy":
+    sigpy.mri.circulant_precond): psf = NUFFTAdjoint of sqrt(w) (ones without weights) onto
+    the doubled modes (nufft_apply scaling 1/sqrt(prod 2N)); per coil
+    xcorr = ifftc(|fftc(resize(conj S_c, 2N))|^2) * psf, p_c = fftc(xcorr)[::2, ...] * scale
+    with scale = prod(2N)^1.5 / prod(N)^2; p = sum_c p_c + lambda; p == 0 -> 1; 1 / p.
+    resize puts the map at the low corner (torch_util.cpp:240-253). Centred order.
+    The reference allocates p with the sample count as its shape and resizes the map with
+    its coil axis included (precond.cpp:94, 100-102), which raise at run time, and its
+    second multiply by sqrt(w) (precond.cpp:108-109) cannot broadcast against the image;
+    this follows the intended image-shaped sum, as sigpy does."""
+    img = list(smaps.shape[1:])
+    ndim = len(img)
+    img2 = [2 * n for n in img]
+    scale = float(np.prod(img2)) ** 1.5 / float(np.prod(img)) ** 2
+    ones = np.ones(coords.shape[1], dtype=np.complex128)
+    if weights is not None:
+        ones = ones * np.sqrt(np.asarray(weights, dtype=np.float64))
+    psf = nufft_apply(coords, [1] + img2, 1, ones[None], exact=exact, tol=tol)[0]
+    idx = (slice(None, None, 2),) * ndim
+    corner = tuple(slice(0, n) for n in img)
+    p_inv = np.zeros(img, dtype=np.complex128)
+    for S in smaps:
+        padded = np.zeros(img2, dtype=np.complex128)
+        padded[corner] = np.conj(S)
+        xcorr = ifftc(np.abs(fftc(padded)) ** 2)
+        xcorr = xcorr * psf
+        p_inv += fftc(xcorr)[idx] * scale
+    p_inv += lam
+    p_inv[p_inv == 0] = 1.0
+    return 1.0 / p_inv
+
+
+def circulant_precond_apply(diag: np.ndarray, x: np.ndarray, centered: bool = True,
+                            adjoint: bool = False) -> np.ndarray:
+    """CirculantPreconditionerOp::apply (precond.cpp:27-50) without a norm: ifftc(d fftc(x))
+    (centred) or ifftn(d fftn(x)); adjointify: fftc(conj(d) ifftc(x)) / fftn(conj(d) ifftn(x)).
+    x is one image [N...]."""
+    if centered:
+        if adjoint:
+            return fftc(np.conj(diag) * ifftc(x, None), None)
+        return ifftc(diag * fftc(x, None), None)
+    if adjoint:
+        return np.fft.fftn(np.conj(diag) * np.fft.ifftn(x))
+    return np.fft.ifftn(diag * np.fft.fftn(x))
+
+
+# ---------------------------------------------------------------------------------------
 # Conjugate gradient (lib/op/opalgs.cpp:50-96)
 # ---------------------------------------------------------------------------------------
 

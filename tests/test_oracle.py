@@ -255,3 +255,93 @@ def test_gradient_descent_lasso_orthonormal_closed_form():
     L = np.linalg.eigvalsh(A.conj().T @ A).max()
     x = O.gradient_descent(lambda v: A.conj().T @ (A @ v - bb), np.zeros(8), None, 1.0 / L, 2000, accelerate=True)
     assert np.abs(x - np.linalg.lstsq(A, bb, rcond=None)[0]).max() < 1e-8
+
+
+def _dense(op, shape):
+    """Matrix of a linear map on arrays of `shape` (column j = op(e_j))."""
+    n = int(np.prod(shape))
+    cols = []
+    for j in range(n):
+        e = np.zeros(n, dtype=np.complex128)
+        e[j] = 1.0
+        cols.append(np.asarray(op(e.reshape(shape))).ravel())
+    return np.stack(cols, axis=1)
+
+
+def test_circulant_precond_is_frobenius_optimal_circulant():
+    """1/diag are the eigenvalues of the circulant matrix closest in Frobenius norm to
+    T = sum_c S_c^H N1 diag(sqrt(w)) N2 S_c, i.e. the diagonal of U T U^H with U the
+    centred unitary DFT (brute force on small 2-D problems): a wrong scale, a dropped
+    sqrt, the other half of the doubled grid ([1::2]) or an uncentred order fails it;
+    lambda is added to those eigenvalues. Even sizes only: for odd N the reference's
+    [::2] reads doubled-grid frequency 2i - N, half a bin off the N grid (kept as is)."""
+    r2 = np.random.default_rng(11)
+    for img in [(6, 8), (4, 10)]:
+        C, M = 2, 50
+        S = r2.standard_normal((C,) + img) + 1j * r2.standard_normal((C,) + img)
+        co = r2.uniform(-np.pi, np.pi, (2, M))
+        w = r2.uniform(0.2, 1.5, M)
+        nm = [1] + list(img)
+
+        def T(x):
+            out = np.zeros(img, dtype=np.complex128)
+            for c in range(C):
+                k = O.nufft_apply(co, nm, 2, (x * S[c])[None], exact=True)
+                out += np.conj(S[c]) * O.nufft_apply(co, nm, 1, k * np.sqrt(w)[None], exact=True)[0]
+            return out
+
+        Tm = _dense(T, img)
+        U = _dense(lambda v: O.fftc(v), img)
+        eig = np.diag(U @ Tm @ U.conj().T)
+        d = O.circulant_precond_diagonal(S, co, w, exact=True)
+        assert np.abs(1.0 / d.ravel() - eig).max() < 1e-10 * np.abs(eig).max()
+        d2 = O.circulant_precond_diagonal(S, co, w, lam=0.3, exact=True)
+        assert np.abs(1.0 / d2.ravel() - (eig + 0.3)).max() < 1e-10 * np.abs(eig).max()
+        # the ES NUFFT PSF gives the same diagonal
+        d3 = O.circulant_precond_diagonal(S, co, w)
+        assert O.rel_l2(d3, d) < 1e-9
+
+
+def test_circulant_precond_apply_is_circular_convolution():
+    """The apply is a circular convolution with ifftn(d) (ifftn(ifftshift(d)) for a
+    centred d: the data are not shifted); the adjointified apply with fftn(conj d) / N.
+    Checked against direct circular sums on odd and even sizes."""
+    r2 = np.random.default_rng(12)
+    for shape in [(4, 6), (5, 3, 4)]:
+        d = r2.standard_normal(shape) + 1j * r2.standard_normal(shape)
+        x = r2.standard_normal(shape) + 1j * r2.standard_normal(shape)
+        n = int(np.prod(shape))
+        idx = np.array(np.unravel_index(np.arange(n), shape))
+
+        def circ(h):
+            out = np.zeros(n, dtype=np.complex128)
+            xf = x.ravel()
+            for k in range(n):
+                j = tuple(((idx[:, k][:, None] - idx) % np.array(shape)[:, None]))
+                out[k] = np.sum(h[j] * xf)
+            return out.reshape(shape)
+
+        for centered in (False, True):
+            dn = np.fft.ifftshift(d) if centered else d
+            h = np.fft.ifftn(dn)
+            assert O.rel_l2(O.circulant_precond_apply(d, x, centered), circ(h)) < 1e-12
+            g = np.fft.fftn(np.conj(dn)) / n
+            assert O.rel_l2(O.circulant_precond_apply(d, x, centered, adjoint=True), circ(g)) < 1e-12
+
+
+def test_preconditioned_cg_matches_dense_solve():
+    """CG with a preconditioner (the _P branch): on an ill-conditioned SPD system with an
+    SPD preconditioner it reaches numpy.linalg.solve, and in fewer iterations than plain CG
+    when P approximates the inverse."""
+    n = 30
+    Q, _ = np.linalg.qr(rnd((n, n)))
+    lam = np.logspace(0, 4, n)
+    A = (Q * lam) @ Q.conj().T
+    Pm = (Q * (1.0 / (lam * np.linspace(0.8, 1.2, n)))) @ Q.conj().T
+    b = rnd(n)
+    xs = np.linalg.solve(A, b)
+    xp = O.conjugate_gradient(lambda v: A @ v, b, np.zeros(n, complex), iters=3 * n, P=lambda v: Pm @ v)
+    assert O.rel_l2(xp, xs) < 1e-8
+    xp5 = O.conjugate_gradient(lambda v: A @ v, b, np.zeros(n, complex), iters=8, P=lambda v: Pm @ v)
+    xc5 = O.conjugate_gradient(lambda v: A @ v, b, np.zeros(n, complex), iters=8)
+    assert O.rel_l2(xp5, xs) < 0.1 * O.rel_l2(xc5, xs)
