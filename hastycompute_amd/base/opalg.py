@@ -174,6 +174,26 @@ def norm(a: Vector) -> torch.Tensor:
     return sum((norm(c) for c in a.children), torch.tensor(0.0))
 
 
+def min(a: Vector) -> torch.Tensor:
+    """Smallest entry over all leaves (opalg.py:443-450)."""
+    if a.istensor():
+        return torch.min(a.tensor)
+    out = min(a.children[0])
+    for c in a.children[1:]:
+        out = torch.minimum(out, min(c))
+    return out
+
+
+def max(a: Vector) -> torch.Tensor:
+    """Largest entry over all leaves (opalg.py:452-459)."""
+    if a.istensor():
+        return torch.max(a.tensor)
+    out = max(a.children[0])
+    for c in a.children[1:]:
+        out = torch.maximum(out, max(c))
+    return out
+
+
 class Operator:
     def __init__(self, ishape, oshape):
         self.ishape = ishape

@@ -1,4 +1,4 @@
-"""Iterative solvers (python/python/hastypy/base/solvers.py:11-162).
+"""Iterative solvers (python/python/hastypy/base/solvers.py:11-251).
 
 ConjugateGradient and GradientDescent keep the reference's update rules, stopping rules
 and attribute names; with the operators of hastycompute_amd.base.sense every vector
@@ -119,3 +119,63 @@ class GradientDescent(IterativeAlg):
                 callback(self.x, i)
             i += 1
         return self.x
+
+
+class PrimalDualHybridGradient(IterativeAlg):
+    """solvers.py:164-251, Chambolle-Pock with the reference's update order:
+    y += sigma K(x_bar); y = proxfc(y, sigma); x -= tau K^H(y); x = proxg(x, tau);
+    theta = 1 / sqrt(1 + 2 acceleration tau_min) (tau *= theta, sigma /= theta unless
+    theta == 1); x_bar = x + theta (x - x_bar_previous). The extrapolation uses the
+    previous x_bar, not the previous x (the reference stores x_old and does not use it),
+    and the prox calls ignore their step argument as ProximalOperator.__call__ does.
+    classic=True extrapolates from the previous x (Chambolle-Pock), which converges where
+    the reference's order may not."""
+
+    def __init__(self, proxfc, proxg, K: Linop, KH: Linop, x: Vector, y: Vector, tau: Vector, sigma: Vector,
+                 acceleration: float, max_iter=100, tol=0.0, classic: bool = False):
+        self.classic = classic
+        self.proxfc, self.proxg, self.K, self.KH = proxfc, proxg, K, KH
+        self.x, self.y = x, y
+        self.acceleration = acceleration
+        self.tau, self.sigma = tau, sigma
+        self.tau_mul = opalg.min(tau).item()
+        self.xtemp = self.x.clone()
+        self.tol = tol
+        self.resids = [-1]
+        self.rel_resids = [-1]
+        self.theta = 1.0
+        super().__init__(max_iter)
+
+    def _update(self):
+        self.y += self.sigma * self.K(self.xtemp)
+        self.y = self.proxfc(self.y, self.sigma)
+        self.xold = self.x.clone()
+        self.x -= self.tau * self.KH(self.y)
+        self.x = self.proxg(self.x, self.tau)
+        self.theta = 1.0 / math.sqrt(1.0 + 2.0 * self.acceleration * self.tau_mul)
+        if not math.isclose(self.theta, 1.0):
+            self.tau_mul *= self.theta
+            self.tau *= self.theta
+            self.sigma /= self.theta
+        self.xtemp = self.x - (self.xold if self.classic else self.xtemp)
+        self.resids.append(opalg.norm(self.xtemp).item())
+        self.xtemp = self.x + self.theta * self.xtemp
+        if self.tol != 0.0:
+            self.rel_resids.append(self.resids[-1] / opalg.norm(self.x))
+
+    def _done(self):
+        return self.iter >= self.max_iter or (self.rel_resids[-1] < self.tol and self.rel_resids[-1] > 0.0)
+
+    def run(self, callback=None, print_info=False):
+        i = 0
+        while not self.done() or i == 0:
+            t0 = time.time()
+            self.update()
+            if print_info:
+                print(f"PDHG Iter: {i} / {self.max_iter}, Res: {self.resids[-1]}, Theta: {self.theta}, "
+                      f"UpdateTime: {time.time() - t0}")
+            if callback is not None:
+                callback(self.x, i)
+            i += 1
+        return self.x
+

@@ -495,6 +495,31 @@ def gradient_descent(gradf: Callable[[np.ndarray], np.ndarray], x0: np.ndarray,
     return x
 
 
+def pdhg(K: Callable, KH: Callable, proxfc: Callable, proxg: Callable, x0: np.ndarray, y0: np.ndarray,
+         tau: float, sigma: float, acceleration: float, iters: int, classic: bool = False):
+    """PrimalDualHybridGradient._update (python/python/hastypy/base/solvers.py:191-218) in
+    the reference's order: y += sigma K(x_bar); y = proxfc(y, sigma); x -= tau K^H(y);
+    x = proxg(x, tau); theta = 1/sqrt(1 + 2 acceleration tau_min), tau *= theta and
+    sigma /= theta unless theta == 1; x_bar = x + theta (x - x_bar_previous) (the
+    previous x_bar: solvers.py:210-215, the stored x_old is unused). classic=True
+    extrapolates from the previous x (Chambolle-Pock). Returns (x, y)."""
+    x = np.array(x0, dtype=np.complex128)
+    y = np.array(y0, dtype=np.complex128)
+    xbar = x.copy()
+    tau_mul = float(tau)
+    for _ in range(iters):
+        y = proxfc(y + sigma * K(xbar), sigma)
+        xold = x
+        x = proxg(x - tau * KH(y), tau)
+        theta = 1.0 / math.sqrt(1.0 + 2.0 * acceleration * tau_mul)
+        if not math.isclose(theta, 1.0):
+            tau_mul *= theta
+            tau *= theta
+            sigma /= theta
+        xbar = x + theta * (x - (xold if classic else xbar))
+    return x, y
+
+
 def five_point_llr(coords_list, smaps, kdata_list, weights_list, x0, stepsize: float, iters: int,
                    block_strides, block_shape, block_iter: int, thresh: float, soft: bool,
                    accelerate: bool = True) -> np.ndarray:

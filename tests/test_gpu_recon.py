@@ -104,3 +104,48 @@ def test_five_point_llr_vs_oracle_loop(B):
                             [k.astype(np.complex128) for k in kdata], [w.astype(np.float64) for w in weights],
                             x0.astype(np.complex128), fp.stepsize, iters, [8, 8, 8], [8, 8, 8], 2, 0.0, True)
     assert O.rel_l2(ref0, ref) > 10 * e
+
+
+def test_pdhg_python_layer(B):
+    """base.solvers.PrimalDualHybridGradient on min_x 0.5|A x - y|^2 + lam |x|_1 with the
+    batched SENSE forward / adjoint as K / K^H equals the oracle's PDHG (reference order)
+    on the explicit operators, for the reference's and the classic extrapolation."""
+    rng = np.random.default_rng(6)
+    n, C = 12, 2
+    coords, kdata, _, S = make_scan(rng, 1, n, C, 1800)
+    co, y = coords[0], kdata[0]
+    opalg, sense, solvers, proximal = B["opalg"], B["sense"], B["solvers"], B["proximal"]
+    cot, St = torch.as_tensor(co).cuda(), torch.as_tensor(S).cuda()
+    K = sense.BatchedSense([cot], St)
+    KH = sense.BatchedSenseAdjoint([cot], St)
+    yt = torch.as_tensor(y).cuda()
+    tau, sigma, lam = 0.4, 0.4, 0.02
+
+    class DualL2(proximal.ProximalOperator):
+        # prox of sigma f*, f(z) = 0.5 |z - y|^2: (v - sigma y) / (1 + sigma)
+        def _apply(self, alpha, inp):
+            t = inp.get_tensorlist()[0]
+            return opalg.Vector([(t - sigma * yt) / (1 + sigma)])
+
+    class Soft(proximal.ProximalOperator):
+        def _apply(self, alpha, inp):
+            t = inp.get_tensor()
+            return opalg.Vector(torch.polar(torch.clamp(t.abs() - lam * tau, min=0.0), t.angle()))
+
+    Sd, cod, yd = S.astype(np.complex128), co.astype(np.float64), y[0].astype(np.complex128)
+    Kd = lambda v: O.sense_apply(cod, v[None], Sd, list(range(C)))  # noqa: E731
+    KHd = lambda w: O.sense_adjoint_apply(cod, w, Sd, list(range(C)))[0]  # noqa: E731
+    fc = lambda v, s: (v - sigma * yd) / (1 + sigma)  # noqa: E731
+    g = lambda v, t: np.exp(1j * np.angle(v)) * np.maximum(np.abs(v) - lam * tau, 0.0)  # noqa: E731
+    for classic in (False, True):
+        x = opalg.Vector(torch.zeros((1, 1, n, n, n), dtype=torch.complex64, device="cuda"))
+        yv = opalg.Vector([torch.zeros((1, C, co.shape[1]), dtype=torch.complex64, device="cuda")])
+        pd = solvers.PrimalDualHybridGradient(DualL2(K.oshape, None, False), Soft(K.ishape, None, False), K, KH, x,
+                                              yv, opalg.Vector(torch.tensor(tau)), opalg.Vector(torch.tensor(sigma)),
+                                              0.0, max_iter=8, classic=classic)
+        xg = pd.run().get_tensor().cpu().numpy()[0, 0]
+        ref, _ = O.pdhg(Kd, KHd, fc, g, np.zeros((n, n, n)), np.zeros((C, co.shape[1])), tau, sigma, 0.0, 8,
+                        classic=classic)
+        e = O.rel_l2(xg, ref)
+        print(f"PDHG classic={classic}: {e:.2e}", flush=True)
+        assert e < 1e-4

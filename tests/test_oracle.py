@@ -345,3 +345,32 @@ def test_preconditioned_cg_matches_dense_solve():
     xp5 = O.conjugate_gradient(lambda v: A @ v, b, np.zeros(n, complex), iters=8, P=lambda v: Pm @ v)
     xc5 = O.conjugate_gradient(lambda v: A @ v, b, np.zeros(n, complex), iters=8)
     assert O.rel_l2(xp5, xs) < 0.1 * O.rel_l2(xc5, xs)
+
+
+def test_pdhg_fixed_point_and_classic_closed_form():
+    """PDHG on min_x 0.5|x - b|^2 + lam |x|_1 with K = I (proxfc = projection onto the
+    lam-ball, proxg(v, t) = (v + t b) / (1 + t)). The classic extrapolation reaches
+    soft(b, lam), with and without acceleration. The reference's order (x_bar from the
+    previous x_bar) does not converge here, but keeps the primal-dual solution
+    (soft(b, lam), b - soft(b, lam)) fixed, and its first step equals the classic one:
+    a sign error in the dual step or a dropped step size breaks both."""
+    b = rnd(40)
+    lam = 0.6
+    soft = np.exp(1j * np.angle(b)) * np.maximum(np.abs(b) - lam, 0.0)
+
+    def clip(v, s):
+        m = np.abs(v)
+        return np.where(m > lam, v * (lam / np.maximum(m, 1e-300)), v)
+
+    I = lambda v: v  # noqa: E731
+    pg = lambda v, t: (v + t * b) / (1 + t)  # noqa: E731
+    for acc in (0.0, 0.5):
+        x, _ = O.pdhg(I, I, clip, pg, np.zeros(40), np.zeros(40), 0.5, 0.5, acc, 300 if acc == 0 else 20000,
+                      classic=True)
+        assert np.abs(x - soft).max() < (1e-8 if acc == 0 else 1e-6), acc
+        xr, yr = O.pdhg(I, I, clip, pg, soft, b - soft, 0.5, 0.5, acc, 25)
+        assert np.abs(xr - soft).max() < 1e-12 and np.abs(yr - (b - soft)).max() < 1e-12
+        x0 = rnd(40)
+        a1 = O.pdhg(I, I, clip, pg, x0, np.zeros(40), 0.5, 0.5, acc, 1)
+        c1 = O.pdhg(I, I, clip, pg, x0, np.zeros(40), 0.5, 0.5, acc, 1, classic=True)
+        assert np.array_equal(a1[0], c1[0]) and np.array_equal(a1[1], c1[1])
