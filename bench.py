@@ -135,7 +135,12 @@ def serve_main(args, torch):
 
     reqs = serving.poisson_requests(args.requests, args.rate, args.seed, iters=args.cg_iters)
     srv = serving.ReconServer()
-    st = srv.run(reqs, realtime=True)
+
+    def progress(msg):  # one line per finished request on stderr (the JSON line stays alone on stdout)
+        if " done " in msg:
+            print(msg, file=sys.stderr, flush=True)
+
+    st = srv.run(reqs, realtime=True, log=progress)
     line = {"metric": "served CG-SENSE recons/s with p50/p99 latency (Poisson stream)",
             "value": st["recons_per_s"], "unit": "recons/s", "n_gpus": 1, "higher_is_better": True,
             "dtype": "c64 (fp32), FP64 CG scalars", "data": "synthetic (seeded phantoms, coil maps, kooshball)",
