@@ -58,6 +58,8 @@ _SIGS = {
     "hasty_circulant_precond_workspace_size": ([c_i64, c_int, P_i64, c_int, c_int], c_sz),
     "hasty_circulant_precond_apply": ([c_vp, c_vp, c_vp, c_i64, c_i64, P_i32, c_int, P_i64, c_int, c_int, c_vp, c_sz,
                                        c_vp], c_int),
+    "hasty_dct3": ([c_vp, c_i64, P_i64, c_int, c_int, c_vp], c_int),
+    "hasty_l1dct_prox": ([c_vp, c_i64, P_i64, c_int, c_dbl, c_vp], c_int),
     "hasty_llr_shifts": ([c_i64, P_i64, P_i32], c_int),
     "hasty_llr_workspace_size": ([P_i64, P_i64, P_i64], c_sz),
     "hasty_llr_normal_blocks": ([c_vp, P_i64, P_i64, P_i64, c_i64, P_i32, c_dbl, c_int, c_vp, c_sz, c_vp], c_int),

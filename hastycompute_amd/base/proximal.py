@@ -6,6 +6,7 @@ absolute `thresh` of its SVTOptions, whatever the solver's step size.
 """
 from __future__ import annotations
 
+from .. import native
 from . import opalg
 from .opalg import Linop, Vector
 from .svt import Normal3DBlocksSVT, Random3DBlocksSVT
@@ -57,6 +58,26 @@ class UnitaryTransform(ProximalOperator):
 
     def _apply(self, alpha, input):
         return self.unitary_adjoint(self.prox(self.unitary(input), alpha))
+
+
+class ProximalL1DCT(ProximalOperator):
+    """prox of lam |DCT(x)|_1 over the axes of `axis` (proximal.py:134-183): per volume
+    dctn, c <- exp(i angle c) max(|c| - alpha, 0), idctn (scipy's unnormalised DCT-II,
+    norm=None), on the GPU through hasty_l1dct_prox."""
+
+    def __init__(self, shape, axis, lamda, inplace=True):
+        self.axis = tuple(bool(a) for a in axis)
+        super().__init__(shape, lamda, inplace)
+
+    @staticmethod
+    def tensor_apply(input, lamda, axis):
+        native.l1dct_prox_(input, axis, float(lamda))
+
+    def _apply(self, alpha, input):
+        output = input if self.inplace else input.clone()
+        for t in output.get_tensorlist():
+            ProximalL1DCT.tensor_apply(t, alpha, self.axis)
+        return output
 
 
 class L2Reg(ProximalOperator):

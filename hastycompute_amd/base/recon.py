@@ -1,5 +1,8 @@
 """Reconstruction drivers (python/python/hastypy/base/recon.py:14-205).
 
+FivePointFULL: gradient descent (FISTA) on the 5 encodings with all frames' samples
+stacked, with an optional L1-DCT prox (recon.py:27-83).
+
 FivePointLLR: FISTA-accelerated proximal gradient for a 5-point 4D-flow scan, the
 gradient A^H W (A x - y) through the batched Toeplitz normal operator and the LLR prox
 over [F, 5, N0, N1, N2] (Casorati rows (encoding, x, y, z), one column per frame).
@@ -47,6 +50,60 @@ def batched_max_eig(coord_vec, smaps, weights_vec=None, max_iter=8, generator=No
         lam = torch.linalg.vector_norm(y.reshape(P, -1), dim=1)
         x = y / lam.view(P, 1, 1, 1, 1).to(y.dtype)
     return lam
+
+
+def stack_frame_datas_fivepoint(coord_vec, kdata_vec):
+    """recon.py:27-44: per encoding, the coordinates of all frames concatenated along the
+    sample axis (dim 1) and their k-space data along dim 2."""
+    new_coord_vec, new_kdata_vec = [], []
+    nframes = len(coord_vec) // 5
+    for enc in range(5):
+        new_coord_vec.append(torch.cat([coord_vec[f * 5 + enc] for f in range(nframes)], dim=1))
+        new_kdata_vec.append(torch.cat([kdata_vec[f * 5 + enc] for f in range(nframes)], dim=2))
+    return new_coord_vec, new_kdata_vec
+
+
+class FivePointFULL:
+    """recon.py:47-83: max eigenvalue of encoding 0's normal operator (5 power
+    iterations), step 1 / max_eig, gradient descent with the batched normal operator and,
+    for lamda != 0, ProximalL1DCT with threshold lamda / step (the prox's step argument is
+    ignored, as ProximalOperator.__call__ does). max_eig may be given to skip the power
+    iteration (it starts from a random vector)."""
+
+    def __init__(self, smaps, coord_vec, kdata_vec, weights_vec=None, streams=None, solver="GD", lamda=0.0,
+                 max_eig=None):
+        self.smaps, self.coord_vec, self.kdata_vec, self.weights_vec = smaps, coord_vec, kdata_vec, weights_vec
+        self.streams, self.solver = streams, solver
+        if max_eig is None:
+            self.max_eig_run()
+        else:
+            self.max_eig = torch.tensor(float(max_eig), dtype=torch.float32)
+        self.stepsize = 1 / self.max_eig
+        if self.solver == "GD":
+            self.sensenormalop = BatchedSenseNormal(self.coord_vec, self.smaps, self.kdata_vec, self.weights_vec,
+                                                    self.streams)
+        else:
+            raise RuntimeError(f"solver {solver!r} is not part of this build (GD is)")
+        self.lamda = lamda / self.stepsize
+        if float(self.lamda) != 0.0:
+            self._dctprox = prox.ProximalL1DCT((5, 1) + tuple(self.smaps.shape[1:]), (True, True, True),
+                                               float(self.lamda))
+        else:
+            self._dctprox = None
+
+    def max_eig_run(self, print_info=False):
+        w = None if self.weights_vec is None else [self.weights_vec[0]]
+        maxeigop = BatchedSenseNormal([self.coord_vec[0]], self.smaps, None, w)
+        self.max_eig = opalg.MaxEig(maxeigop, torch.complex64, max_iter=5).run(print_info=print_info).to(
+            torch.float32)
+
+    def run(self, image, iter=50, callback=None, accelerate=True, print_info=False):
+        image = Vector(image)
+        if self.solver == "GD":
+            gd = GradientDescent(self.sensenormalop, image, self._dctprox, float(1 / self.max_eig),
+                                 accelerate=accelerate, max_iter=iter)
+            image = gd.run(callback, print_info=print_info)
+        return image.get_tensor()
 
 
 class FivePointLLR:

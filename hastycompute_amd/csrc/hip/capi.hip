@@ -421,6 +421,22 @@ int hasty_circulant_precond_apply(const void* in, void* out, const void* diag, i
     });
 }
 
+int hasty_dct3(void* data, int64_t batch, const int64_t* dims3, int axis_mask, int inverse, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(data && dims3, "null argument");
+        dct3((float2*)data, batch, dims3, axis_mask, inverse, as_stream(stream));
+    });
+}
+
+int hasty_l1dct_prox(void* data, int64_t batch, const int64_t* dims3, int axis_mask, double thresh, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(data && dims3, "null argument");
+        l1dct_prox((float2*)data, batch, dims3, axis_mask, (float)thresh, as_stream(stream));
+    });
+}
+
 int hasty_llr_shifts(int64_t block_iter, const int64_t* block_shape, int32_t* shifts)
 {
     return guarded([&] {

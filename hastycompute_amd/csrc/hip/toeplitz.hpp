@@ -44,6 +44,10 @@ void sense_normal_toeplitz(const float2* x, float2* out, const float2* smaps, co
                            const int64_t* n3, void* workspace, size_t ws_bytes, int coil_batch, hipStream_t s,
                            int coil_pstride = 0, const int* pcoils_host = nullptr, const int* pcoils_dev = nullptr);
 
+// dct.hip
+void dct3(float2* data, int64_t batch, const int64_t* dims, int axis_mask, int inverse, hipStream_t s);
+void l1dct_prox(float2* data, int64_t batch, const int64_t* dims, int axis_mask, float thresh, hipStream_t s);
+
 // precond.hip
 void circulant_precond_build(const float2* smaps, int64_t ncoil, const void* coords, int coord_fp64, int64_t M,
                              int ndim, const int64_t* nmodes, const float* weights, double lambda, int centered,

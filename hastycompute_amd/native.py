@@ -294,6 +294,32 @@ def circulant_precond_apply(x: torch.Tensor, diag: torch.Tensor, out: Optional[t
     return out
 
 
+def _vol_view(x: torch.Tensor, naxes: int):
+    _cuda(x, "x")
+    _require(x.dtype == torch.complex64 and x.is_contiguous(), "x must be contiguous complex64")
+    _require(1 <= naxes <= 3 and x.dim() >= naxes, "1 to 3 transform axes")
+    dims = [1] * (3 - naxes) + list(x.shape[x.dim() - naxes:])
+    batch = x.numel() // max(1, int(torch.tensor(dims).prod().item()))
+    return batch, dims
+
+
+def dctn_(x: torch.Tensor, axes: Sequence[bool], inverse: bool = False) -> torch.Tensor:
+    """In-place unnormalised DCT-II (or its inverse) over the last len(axes) dims of x
+    where axes[i] is True (scipy.fft.dctn / idctn, norm=None); see hasty_dct3."""
+    batch, dims = _vol_view(x, len(axes))
+    mask = sum(1 << (3 - len(axes) + i) for i, a in enumerate(axes) if a)
+    C.call("hasty_dct3", x.data_ptr(), batch, C.i64arr(dims), mask, int(bool(inverse)), C.stream_ptr())
+    return x
+
+
+def l1dct_prox_(x: torch.Tensor, axes: Sequence[bool], thresh: float) -> torch.Tensor:
+    """ProximalL1DCT.tensor_apply in place: dctn, complex soft threshold, idctn."""
+    batch, dims = _vol_view(x, len(axes))
+    mask = sum(1 << (3 - len(axes) + i) for i, a in enumerate(axes) if a)
+    C.call("hasty_l1dct_prox", x.data_ptr(), batch, C.i64arr(dims), mask, float(thresh), C.stream_ptr())
+    return x
+
+
 def llr_shifts(block_iter: int, block_shape: Sequence[int]):
     out = C.i32arr([0] * (3 * block_iter))
     C.call("hasty_llr_shifts", block_iter, C.i64arr(block_shape), out)

@@ -158,6 +158,19 @@ int hasty_cg_iterate_pz(int64_t nproblems, int64_t n, const void* r, const void*
                         void* workspace, void* stream);
 
 /* ---------------------------------------------------------------------------------
+ * DCT and the L1-DCT prox.
+ * ------------------------------------------------------------------------------- */
+/* In place, on batch volumes [batch][D0][D1][D2] complex64, along the axes whose bit is
+ * set in axis_mask (bit d: axis Dd, each <= 512): the unnormalised DCT-II
+ * y_k = 2 sum_n x_n cos(pi k (2n+1) / 2N) or its inverse (scipy.fft.dctn / idctn with
+ * norm=None, which ProximalL1DCT calls through cupyx, python/python/hastypy/base/
+ * proximal.py:148-152). */
+int hasty_dct3(void* data, int64_t batch, const int64_t* dims3, int axis_mask, int inverse, void* stream);
+/* Replaces ProximalL1DCT.tensor_apply (proximal.py:139-163): dct, then
+ * c <- exp(i angle c) max(|c| - thresh, 0), then the inverse dct, in place. */
+int hasty_l1dct_prox(void* data, int64_t batch, const int64_t* dims3, int axis_mask, double thresh, void* stream);
+
+/* ---------------------------------------------------------------------------------
  * Circulant preconditioner.
  * ------------------------------------------------------------------------------- */
 /* Replaces CirculantPreconditionerOp::build_diagonal (lib/mri/precond.cpp:65-117, after

@@ -548,6 +548,41 @@ def five_point_llr(coords_list, smaps, kdata_list, weights_list, x0, stepsize: f
     return x.reshape((F, 5) + sp)
 
 
+def l1dct_prox(x: np.ndarray, lam: float, axis: Sequence[bool]) -> np.ndarray:
+    """ProximalL1DCT.tensor_apply (python/python/hastypy/base/proximal.py:139-163): over
+    every trailing volume of len(axis) dims, dctn over the axes marked True (scipy's
+    unnormalised DCT-II, norm=None, what cupyx.scipy.fft.dctn computes), then
+    exp(i angle c) max(|c| - lam, 0), then idctn."""
+    import scipy.fft as sfft
+    x = np.array(x, dtype=np.complex128)
+    nd = len(axis)
+    axes = tuple(x.ndim - nd + i for i, a in enumerate(axis) if a)
+    c = sfft.dctn(x, axes=axes)
+    c = np.exp(1j * np.angle(c)) * np.maximum(np.abs(c) - lam, 0.0)
+    return sfft.idctn(c, axes=axes)
+
+
+def five_point_full(coords_list, smaps, kdata_list, weights_list, x0, max_eig: float, lamda: float, iters: int,
+                    accelerate: bool = True) -> np.ndarray:
+    """FivePointFULL.run with solver 'GD' (python/python/hastypy/base/recon.py:47-83) on the
+    5 stacked encodings: x0 [5, 1, N...], step 1 / max_eig, gradient sense_normal_apply with
+    kdata/weights, prox l1dct_prox with threshold lamda * max_eig (lamda / step) when
+    lamda != 0."""
+    coils = list(range(smaps.shape[0]))
+
+    def grad(xg):
+        out = np.empty_like(xg)
+        for p in range(xg.shape[0]):
+            w = None if weights_list is None else weights_list[p]
+            kd = None if kdata_list is None else kdata_list[p][0]
+            out[p] = sense_normal_apply(coords_list[p], xg[p], smaps, coils, kdata=kd, weights=w)
+        return out
+
+    th = lamda * max_eig
+    prox = (lambda v: l1dct_prox(v, th, (True, True, True))) if lamda != 0.0 else None
+    return gradient_descent(grad, np.asarray(x0, dtype=np.complex128), prox, 1.0 / max_eig, iters, accelerate)
+
+
 # ---------------------------------------------------------------------------------------
 # SVT / LLR (lib/op/svt.cpp, lib/op/svt.ixx)
 # ---------------------------------------------------------------------------------------

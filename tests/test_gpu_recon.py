@@ -149,3 +149,52 @@ def test_pdhg_python_layer(B):
         e = O.rel_l2(xg, ref)
         print(f"PDHG classic={classic}: {e:.2e}", flush=True)
         assert e < 1e-4
+
+
+@pytest.mark.parametrize("shape,axis", [((2, 1, 12, 10, 9), (True, True, True)), ((3, 16, 20), (False, True)),
+                                        ((1, 1, 7, 8, 5), (True, False, True))])
+def test_dct_and_l1dct_prox_vs_oracle(B, shape, axis):
+    import scipy.fft as sfft
+    from hastycompute_amd import native
+
+    rng = np.random.default_rng(sum(shape))
+    x = rnd(rng, shape)
+    nd = len(axis)
+    axes = tuple(len(shape) - nd + i for i, a in enumerate(axis) if a)
+    xt = torch.as_tensor(x.astype(np.complex64)).cuda()
+    native.dctn_(xt, axis)
+    torch.cuda.synchronize()
+    assert O.rel_l2(xt.cpu().numpy(), sfft.dctn(x, axes=axes)) < 2e-6
+    native.dctn_(xt, axis, inverse=True)
+    torch.cuda.synchronize()
+    assert O.rel_l2(xt.cpu().numpy(), x) < 2e-6
+    lam = 0.3 * float(np.median(np.abs(sfft.dctn(x, axes=axes))))
+    prox = B["proximal"].ProximalL1DCT(shape, axis, lam, inplace=False)
+    got = prox(B["opalg"].Vector(torch.as_tensor(x.astype(np.complex64)).cuda())).get_tensor().cpu().numpy()
+    e = O.rel_l2(got, O.l1dct_prox(x, lam, axis))
+    print(f"l1dct prox {shape} {axis}: {e:.2e}", flush=True)
+    assert e < 2e-6
+
+
+def test_five_point_full_vs_oracle(B):
+    """FivePointFULL (recon.py:47-83): frames stacked per encoding, GD with the L1-DCT prox
+    against the oracle's loop with the same max eigenvalue."""
+    rng = np.random.default_rng(8)
+    n, C, nframes = 12, 2, 2
+    coords, kdata, weights, S = make_scan(rng, nframes, n, C, 600)
+    recon = B["recon"]
+    cvec = [torch.as_tensor(c).cuda() for c in coords]
+    kvec = [torch.as_tensor(k).cuda() for k in kdata]
+    wvec = [torch.as_tensor(w).cuda() for w in weights]
+    sc, sk = recon.stack_frame_datas_fivepoint(cvec, kvec)
+    sw = [torch.cat([wvec[f * 5 + e] for f in range(nframes)]) for e in range(5)]
+    max_eig, lamda, iters = 3.0, 2e-3, 4
+    fp = recon.FivePointFULL(torch.as_tensor(S).cuda(), sc, sk, sw, lamda=lamda, max_eig=max_eig)
+    x0 = (0.1 * rnd(rng, (5, 1, n, n, n))).astype(np.complex64)
+    xg = fp.run(torch.as_tensor(x0).cuda(), iter=iters).cpu().numpy()
+    ref = O.five_point_full([c.cpu().numpy().astype(np.float64) for c in sc], S.astype(np.complex128),
+                            [k.cpu().numpy().astype(np.complex128) for k in sk],
+                            [w.cpu().numpy().astype(np.float64) for w in sw], x0, max_eig, lamda, iters)
+    e = O.rel_l2(xg, ref)
+    print(f"FivePointFULL {iters} iterations: {e:.2e}", flush=True)
+    assert e < 1e-4

@@ -374,3 +374,26 @@ def test_pdhg_fixed_point_and_classic_closed_form():
         a1 = O.pdhg(I, I, clip, pg, x0, np.zeros(40), 0.5, 0.5, acc, 1)
         c1 = O.pdhg(I, I, clip, pg, x0, np.zeros(40), 0.5, 0.5, acc, 1, classic=True)
         assert np.array_equal(a1[0], c1[0]) and np.array_equal(a1[1], c1[1])
+
+
+def test_dct_matches_cosine_sums_and_l1dct_prox():
+    """The oracle's L1-DCT prox rests on scipy's dctn / idctn: checked against the cosine
+    sums y_k = 2 sum x_n cos(pi k (2n+1) / 2N) and their inverse on an odd/even 3-D size,
+    one and all axes; lam = 0 gives back x, a huge lam gives 0."""
+    import scipy.fft as sfft
+    x = rnd((5, 6, 3))
+    for axes in [(0, 1, 2), (1,)]:
+        ref = x.copy()
+        for ax in axes:
+            N = x.shape[ax]
+            n = np.arange(N)
+            Cm = 2.0 * np.cos(np.pi * np.outer(n, 2 * n + 1) / (2 * N))
+            ref = np.moveaxis(np.tensordot(Cm, np.moveaxis(ref, ax, 0), axes=(1, 0)), 0, ax)
+        assert O.rel_l2(sfft.dctn(x, axes=axes), ref) < 1e-13
+        assert O.rel_l2(sfft.idctn(ref, axes=axes), x) < 1e-13
+    assert O.rel_l2(O.l1dct_prox(x, 0.0, (True, True, True)), x) < 1e-13
+    assert np.abs(O.l1dct_prox(x, 1e6, (True, True, True))).max() == 0.0
+    # a leading batch axis is not transformed
+    xb = rnd((2, 4, 5))
+    pb = O.l1dct_prox(xb, 0.5, (True, True))
+    assert O.rel_l2(pb[1], O.l1dct_prox(xb[1], 0.5, (True, True))) < 1e-13
