@@ -548,6 +548,27 @@ def five_point_llr(coords_list, smaps, kdata_list, weights_list, x0, stepsize: f
     return x.reshape((F, 5) + sp)
 
 
+def create_spoke(samp_per_spoke: int, xangle: float, zangle: float, crop_factor: float = 1.0,
+                 jitter: Optional[np.ndarray] = None) -> np.ndarray:
+    """create_spoke, method 'PCVIPR' (python/python/hastypy/util/image_creation.py:13-57),
+    with the angles given and the Gaussian jitter passed in ([3, S], zero if None): the
+    line k_z = pi linspace(-1/3, 1, S) plus jitter, rotated by Rz(zangle) Rx(xangle), then
+    scaled by crop_factor and cut to |k_d| < pi when crop_factor != 1."""
+    S = samp_per_spoke
+    spoke = np.zeros((3, S))
+    if jitter is not None:
+        spoke += jitter
+    spoke[2] += np.pi * np.linspace(-1.0 / 3.0, 1.0, S)
+    cx, sx, cz, sz = math.cos(xangle), math.sin(xangle), math.cos(zangle), math.sin(zangle)
+    rx = np.array([[1.0, 0.0, 0.0], [0.0, cx, -sx], [0.0, sx, cx]])
+    rz = np.array([[cz, -sz, 0.0], [sz, cz, 0.0], [0.0, 0.0, 1.0]])
+    spoke = rz @ rx @ spoke
+    if crop_factor != 1.0:
+        spoke = spoke * crop_factor
+        spoke = spoke[:, np.all(np.abs(spoke) < np.pi, axis=0)]
+    return spoke
+
+
 def l1dct_prox(x: np.ndarray, lam: float, axis: Sequence[bool]) -> np.ndarray:
     """ProximalL1DCT.tensor_apply (python/python/hastypy/base/proximal.py:139-163): over
     every trailing volume of len(axis) dims, dctn over the axes marked True (scipy's

@@ -198,3 +198,27 @@ def test_five_point_full_vs_oracle(B):
     e = O.rel_l2(xg, ref)
     print(f"FivePointFULL {iters} iterations: {e:.2e}", flush=True)
     assert e < 1e-4
+
+
+def test_create_spokes_vs_oracle():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from hastycompute_amd import simulate
+
+    xa, za = [0.3, 2.0, 1.2], [1.1, 5.0, 0.2]
+    sp = simulate.create_spokes(3, 40, noise=0.0, xangles=xa, zangles=za).cpu().numpy()
+    for i in range(3):
+        ref = O.create_spoke(40, xa[i], za[i])
+        assert np.abs(sp[:, i * 40:(i + 1) * 40] - ref).max() < 1e-5
+    cr = simulate.create_spokes(3, 40, noise=0.0, crop_factor=1.4, xangles=xa, zangles=za)
+    for i in range(3):
+        ref = O.create_spoke(40, xa[i], za[i], crop_factor=1.4)
+        assert cr[i].shape == ref.shape and np.abs(cr[i].cpu().numpy() - ref).max() < 1e-5
+    # jitter: the samples leave the line by N(0, noise) per axis
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1)
+    jit = simulate.create_spokes(400, 64, noise=0.01, xangles=[0.0] * 400, zangles=[0.0] * 400, generator=g)
+    std = float(jit[0].std())
+    assert 0.009 < std < 0.011
+    rnd_angles = simulate.create_spokes(200, 16, generator=g)
+    assert rnd_angles.shape == (3, 3200) and float(rnd_angles.abs().max()) < 1.1 * np.pi

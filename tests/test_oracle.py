@@ -397,3 +397,19 @@ def test_dct_matches_cosine_sums_and_l1dct_prox():
     xb = rnd((2, 4, 5))
     pb = O.l1dct_prox(xb, 0.5, (True, True))
     assert O.rel_l2(pb[1], O.l1dct_prox(xb[1], 0.5, (True, True))) < 1e-13
+
+
+def test_create_spoke_closed_form():
+    """Without jitter a PC-VIPR spoke is t (sin x sin z, -sin x cos z, cos x) for
+    t = pi linspace(-1/3, 1, S): a swapped rotation order or a sign error in either
+    rotation fails; cropping keeps exactly the samples inside the cube."""
+    S = 50
+    for xa, za in [(0.3, 1.1), (2.0, 5.0), (0.0, 0.7)]:
+        t = np.pi * np.linspace(-1.0 / 3.0, 1.0, S)
+        d = np.array([np.sin(xa) * np.sin(za), -np.sin(xa) * np.cos(za), np.cos(xa)])
+        sp = O.create_spoke(S, xa, za)
+        assert np.abs(sp - d[:, None] * t[None, :]).max() < 1e-12
+        c = O.create_spoke(S, xa, za, crop_factor=1.5)
+        full = 1.5 * d[:, None] * t[None, :]
+        kept = full[:, np.all(np.abs(full) < np.pi, axis=0)]
+        assert c.shape == kept.shape and np.abs(c - kept).max() < 1e-12
