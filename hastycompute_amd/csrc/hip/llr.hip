@@ -8,8 +8,9 @@
 // with g = max(s - lambda, 0)/s (soft) or 1 for the `rank` largest s (hard).
 //   1. gram:  MFMA 16x16x4 f32 (FP32 accumulate), upper triangle of G, rows split over
 //             workgroups into partial Grams summed in a fixed order (deterministic);
-//   2. eig:   cyclic complex Jacobi in FP32 in LDS, parallel (round-robin) ordering,
-//             then W = V diag(g) V^H;
+//   2. eig:   cyclic complex Jacobi in FP32 on the upper triangle in LDS, parallel
+//             (round-robin) ordering, rotations recorded; the eigenvectors that survive
+//             the threshold are rebuilt from them, then W = V diag(g) V^H;
 //   3. apply: MFMA A <- A W in place, 16-row slabs staged through LDS.
 // Blocks are read and written directly in the image; no gather/scatter copies.
 #include <algorithm>
@@ -25,6 +26,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kThreads = 256;
 constexpr int kMaxTilesPerWave = 9;  // F <= 128 -> 8x8 tiles -> 36 upper tiles / 4 waves
+constexpr int kMaxSweeps = 15;       // Jacobi sweeps per block (4-6 reach FP32 rounding)
 
 __device__ __forceinline__ int64_t row_addr(const LlrGeom& g, int3 c, int64_t r)
 {
@@ -135,70 +137,136 @@ __global__ void __launch_bounds__(kThreads) gram_kernel(const float2* __restrict
     }
 }
 
+// Element (x, y), x <= y, of the row-major upper triangle of an n x n matrix.
+__device__ __forceinline__ int tri_idx(int x, int y, int n) { return ((x * (2 * n - x + 1)) >> 1) + (y - x); }
+
+// G(x, y) of a Hermitian matrix kept as its upper triangle U.
+__device__ __forceinline__ float2 herm_get(const float2* U, int x, int y, int n)
+{
+    const bool lower = x > y;
+    const float2 v = U[lower ? tri_idx(y, x, n) : tri_idx(x, y, n)];
+    return float2{v.x, lower ? -v.y : v.y};
+}
+__device__ __forceinline__ void herm_put(float2* U, int x, int y, int n, float2 v)
+{
+    const bool lower = x > y;
+    U[lower ? tri_idx(y, x, n) : tri_idx(x, y, n)] = float2{v.x, lower ? -v.y : v.y};
+}
+
+// Orders the LDS accesses of one wave: what a lane wrote is visible to every lane after it.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sum over the workgroup, returned to every thread.
+__device__ double block_sum(double v, double* red)
+{
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+    for (int o = kThreads / 2; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// Byte offsets of the eigensolver's LDS arrays for an n x n problem (n even):
+//   U   [n (n + 1) / 2] complex  upper triangle of G (later: eigenvectors)
+//   rc  [n / 2] float4           rotation of each pair: c, s, cos(phi), sin(phi)
+//   pq  [n / 2] int              pair positions p | q << 8
+//   gk  [n] float                g of each eigenvalue
+//   kl  [n] int                  kept columns
+//   tbl [(n/2) (n/2 + 1) / 2]    (k, l) of each upper 2x2 block as k | l << 8
+struct EigLayout {
+    int rc, pq, gk, kl, tbl, bytes;
+    __host__ __device__ explicit EigLayout(int n)
+    {
+        const int half = n / 2;
+        const int nb = half * (half + 1) / 2;
+        rc = ((8 * (n * (n + 1) / 2) + 15) / 16) * 16;
+        pq = rc + 16 * half;
+        gk = pq + 4 * half;
+        kl = gk + 4 * n;
+        tbl = kl + 4 * n;
+        bytes = tbl + ((2 * nb + 15) / 16) * 16;
+    }
+};
+
 // Jacobi eigensolver + W = V diag(g) V^H, one workgroup per block.
-// LDS: G[n][n] and V[n][n] complex float (n = F rounded up to even).
+//
+// G (n = F rounded up to even) lives in LDS as its upper triangle. Each round of the
+// round-robin ordering rotates n/2 disjoint pairs: the rotations are written to `rot`
+// and G <- J^H G J is applied to the upper 2x2 blocks only. V is never formed: after
+// convergence, the eigenvectors with g != 0 are rebuilt as v_k = J_1 J_2 ... J_R e_k by
+// replaying the recorded rotations backwards, one wave per vector (its lanes are the
+// pairs, so a round needs no workgroup barrier). The work of that pass scales with the
+// number of singular values that survive the threshold.
 __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restrict__ gpart, int nsplit, int F, int Fp,
                                                         float thresh, int soft, int rank, int max_sweeps,
-                                                        float2* __restrict__ Wout, float* __restrict__ eigout)
+                                                        float4* rot_all, float2* __restrict__ Wout,
+                                                        float* __restrict__ eigout)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = F + (F & 1);
-    float2* G = reinterpret_cast<float2*>(smem_raw);
-    float2* V = G + n * n;
-    float* rc = reinterpret_cast<float*>(V + n * n);  // per pair: c, s, cos(phi), sin(phi)
-    int* pp = reinterpret_cast<int*>(rc + 4 * (n / 2));
-    int* pq = pp + n / 2;
+    const int half = n / 2;
+    const int ntri = n * (n + 1) / 2;
+    const int nb = half * (half + 1) / 2;
+    const EigLayout L(n);
+    float2* U = reinterpret_cast<float2*>(smem_raw);
+    float4* rcs = reinterpret_cast<float4*>(smem_raw + L.rc);
+    int* pq = reinterpret_cast<int*>(smem_raw + L.pq);
+    float* gk = reinterpret_cast<float*>(smem_raw + L.gk);
+    int* kl = reinterpret_cast<int*>(smem_raw + L.kl);
+    unsigned short* tbl = reinterpret_cast<unsigned short*>(smem_raw + L.tbl);
     __shared__ double red[kThreads];
+    __shared__ int nk_s;
     const int blk = blockIdx.x;
     const int tid = threadIdx.x;
+    float4* rot = rot_all + (int64_t)blk * max_sweeps * (n - 1) * half;
 
-    // Load G (sum of partial Grams in split order), Hermitian-symmetrised; V = I.
+    // Upper triangle of G (sum of the partial Grams in split order); block table.
     const float2* Gb = gpart + (int64_t)blk * nsplit * Fp * Fp;
     for (int e = tid; e < n * n; e += kThreads) {
-        const int i = e / n, j = e % n;
+        const int i = e / n, j = e - (e / n) * n;
+        if (j < i) continue;
         float2 v = float2{0.f, 0.f};
-        if (i < F && j < F) {
-            const int ii = min(i, j), jj = max(i, j);
+        if (j < F) {
             for (int s = 0; s < nsplit; ++s) {
-                const float2 t = Gb[(int64_t)s * Fp * Fp + (int64_t)ii * Fp + jj];
+                const float2 t = Gb[(int64_t)s * Fp * Fp + (int64_t)i * Fp + j];
                 v.x += t.x;
                 v.y += t.y;
             }
-            if (i > j) v.y = -v.y;
             if (i == j) v.y = 0.f;
         }
-        G[e] = v;
-        V[e] = float2{i == j ? 1.f : 0.f, 0.f};
+        U[tri_idx(i, j, n)] = v;
+    }
+    for (int e = tid; e < half * half; e += kThreads) {
+        const int k = e / half, l = e - (e / half) * half;
+        if (l >= k) tbl[tri_idx(k, l, half)] = (unsigned short)(k | (l << 8));
     }
     __syncthreads();
 
-    // ||G||_F^2 for the stopping rule
-    double nrm = 0.0;
-    for (int e = tid; e < n * n; e += kThreads) nrm += (double)G[e].x * G[e].x + (double)G[e].y * G[e].y;
-    red[tid] = nrm;
-    __syncthreads();
-    for (int off = kThreads / 2; off > 0; off >>= 1) {
-        if (tid < off) red[tid] += red[tid + off];
-        __syncthreads();
-    }
-    const double total = red[0];
-    __syncthreads();
+    // |G|_F^2 = 2 sum(triangle) - sum(diagonal); the off-diagonal mass is 2 (sum(triangle) - sum(diagonal))
+    auto tri_minus_diag = [&](double wt) {
+        double a = 0.0;
+        for (int e = tid; e < ntri; e += kThreads) a += wt * ((double)U[e].x * U[e].x + (double)U[e].y * U[e].y);
+        for (int x = tid; x < n; x += kThreads) {
+            const float2 d = U[tri_idx(x, x, n)];
+            a -= (double)d.x * d.x + (double)d.y * d.y;
+        }
+        return a;
+    };
+    const double total = block_sum(tri_minus_diag(2.0), red);
 
-    const int half = n / 2;
-    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-        double off = 0.0;
-        for (int e = tid; e < n * n; e += kThreads) {
-            const int i = e / n, j = e % n;
-            if (i != j) off += (double)G[e].x * G[e].x + (double)G[e].y * G[e].y;
-        }
-        red[tid] = off;
-        __syncthreads();
-        for (int o = kThreads / 2; o > 0; o >>= 1) {
-            if (tid < o) red[tid] += red[tid + o];
-            __syncthreads();
-        }
-        const double offn = red[0];
-        __syncthreads();
+    int sweeps = 0;
+    for (; sweeps < max_sweeps; ++sweeps) {
+        const double offn = 2.0 * block_sum(tri_minus_diag(1.0), red);
         // off-diagonal mass below (1e-5 |G|_F)^2: at FP32 rounding (n eps |G| ~ 6e-6 |G|
         // for n = 100) further sweeps stop paying
         if (offn <= 1e-10 * total || total == 0.0) break;
@@ -207,12 +275,11 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
             // round-robin pairing: position 0 fixed, positions 1..n-1 rotate
             if (tid < half) {
                 const int a = tid == 0 ? 0 : 1 + (tid - 1 + round) % (n - 1);
-                const int b = 1 + (n - 1 - tid - 1 + round) % (n - 1);
+                const int b = 1 + (n - 2 - tid + round) % (n - 1);
                 const int p = min(a, b), q = max(a, b);
-                pp[tid] = p;
-                pq[tid] = q;
-                const float ap = G[p * n + p].x, dq = G[q * n + q].x;
-                const float2 bpq = G[p * n + q];
+                pq[tid] = p | (q << 8);
+                const float ap = U[tri_idx(p, p, n)].x, dq = U[tri_idx(q, q, n)].x;
+                const float2 bpq = U[tri_idx(p, q, n)];
                 const float ab = hypotf(bpq.x, bpq.y);
                 float c = 1.f, s = 0.f, cf = 1.f, sf = 0.f;
                 if (ab > 1e-30f && ab > 1e-9f * (fabsf(ap) + fabsf(dq))) {
@@ -223,24 +290,27 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
                     c = 1.f / sqrtf(1.f + t * t);
                     s = t * c;
                 }
-                rc[4 * tid + 0] = c;
-                rc[4 * tid + 1] = s;
-                rc[4 * tid + 2] = cf;
-                rc[4 * tid + 3] = sf;
+                const float4 r4 = make_float4(c, s, cf, sf);
+                rcs[tid] = r4;
+                rot[(int64_t)(sweeps * (n - 1) + round) * half + tid] = r4;
             }
             __syncthreads();
             // G <- J^H G J, block by block: the rows of pair k and the columns of pair l
-            // form a 2x2 block that maps to J_k^H G_kl J_l, read and written by one thread
-            // (rows: r_p = c r_p - s e^{i phi} r_q, r_q = s r_p + c e^{i phi} r_q; columns the
-            // same with e^{-i phi}). The diagonal blocks end exactly diagonal and real.
-            for (int e = tid; e < half * half; e += kThreads) {
-                const int k = e / half, l = e % half;
-                const int pk = pp[k], qk = pq[k], pl = pp[l], ql = pq[l];
-                const float ck = rc[4 * k], sk = rc[4 * k + 1];
-                const float2 phk = float2{rc[4 * k + 2], rc[4 * k + 3]};
-                const float cl = rc[4 * l], sl = rc[4 * l + 1];
-                const float2 phl = float2{rc[4 * l + 2], -rc[4 * l + 3]};
-                const float2 a = G[pk * n + pl], bb = G[pk * n + ql], cc = G[qk * n + pl], d = G[qk * n + ql];
+            // form a 2x2 block that maps to J_k^H G_kl J_l (rows: r_p = c r_p - s e^{i phi} r_q,
+            // r_q = s r_p + c e^{i phi} r_q; columns the same with e^{-i phi}). Only the blocks
+            // with k <= l are formed; each element of the triangle belongs to exactly one of
+            // them, and the diagonal blocks end exactly diagonal and real.
+            for (int e = tid; e < nb; e += kThreads) {
+                const int t = tbl[e];
+                const int k = t & 255, l = t >> 8;
+                const int pqk = pq[k], pql = pq[l];
+                const int pk = pqk & 255, qk = pqk >> 8, pl = pql & 255, ql = pql >> 8;
+                const float4 rk = rcs[k], rl = rcs[l];
+                const float ck = rk.x, sk = rk.y, cl = rl.x, sl = rl.y;
+                const float2 phk = float2{rk.z, rk.w};
+                const float2 phl = float2{rl.z, -rl.w};
+                const float2 a = herm_get(U, pk, pl, n), bb = herm_get(U, pk, ql, n);
+                const float2 cc = herm_get(U, qk, pl, n), d = herm_get(U, qk, ql, n);
                 const float2 ea = cmul(phk, cc), eb = cmul(phk, d);
                 const float2 a1 = float2{ck * a.x - sk * ea.x, ck * a.y - sk * ea.y};
                 const float2 b1 = float2{ck * bb.x - sk * eb.x, ck * bb.y - sk * eb.y};
@@ -248,38 +318,30 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
                 const float2 d1 = float2{sk * bb.x + ck * eb.x, sk * bb.y + ck * eb.y};
                 const float2 fb = cmul(phl, b1), fd = cmul(phl, d1);
                 float2 na = float2{cl * a1.x - sl * fb.x, cl * a1.y - sl * fb.y};
-                float2 nb = float2{sl * a1.x + cl * fb.x, sl * a1.y + cl * fb.y};
-                float2 nc = float2{cl * c1.x - sl * fd.x, cl * c1.y - sl * fd.y};
+                float2 nbv = float2{sl * a1.x + cl * fb.x, sl * a1.y + cl * fb.y};
+                const float2 nc = float2{cl * c1.x - sl * fd.x, cl * c1.y - sl * fd.y};
                 float2 nd = float2{sl * c1.x + cl * fd.x, sl * c1.y + cl * fd.y};
                 if (k == l) {
                     na.y = 0.f;
                     nd.y = 0.f;
-                    if (sk != 0.f) nb = nc = float2{0.f, 0.f};
+                    if (sk != 0.f) nbv = float2{0.f, 0.f};
+                    U[tri_idx(pk, pk, n)] = na;
+                    U[tri_idx(pk, qk, n)] = nbv;
+                    U[tri_idx(qk, qk, n)] = nd;
+                } else {
+                    herm_put(U, pk, pl, n, na);
+                    herm_put(U, pk, ql, n, nbv);
+                    herm_put(U, qk, pl, n, nc);
+                    herm_put(U, qk, ql, n, nd);
                 }
-                G[pk * n + pl] = na;
-                G[pk * n + ql] = nb;
-                G[qk * n + pl] = nc;
-                G[qk * n + ql] = nd;
-            }
-            // V <- V J (the columns of each pair)
-            for (int e = tid; e < half * n; e += kThreads) {
-                const int k = e % half, i = e / half;
-                const int p = pp[k], q = pq[k];
-                const float c = rc[4 * k], s = rc[4 * k + 1];
-                const float2 phc = float2{rc[4 * k + 2], -rc[4 * k + 3]};
-                const float2 vp = V[i * n + p], vq = V[i * n + q];
-                const float2 eq = cmul(phc, vq);
-                V[i * n + p] = float2{c * vp.x - s * eq.x, c * vp.y - s * eq.y};
-                V[i * n + q] = float2{s * vp.x + c * eq.x, s * vp.y + c * eq.y};
             }
             __syncthreads();
         }
     }
 
-    // g_k from eigenvalues (stored in rc[0..n) after the rotation scratch is done)
-    float* gk = rc;
+    // g_k from the eigenvalues (the diagonal)
     for (int k = tid; k < n; k += kThreads) {
-        const float lam = G[k * n + k].x;
+        const float lam = U[tri_idx(k, k, n)].x;
         float gv = 0.f;
         if (k < F) {
             if (soft) {
@@ -288,7 +350,7 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
             } else {
                 int cnt = 0;
                 for (int j = 0; j < F; ++j) {
-                    const float lj = G[j * n + j].x;
+                    const float lj = U[tri_idx(j, j, n)].x;
                     if (lj > lam || (lj == lam && j < k)) ++cnt;
                 }
                 gv = cnt < rank ? 1.f : 0.f;
@@ -298,22 +360,96 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
         gk[k] = gv;
     }
     __syncthreads();
+    // kept columns, in increasing order
+    const int lane = tid & 63, wave = tid >> 6;
+    if (wave == 0) {
+        int base = 0;
+        for (int c = 0; c < n; c += 64) {
+            const int x = c + lane;
+            const bool keep = x < n && gk[x] != 0.f;
+            const unsigned long long m = __ballot(keep);
+            if (keep) kl[base + __popcll(m & ((1ull << lane) - 1ull))] = x;
+            base += __popcll(m);
+        }
+        if (lane == 0) nk_s = base;
+    }
+    __syncthreads();
+    const int nk = nk_s;
 
+    // The triangle is free now: it holds up to n/2 eigenvectors X[v][n] at a time.
+    float2* X = U;
     float2* W = Wout + (int64_t)blk * Fp * Fp;
-    for (int e = tid; e < Fp * Fp; e += kThreads) {
-        const int i = e / Fp, j = e % Fp;
-        float2 acc = float2{0.f, 0.f};
-        if (i < F && j < F) {
-            for (int k = 0; k < n; ++k) {
-                const float g = gk[k];
-                if (g == 0.f) continue;
-                const float2 vi = V[i * n + k], vj = V[j * n + k];
-                // V[i][k] g conj(V[j][k])
-                acc.x += g * (vi.x * vj.x + vi.y * vj.y);
-                acc.y += g * (vi.y * vj.x - vi.x * vj.y);
+    const int R = sweeps * (n - 1);
+    const int cap = half;
+    for (int c0 = 0; c0 < max(nk, 1); c0 += cap) {
+        const int nkc = min(cap, nk - c0);
+        for (int e = tid; e < nkc * n; e += kThreads) {
+            const int j = e / n, x = e - (e / n) * n;
+            X[e] = float2{x == kl[c0 + j] ? 1.f : 0.f, 0.f};
+        }
+        __syncthreads();
+        if (wave < nkc) {
+            // lane m < n/2 applies the rotation of pair m to the wave's vectors, round R-1
+            // down to 0; rotations are prefetched 8 rounds ahead. With rr = r mod (n-1) the
+            // pair is (1 + (m - 1 + rr) mod (n-1), 1 + (n - 2 - m + rr) mod (n-1)), m = 0
+            // taking position 0 for the first; both residues step down by one per round.
+            const bool act = lane < half;
+            int A = act && lane > 0 ? (lane - 1 + n - 2) % (n - 1) : 0;
+            int B = act ? (2 * n - 4 - lane) % (n - 1) : 0;
+            float4 cur[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int r = R - 1 - j;
+                cur[j] = act && r >= 0 ? rot[(int64_t)r * half + lane] : make_float4(1.f, 0.f, 1.f, 0.f);
+            }
+            for (int r0 = R - 1; r0 >= 0; r0 -= 8) {
+                float4 nxt[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int r = r0 - 8 - j;
+                    nxt[j] = act && r >= 0 ? rot[(int64_t)r * half + lane] : make_float4(1.f, 0.f, 1.f, 0.f);
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (r0 - j < 0) break;
+                    if (act) {
+                        const int a = lane == 0 ? 0 : 1 + A;
+                        const int b = 1 + B;
+                        const int p = min(a, b), q = max(a, b);
+                        const float c = cur[j].x, s = cur[j].y;
+                        const float2 eph = float2{cur[j].z, -cur[j].w};
+                        for (int v = wave; v < nkc; v += kThreads / 64) {
+                            float2* xv = X + v * n;
+                            const float2 xp = xv[p], xq = xv[q];
+                            xv[p] = float2{c * xp.x + s * xq.x, c * xp.y + s * xq.y};
+                            xv[q] = cmul(eph, float2{c * xq.x - s * xp.x, c * xq.y - s * xp.y});
+                        }
+                        A = A == 0 ? n - 2 : A - 1;
+                        B = B == 0 ? n - 2 : B - 1;
+                    }
+                    wave_lds_sync();
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) cur[j] = nxt[j];
             }
         }
-        W[e] = acc;
+        __syncthreads();
+        // W (+)= sum_v g_v X_v X_v^H
+        for (int e = tid; e < Fp * Fp; e += kThreads) {
+            const int i = e / Fp, j = e - (e / Fp) * Fp;
+            float2 acc = c0 == 0 ? float2{0.f, 0.f} : W[e];
+            if (i < F && j < F) {
+                for (int v = 0; v < nkc; ++v) {
+                    const float g = gk[kl[c0 + v]];
+                    const float2 vi = X[v * n + i], vj = X[v * n + j];
+                    // v[i] g conj(v[j])
+                    acc.x += g * (vi.x * vj.x + vi.y * vj.y);
+                    acc.y += g * (vi.y * vj.x - vi.x * vj.y);
+                }
+            }
+            W[e] = acc;
+        }
+        __syncthreads();
     }
 }
 
@@ -412,24 +548,29 @@ __global__ void __launch_bounds__(kThreads) apply_kernel(float2* __restrict__ da
 
 int llr_padded(int64_t F) { return (int)(((F + 15) / 16) * 16); }
 
-size_t llr_eig_smem(int64_t F)
+size_t llr_eig_smem(int64_t F) { return (size_t)EigLayout((int)(F + (F & 1))).bytes; }
+
+// Recorded Jacobi rotations per block: kMaxSweeps sweeps of n - 1 rounds of n / 2 pairs.
+static int64_t rot_per_block(int64_t F)
 {
-    const int n = (int)(F + (F & 1));
-    return sizeof(float2) * 2 * (size_t)n * n + sizeof(float) * 4 * (n / 2) + sizeof(int) * n;
+    const int64_t n = F + (F & 1);
+    return (int64_t)kMaxSweeps * (n - 1) * (n / 2);
 }
 
-// Workspace: gpart [nblk][nsplit][Fp][Fp] + W [nblk][Fp][Fp] complex float.
+// Workspace: gpart [nblk][nsplit][Fp][Fp] + W [nblk][Fp][Fp] complex float, then the
+// rotations [nblk][kMaxSweeps][n - 1][n / 2] float4.
 size_t llr_workspace_bytes(int64_t nblk, int64_t F, int nsplit)
 {
     const int64_t Fp = llr_padded(F);
-    return sizeof(float2) * (size_t)nblk * (size_t)(nsplit + 1) * Fp * Fp;
+    return sizeof(float2) * (size_t)nblk * (size_t)(nsplit + 1) * Fp * Fp +
+           sizeof(float4) * (size_t)nblk * (size_t)rot_per_block(F);
 }
 
 void llr_check(int64_t F)
 {
     HASTY_REQUIRE(F >= 1 && F <= 128, "LLR supports 1..128 columns (frames)");
     // dynamic LDS of the eigensolver plus its 2 KiB static reduction buffer
-    HASTY_REQUIRE(llr_eig_smem(F) + sizeof(double) * kThreads <= 160 * 1024, "LLR eigensolver tile exceeds LDS");
+    HASTY_REQUIRE(llr_eig_smem(F) + sizeof(double) * kThreads + 16 <= 160 * 1024, "LLR eigensolver tile exceeds LDS");
 }
 
 // Threshold the blocks at `corners_dev` (all disjoint) in place.
@@ -444,6 +585,7 @@ void llr_blocks(float2* data, const LlrGeom& g, const int3* corners_dev, int64_t
     const int64_t R = g.E * g.bx * g.by * g.bz;
     float2* gpart = reinterpret_cast<float2*>(ws);
     float2* W = gpart + (int64_t)nblk * nsplit * Fp * Fp;
+    float4* rot = reinterpret_cast<float4*>(W + (int64_t)nblk * Fp * Fp);
     int64_t rps = ceil_div(ceil_div(R, nsplit), 16) * 16;
     {
         const size_t smem = sizeof(float) * 2 * 16 * Fp;
@@ -457,7 +599,7 @@ void llr_blocks(float2* data, const LlrGeom& g, const int3* corners_dev, int64_t
         allow_lds(eig_w_kernel, smem);
         KernelTimer t("llr_eig", s);
         hipLaunchKernelGGL(eig_w_kernel, dim3((unsigned)nblk), dim3(kThreads), smem, s, (const float2*)gpart, nsplit,
-                           (int)g.F, Fp, thresh, soft, rank, 15, W, eig_out);
+                           (int)g.F, Fp, thresh, soft, rank, kMaxSweeps, rot, W, eig_out);
         check_launch("eig_w_kernel");
     }
     {

@@ -296,3 +296,31 @@ def test_svt_matrix_vs_numpy(N):
         e2 = O.rel_l2(h(Ah), O.svt_hard(A.astype(np.complex128), 4))
         print(f"svt R={R} F={F}: soft {e:.2e} hard {e2:.2e}", flush=True)
         assert e < 1e-4 and e2 < 1e-4
+
+
+@pytest.mark.parametrize("R,F", [(300, 33), (400, 128), (250, 100), (40, 7)])
+def test_svt_matrix_many_kept(N, R, F):
+    """Thresholds that keep all, about half, or one singular value: the eigenvectors are
+    rebuilt in chunks of F/2, so keeping more than F/2 takes several; odd F pads G."""
+    rng = np.random.default_rng(R + F)
+    # prescribed spectrum, halved at the two hard-threshold ranks so the kept subspace is
+    # well separated (a Gram matrix in FP32 resolves gaps down to ~ F eps |G|)
+    k = np.arange(F)
+    sv = (1.0 + 0.5 * (1.0 - k / F)) * np.where(k >= F - 1, 0.5, 1.0) * np.where(k >= F // 2 + 1, 0.5, 1.0)
+    sv[0] *= 3.0
+    Q1 = np.linalg.qr(rnd(rng, (R, F)))[0]
+    Q2 = np.linalg.qr(rnd(rng, (F, F)))[0]
+    A = ((Q1 * sv[None, :]) @ Q2.conj().T).astype(np.complex64)
+    s = np.linalg.svd(A.astype(np.complex128), compute_uv=False)
+    for lam in (0.5 * s[-1], s[F // 2], 0.5 * (s[0] + s[1])):
+        Ag = g(A)
+        N.svt_matrix(Ag, lam, True)
+        e = O.rel_l2(h(Ag), O.svt_soft(A.astype(np.complex128), lam))
+        print(f"svt R={R} F={F} kept={int((s > lam).sum())}: soft {e:.2e}", flush=True)
+        assert e < 1e-4
+    for rank in (F - 1, F // 2 + 1):
+        Ah = g(A)
+        N.svt_matrix(Ah, rank, False)
+        e2 = O.rel_l2(h(Ah), O.svt_hard(A.astype(np.complex128), rank))
+        print(f"svt R={R} F={F} hard rank={rank}: {e2:.2e}", flush=True)
+        assert e2 < 1e-4
