@@ -454,94 +454,145 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
 }
 
 // A <- A W for rows [chunk*rows_per_wg, ...) of each block, in place in the image.
-__global__ void __launch_bounds__(kThreads) apply_kernel(float2* __restrict__ data, LlrGeom g, const int3* corners, int Fp,
-                                                        int64_t rows_per_wg, const float2* __restrict__ Wall)
+//
+// Each wave owns 16-row slabs and forms out^T = W^T A^T with v_mfma_f32_16x16x4f32:
+// the A^T operand (lane: row l & 15, frame 4 ks + l / 16) comes straight from the image,
+// 16 lanes reading one 128-byte z-line of a frame, and the slab's output tile lands with
+// the same lane layout, so loads and stores are whole lines. W^T comes from LDS (loaded
+// once per workgroup). Complex products take three real MFMAs (Gauss):
+//   P1 = Wr Ar, P2 = Wi Ai, P3 = (Wr + Wi)(Ar + Ai);  Re = P1 - P2, Im = P3 - P1 - P2.
+// K runs over F rounded up to 4 only. A wave reads a slab whole before writing it and
+// slabs are disjoint, so the update is in place; the next slab is loaded during the
+// current one's MFMAs. The main loop has no workgroup barrier.
+constexpr int kApplyThreads = 512;
+constexpr int kApplySlabsPerWave = 4;
+
+// LDS row stride of W (complex elements): an odd multiple of 16, so the four k rows one
+// MFMA operand reads fall in disjoint bank groups.
+__host__ __device__ inline int apply_stride(int Fp) { return ((Fp / 16) % 2 == 0) ? Fp + 16 : Fp; }
+
+template <int NT>
+__global__ void __launch_bounds__(kApplyThreads) apply_kernel(float2* __restrict__ data, LlrGeom g,
+                                                              const int3* corners, int64_t rows_per_wg,
+                                                              const float2* __restrict__ Wall)
 {
+    constexpr int Fp = 16 * NT;
+    constexpr int KS = 4 * NT;  // k-steps for F = Fp
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int Fs = Fp + 1;  // padded row stride of the A slab
-    float* sWr = reinterpret_cast<float*>(smem_raw);
-    float* sWi = sWr + Fp * Fp;
-    float* sAr = sWi + Fp * Fp;
-    float* sAi = sAr + 16 * Fs;
+    const int F = (int)g.F;
+    const int K4 = (F + 3) / 4;
+    const int S = apply_stride(Fp);
+    float2* sW = reinterpret_cast<float2*>(smem_raw);  // W[k][j] at k * S + j
     const int blk = blockIdx.x;
     const int3 c = corners[blk];
     const int64_t R = g.E * g.bx * g.by * g.bz;
     const int64_t r_begin = (int64_t)blockIdx.y * rows_per_wg;
     const int64_t r_end = min(R, r_begin + rows_per_wg);
     if (r_begin >= r_end) return;
-    const int F = (int)g.F;
-    const int nt = Fp / 16;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const int n = lane & 15;   // slab row of this lane
+    const int kg = lane >> 4;  // k within a k-step
 
     const float2* W = Wall + (int64_t)blk * Fp * Fp;
-    for (int e = threadIdx.x; e < Fp * Fp; e += kThreads) {
-        const float2 w = W[e];
-        sWr[e] = w.x;
-        sWi[e] = w.y;
-    }
-    for (int e = threadIdx.x; e < 16 * Fs; e += kThreads) {
-        sAr[e] = 0.f;
-        sAi[e] = 0.f;
+    for (int e = threadIdx.x; e < 4 * K4 * Fp; e += kApplyThreads) {
+        const int k = e / Fp, j = e - (e / Fp) * Fp;
+        sW[k * S + j] = k < F ? W[(int64_t)k * Fp + j] : float2{0.f, 0.f};
     }
     __syncthreads();
 
-    for (int64_t r0 = r_begin; r0 < r_end; r0 += 16) {
-        for (int e = threadIdx.x; e < 16 * F; e += kThreads) {
-            const int kr = e & 15;
-            const int f = e >> 4;
-            const int64_t r = r0 + kr;
-            float2 v = float2{0.f, 0.f};
-            if (r < r_end) v = data[row_addr(g, c, r) + (int64_t)f * g.fstride];
-            sAr[kr * Fs + f] = v.x;
-            sAi[kr * Fs + f] = v.y;
-        }
-        __syncthreads();
-        floatx4 oR[2], oI[2];
+    // one k-step per register: lane holds A[row r0 + n][frame 4 ks + kg]. Frames past F
+    // (the last k-step) read frame F - 1 and rows past the end read the last row: both
+    // meet zero rows of W or are not stored, so no per-lane load masks are needed.
+    auto load_slab = [&](int64_t r0, float2* a, int64_t& addr, bool& valid) {
+        const int64_t r = r0 + n;
+        valid = r < r_end;
+        addr = row_addr(g, c, valid ? r : r_end - 1);
+        int64_t fs = g.fstride;
+        int fm = F - 1, k4 = K4;
+        asm volatile("" : "+s"(fs), "+s"(fm), "+s"(k4));  // opaque per slab: nothing here is hoisted out of the slab loop
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            oR[k] = floatx4{0.f, 0.f, 0.f, 0.f};
-            oI[k] = floatx4{0.f, 0.f, 0.f, 0.f};
-        }
-        const int m = lane & 15;
-        for (int ks = 0; ks < Fp / 4; ++ks) {
-            const int kk = ks * 4 + (lane >> 4);
-            const float ar = sAr[m * Fs + kk];
-            const float ai = sAi[m * Fs + kk];
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int tj = wave + 4 * k;
-                if (tj >= nt) continue;
-                const float wr = sWr[kk * Fp + tj * 16 + m];
-                const float wi = sWi[kk * Fp + tj * 16 + m];
-                oR[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, wr, oR[k], 0, 0, 0);
-                oR[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, wi, oR[k], 0, 0, 0);
-                oI[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, wi, oI[k], 0, 0, 0);
-                oI[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, wr, oI[k], 0, 0, 0);
+        for (int ks = 0; ks < KS; ++ks) {
+            if (ks < k4) {
+                const int k = min(4 * ks + kg, fm);
+                a[ks] = data[addr + (int64_t)k * fs];
+            } else {
+                a[ks] = float2{0.f, 0.f};
             }
         }
-        __syncthreads();  // every wave is done reading the slab
+    };
+
+    const int waves = kApplyThreads / 64;
+    float2 a[KS];
+    int64_t addr;
+    bool valid;
+    int64_t r0 = r_begin + 16 * wave;
+    if (r0 >= r_end) return;  // no barrier follows
+    load_slab(r0, a, addr, valid);
+    for (; r0 < r_end; r0 += 16 * waves) {
+        floatx4 p1[NT], p2[NT], p3[NT];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int tj = wave + 4 * k;
-            if (tj >= nt) continue;
+        for (int t = 0; t < NT; ++t) {
+            p1[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+            p2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+            p3[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+        int woff = kg * S + n, k4 = K4;
+        asm volatile("" : "+v"(woff), "+s"(k4));  // per-k-step LDS bases and tests are formed here, not hoisted
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = 4 * (lane >> 4) + v;
-                const int col = tj * 16 + (lane & 15);
-                sAr[row * Fs + col] = oR[k][v];
-                sAi[row * Fs + col] = oI[k][v];
+        for (int ks = 0; ks < KS; ++ks) {
+            if (ks < k4) {
+                const float ar = a[ks].x, ai = a[ks].y, as = ar + ai;
+                const float2* wrow = sW + woff + 4 * ks * S;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const float2 w = wrow[16 * t];
+                    p1[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.x, ar, p1[t], 0, 0, 0);
+                    p2[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.y, ai, p2[t], 0, 0, 0);
+                    p3[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.x + w.y, as, p3[t], 0, 0, 0);
+                }
             }
         }
-        __syncthreads();
-        for (int e = threadIdx.x; e < 16 * F; e += kThreads) {
-            const int kr = e & 15;
-            const int f = e >> 4;
-            const int64_t r = r0 + kr;
-            if (r < r_end) data[row_addr(g, c, r) + (int64_t)f * g.fstride] = float2{sAr[kr * Fs + f], sAi[kr * Fs + f]};
+        // the operand registers are free: start the next slab's loads before the stores
+        const int64_t addr_out = addr;
+        const bool valid_out = valid;
+        if (r0 + 16 * waves < r_end) load_slab(r0 + 16 * waves, a, addr, valid);
+        // lane holds out[row n][col 16 t + 4 kg + v]; only the last tile can pass F
+        if (valid_out) {
+            int64_t fs = g.fstride;
+            asm volatile("" : "+s"(fs));
+            float2* q = data + addr_out + (int64_t)(4 * kg) * fs;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if (t < NT - 1 || 16 * t + 4 * kg + v < F) {
+                        const float re = p1[t][v] - p2[t][v];
+                        const float im = p3[t][v] - p1[t][v] - p2[t][v];
+                        *q = float2{re, im};
+                    }
+                    q += v < 3 ? fs : 13 * fs;
+                }
+            }
         }
-        __syncthreads();
     }
+}
+
+size_t apply_smem(int64_t F)
+{
+    const int Fp = llr_padded(F);
+    return sizeof(float) * 2 * (size_t)(4 * ((F + 3) / 4)) * apply_stride(Fp);
+}
+
+template <int NT>
+void launch_apply(float2* data, const LlrGeom& g, const int3* corners, int64_t nblk, int64_t rows_per_wg,
+                  int64_t nchunk, const float2* W, hipStream_t s)
+{
+    const size_t smem = apply_smem(g.F);
+    allow_lds(apply_kernel<NT>, smem);
+    hipLaunchKernelGGL(apply_kernel<NT>, dim3((unsigned)nblk, (unsigned)nchunk), dim3(kApplyThreads), smem, s, data,
+                       g, corners, rows_per_wg, W);
+    check_launch("apply_kernel");
 }
 
 }  // namespace
@@ -603,14 +654,21 @@ void llr_blocks(float2* data, const LlrGeom& g, const int3* corners_dev, int64_t
         check_launch("eig_w_kernel");
     }
     {
-        const int64_t rows_per_wg = 256;
+        const int64_t rows_per_wg = 16 * (kApplyThreads / 64) * kApplySlabsPerWave;
         const int64_t nchunk = ceil_div(R, rows_per_wg);
-        const size_t smem = sizeof(float) * (2 * (size_t)Fp * Fp + 2 * 16 * (size_t)(Fp + 1));
-        allow_lds(apply_kernel, smem);
+        HASTY_REQUIRE(nchunk < 65536, "LLR block has too many rows for one launch");
         KernelTimer t("llr_apply", s);
-        hipLaunchKernelGGL(apply_kernel, dim3((unsigned)nblk, (unsigned)nchunk), dim3(kThreads), smem, s, data, g,
-                           corners_dev, Fp, rows_per_wg, (const float2*)W);
-        check_launch("apply_kernel");
+        const float2* Wc = W;
+        switch (Fp / 16) {
+        case 1: launch_apply<1>(data, g, corners_dev, nblk, rows_per_wg, nchunk, Wc, s); break;
+        case 2: launch_apply<2>(data, g, corners_dev, nblk, rows_per_wg, nchunk, Wc, s); break;
+        case 3: launch_apply<3>(data, g, corners_dev, nblk, rows_per_wg, nchunk, Wc, s); break;
+        case 4: launch_apply<4>(data, g, corners_dev, nblk, rows_per_wg, nchunk, Wc, s); break;
+        case 5: launch_apply<5>(data, g, corners_dev, nblk, rows_per_wg, nchunk, Wc, s); break;
+        case 6: launch_apply<6>(data, g, corners_dev, nblk, rows_per_wg, nchunk, Wc, s); break;
+        case 7: launch_apply<7>(data, g, corners_dev, nblk, rows_per_wg, nchunk, Wc, s); break;
+        default: launch_apply<8>(data, g, corners_dev, nblk, rows_per_wg, nchunk, Wc, s); break;
+        }
     }
 }
 
