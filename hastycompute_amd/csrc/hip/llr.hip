@@ -53,12 +53,28 @@ __device__ __forceinline__ void tile_ij(int t, int nt, int& ti, int& tj)
     tj = i + rem;
 }
 
+// Partial Gram G = A^H A over rows [split * rows_per_split, ...) of each block.
+//
+// Rows are staged through LDS in stages of kGramStage rows, frame-major ([f][row], row
+// stride kGramStage + 2 complex, so both the staging stores and the MFMA operand reads
+// are free of bank conflicts), double-buffered: the next stage's loads (32 threads read
+// 32 rows of one frame: whole z-lines) are in flight while the current stage's MFMAs run,
+// one barrier per stage. Each wave owns up to kMaxTilesPerWave upper 16x16 tiles of G;
+// conj(a) b takes three real MFMAs:
+//   P1 = ar br, P2 = ai bi, P3 = (ar - ai)(br + bi);  Re = P1 + P2, Im = P3 - P1 + P2.
+constexpr int kGramStage = 32;
+constexpr int kGramRowStride = kGramStage + 2;
+
+template <int NT>
 __global__ void __launch_bounds__(kThreads) gram_kernel(const float2* __restrict__ data, LlrGeom g, const int3* corners,
-                                                       int Fp, int64_t rows_per_split, float2* __restrict__ gpart)
+                                                       int64_t rows_per_split, float2* __restrict__ gpart)
 {
+    constexpr int Fp = 16 * NT;
+    constexpr int PER = kGramStage * Fp / kThreads;  // staged elements per thread
+    static_assert(kGramStage * Fp % kThreads == 0, "stage must split evenly over the threads");
+    static_assert(kThreads % kGramStage == 0, "a thread keeps one row of every stage");
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    float* sAr = reinterpret_cast<float*>(smem_raw);
-    float* sAi = sAr + 16 * Fp;
+    float2* slab = reinterpret_cast<float2*>(smem_raw);  // [2][Fp][kGramRowStride]
     const int blk = blockIdx.x;
     const int split = blockIdx.y;
     const int nsplit = gridDim.y;
@@ -67,59 +83,71 @@ __global__ void __launch_bounds__(kThreads) gram_kernel(const float2* __restrict
     const int64_t r_begin = split * rows_per_split;
     const int64_t r_end = min(R, r_begin + rows_per_split);
     const int F = (int)g.F;
-    const int nt = Fp / 16;
-    const int ntile = nt * (nt + 1) / 2;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int ntile = NT * (NT + 1) / 2;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int m = lane & 15, kg = lane >> 4;
 
-    for (int i = threadIdx.x; i < 16 * Fp; i += kThreads) {
-        sAr[i] = 0.f;
-        sAi[i] = 0.f;
-    }
-
-    floatx4 accR[kMaxTilesPerWave], accI[kMaxTilesPerWave];
+    floatx4 p1[kMaxTilesPerWave], p2[kMaxTilesPerWave], p3[kMaxTilesPerWave];
     int tI[kMaxTilesPerWave], tJ[kMaxTilesPerWave];
 #pragma unroll
     for (int k = 0; k < kMaxTilesPerWave; ++k) {
-        accR[k] = floatx4{0.f, 0.f, 0.f, 0.f};
-        accI[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+        p1[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+        p2[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+        p3[k] = floatx4{0.f, 0.f, 0.f, 0.f};
         const int t = wave + 4 * k;
         if (t < ntile)
-            tile_ij(t, nt, tI[k], tJ[k]);
+            tile_ij(t, NT, tI[k], tJ[k]);
         else
             tI[k] = tJ[k] = -1;
     }
-    __syncthreads();
 
-    for (int64_t r0 = r_begin; r0 < r_end; r0 += 16) {
-        // stage 16 rows x F columns; 16 consecutive threads read 16 consecutive rows
-        for (int e = threadIdx.x; e < 16 * F; e += kThreads) {
-            const int kr = e & 15;
-            const int f = e >> 4;
-            const int64_t r = r0 + kr;
-            float2 v = float2{0.f, 0.f};
-            if (r < r_end) v = data[row_addr(g, c, r) + (int64_t)f * g.fstride];
-            sAr[kr * Fp + f] = v.x;
-            sAi[kr * Fp + f] = v.y;
+    // staging: this thread's row of every stage and its frames f0 + (kThreads / kGramStage) i
+    const int srow = tid % kGramStage;
+    const int f0 = tid / kGramStage;
+    constexpr int fstep = kThreads / kGramStage;
+    float2 st[PER];
+    auto load_stage = [&](int64_t r0) {
+        const int64_t r = r0 + srow;
+        const bool ok = r < r_end;
+        const int64_t base = row_addr(g, c, ok ? r : r_begin);
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int f = f0 + fstep * i;
+            st[i] = (ok && f < F) ? data[base + (int64_t)f * g.fstride] : float2{0.f, 0.f};
         }
-        __syncthreads();
+    };
+    auto store_stage = [&](int buf) {
+        float2* s = slab + buf * Fp * kGramRowStride;
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-            const int krow = ks * 4 + (lane >> 4);
-            const int col = lane & 15;
+        for (int i = 0; i < PER; ++i) s[(f0 + fstep * i) * kGramRowStride + srow] = st[i];
+    };
+
+    const int64_t nst = r_end > r_begin ? (r_end - r_begin + kGramStage - 1) / kGramStage : 0;
+    if (nst > 0) {
+        load_stage(r_begin);
+        store_stage(0);
+    }
+    __syncthreads();
+    for (int64_t s = 0; s < nst; ++s) {
+        const int buf = (int)(s & 1);
+        if (s + 1 < nst) load_stage(r_begin + (s + 1) * kGramStage);
+        const float2* sb = slab + buf * Fp * kGramRowStride + m * kGramRowStride + kg;
 #pragma unroll
-            for (int k = 0; k < kMaxTilesPerWave; ++k) {
-                if (tI[k] < 0) continue;
-                const float ar = sAr[krow * Fp + tI[k] * 16 + col];
-                const float ai = sAi[krow * Fp + tI[k] * 16 + col];
-                const float br = sAr[krow * Fp + tJ[k] * 16 + col];
-                const float bi = sAi[krow * Fp + tJ[k] * 16 + col];
-                accR[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, accR[k], 0, 0, 0);
-                accR[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, bi, accR[k], 0, 0, 0);
-                accI[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, accI[k], 0, 0, 0);
-                accI[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, br, accI[k], 0, 0, 0);
+        for (int k = 0; k < kMaxTilesPerWave; ++k) {
+            if (tI[k] < 0) continue;
+            const float2* pa = sb + 16 * tI[k] * kGramRowStride;
+            const float2* pb = sb + 16 * tJ[k] * kGramRowStride;
+#pragma unroll
+            for (int ks = 0; ks < kGramStage / 4; ++ks) {
+                const float2 a = pa[4 * ks], b = pb[4 * ks];
+                p1[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, p1[k], 0, 0, 0);
+                p2[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, p2[k], 0, 0, 0);
+                p3[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x - a.y, b.x + b.y, p3[k], 0, 0, 0);
             }
         }
+        if (s + 1 < nst) store_stage(buf ^ 1);
         __syncthreads();
     }
 
@@ -129,12 +157,27 @@ __global__ void __launch_bounds__(kThreads) gram_kernel(const float2* __restrict
         if (tI[k] < 0) continue;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int i = tI[k] * 16 + 4 * (lane >> 4) + v;
-            const int j = tJ[k] * 16 + (lane & 15);
-            G[(int64_t)i * Fp + j] = float2{accR[k][v], accI[k][v]};
-            if (tI[k] != tJ[k]) G[(int64_t)j * Fp + i] = float2{accR[k][v], -accI[k][v]};
+            const int i = tI[k] * 16 + 4 * kg + v;
+            const int j = tJ[k] * 16 + m;
+            const float re = p1[k][v] + p2[k][v];
+            const float im = p3[k][v] - p1[k][v] + p2[k][v];
+            G[(int64_t)i * Fp + j] = float2{re, im};
+            if (tI[k] != tJ[k]) G[(int64_t)j * Fp + i] = float2{re, -im};
         }
     }
+}
+
+size_t gram_smem(int64_t F) { return sizeof(float2) * 2 * (size_t)llr_padded(F) * kGramRowStride; }
+
+template <int NT>
+void launch_gram(const float2* data, const LlrGeom& g, const int3* corners, int64_t nblk, int nsplit, int64_t rps,
+                 float2* gpart, hipStream_t s)
+{
+    const size_t smem = gram_smem(g.F);
+    allow_lds(gram_kernel<NT>, smem);
+    hipLaunchKernelGGL(gram_kernel<NT>, dim3((unsigned)nblk, (unsigned)nsplit), dim3(kThreads), smem, s, data, g,
+                       corners, rps, gpart);
+    check_launch("gram_kernel");
 }
 
 // Element (x, y), x <= y, of the row-major upper triangle of an n x n matrix.
@@ -639,11 +682,18 @@ void llr_blocks(float2* data, const LlrGeom& g, const int3* corners_dev, int64_t
     float4* rot = reinterpret_cast<float4*>(W + (int64_t)nblk * Fp * Fp);
     int64_t rps = ceil_div(ceil_div(R, nsplit), 16) * 16;
     {
-        const size_t smem = sizeof(float) * 2 * 16 * Fp;
         KernelTimer t("llr_gram", s);
-        hipLaunchKernelGGL(gram_kernel, dim3((unsigned)nblk, (unsigned)nsplit), dim3(kThreads), smem, s,
-                           (const float2*)data, g, corners_dev, Fp, rps, gpart);
-        check_launch("gram_kernel");
+        const float2* d = data;
+        switch (Fp / 16) {
+        case 1: launch_gram<1>(d, g, corners_dev, nblk, nsplit, rps, gpart, s); break;
+        case 2: launch_gram<2>(d, g, corners_dev, nblk, nsplit, rps, gpart, s); break;
+        case 3: launch_gram<3>(d, g, corners_dev, nblk, nsplit, rps, gpart, s); break;
+        case 4: launch_gram<4>(d, g, corners_dev, nblk, nsplit, rps, gpart, s); break;
+        case 5: launch_gram<5>(d, g, corners_dev, nblk, nsplit, rps, gpart, s); break;
+        case 6: launch_gram<6>(d, g, corners_dev, nblk, nsplit, rps, gpart, s); break;
+        case 7: launch_gram<7>(d, g, corners_dev, nblk, nsplit, rps, gpart, s); break;
+        default: launch_gram<8>(d, g, corners_dev, nblk, nsplit, rps, gpart, s); break;
+        }
     }
     {
         const size_t smem = llr_eig_smem(g.F);
