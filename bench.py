@@ -65,6 +65,7 @@ def parse():
                     help="normal: the batched normal operator (default); llr: the LLR prox on 128^3 x 100 frames; "
                          "serve: the Poisson request stream through the continuous-batching server")
     ap.add_argument("--llr-frames", type=int, default=100)
+    ap.add_argument("--llr-max-sweeps", type=int, default=None, help="cap on the LLR Jacobi sweeps (1..15)")
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--rate", type=float, default=2.0)
     ap.add_argument("--seed", type=int, default=2024)
@@ -84,6 +85,9 @@ def llr_main(args, torch, native):
 
     dev = torch.device("cuda:0")
     F, n, b = args.llr_frames, args.n, 16
+    if args.llr_max_sweeps is not None:
+        from hastycompute_amd import _capi
+        _capi.call("hasty_set_option", b"llr_max_sweeps", args.llr_max_sweeps)
     g = torch.Generator(device=dev)
     g.manual_seed(args.seed)
     U = torch.randn((5, n ** 3), dtype=torch.complex64, device=dev, generator=g)

@@ -27,6 +27,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr int kThreads = 256;
 constexpr int kMaxTilesPerWave = 9;  // F <= 128 -> 8x8 tiles -> 36 upper tiles / 4 waves
 constexpr int kMaxSweeps = 15;       // Jacobi sweeps per block (4-6 reach FP32 rounding)
+int g_max_sweeps = kMaxSweeps;       // cap in force (option "llr_max_sweeps")
 
 __device__ __forceinline__ int64_t row_addr(const LlrGeom& g, int3 c, int64_t r)
 {
@@ -180,21 +181,8 @@ void launch_gram(const float2* data, const LlrGeom& g, const int3* corners, int6
     check_launch("gram_kernel");
 }
 
-// Element (x, y), x <= y, of the row-major upper triangle of an n x n matrix.
-__device__ __forceinline__ int tri_idx(int x, int y, int n) { return ((x * (2 * n - x + 1)) >> 1) + (y - x); }
-
-// G(x, y) of a Hermitian matrix kept as its upper triangle U.
-__device__ __forceinline__ float2 herm_get(const float2* U, int x, int y, int n)
-{
-    const bool lower = x > y;
-    const float2 v = U[lower ? tri_idx(y, x, n) : tri_idx(x, y, n)];
-    return float2{v.x, lower ? -v.y : v.y};
-}
-__device__ __forceinline__ void herm_put(float2* U, int x, int y, int n, float2 v)
-{
-    const bool lower = x > y;
-    U[lower ? tri_idx(y, x, n) : tri_idx(x, y, n)] = float2{v.x, lower ? -v.y : v.y};
-}
+// Upper-triangle block (k, l), k <= l, of an m x m block grid, row-major.
+__device__ __forceinline__ int tri_idx(int k, int l, int m) { return (__mul24(k, 2 * m - k + 1) >> 1) + (l - k); }
 
 // Orders the LDS accesses of one wave: what a lane wrote is visible to every lane after it.
 __device__ __forceinline__ void wave_lds_sync()
@@ -219,37 +207,75 @@ __device__ double block_sum(double v, double* red)
     return r;
 }
 
-// Byte offsets of the eigensolver's LDS arrays for an n x n problem (n even):
-//   U   [n (n + 1) / 2] complex  upper triangle of G (later: eigenvectors)
-//   rc  [n / 2] float4           rotation of each pair: c, s, cos(phi), sin(phi)
-//   pq  [n / 2] int              pair positions p | q << 8
-//   gk  [n] float                g of each eigenvalue
-//   kl  [n] int                  kept columns
-//   tbl [(n/2) (n/2 + 1) / 2]    (k, l) of each upper 2x2 block as k | l << 8
+// Parallel (round-robin) Jacobi ordering on n = 2 h indices. In round rr pair x joins
+//   a_x = x == 0 ? 0 : 1 + (x - 1 + rr) mod (n - 1),   b_x = 1 + (n - 2 - x + rr) mod (n - 1).
+// The eigensolver keeps G in "slot" order: pair x always sits in slots (2x, 2x + 1) =
+// (a_x, b_x), and between rounds the data moves by the fixed slot permutation below
+// (slot 0 stays, the a side shifts down, the b side shifts up, the ends wrap).
+__host__ __device__ inline int slot_next(int s, int h)
+{
+    if (h == 1 || s == 0) return s;
+    const int x = s >> 1;
+    if ((s & 1) == 0) return x == 1 ? 1 : 2 * (x - 1);
+    return x == h - 1 ? 2 * (h - 1) : 2 * (x + 1) + 1;
+}
+// Index held by slot s in round 0, and its inverse.
+__host__ __device__ inline int slot0_index(int s, int n) { return s == 0 ? 0 : ((s & 1) == 0 ? s >> 1 : n - 1 - (s >> 1)); }
+__host__ __device__ inline int index_slot0(int i, int n) { return i < n / 2 ? 2 * i : 2 * (n - 1 - i) + 1; }
+
+// Byte offsets of the eigensolver's LDS arrays for an n x n problem (n = 2 h):
+//   Gb  [h (h + 1) / 2][4] complex  G in slot order as upper 2x2 blocks (s, t) =
+//                                   (2k + i, 2l + j), element 2 i + j; in a diagonal
+//                                   block element 2 (the mirror of 1) is not kept.
+//                                   Later: up to h eigenvectors [h][n].
+//   rc  [h] float4                  rotation of each pair: c, s, cos(phi), sin(phi)
+//   gk  [n] float                   g of each eigenvalue
+//   kl  [n] int                     kept columns
 struct EigLayout {
-    int rc, pq, gk, kl, tbl, bytes;
+    int rc, gk, kl, bytes;
     __host__ __device__ explicit EigLayout(int n)
     {
-        const int half = n / 2;
-        const int nb = half * (half + 1) / 2;
-        rc = ((8 * (n * (n + 1) / 2) + 15) / 16) * 16;
-        pq = rc + 16 * half;
-        gk = pq + 4 * half;
+        const int h = n / 2;
+        rc = 8 * 4 * (h * (h + 1) / 2);
+        gk = rc + 16 * h;
         kl = gk + 4 * n;
-        tbl = kl + 4 * n;
-        bytes = tbl + ((2 * nb + 15) / 16) * 16;
+        bytes = kl + 4 * n;
     }
 };
 
+// Upper 2x2 blocks (pairs k <= l) one thread updates per round, at most
+// ceil(64 * 65 / 2 / kThreads) = 9 (n = 128); the kernel is instantiated per count so the
+// registers it holds fit two workgroups per CU.
+constexpr int kMaxBlkPerThread = 9;
+__host__ __device__ inline int eig_blk_per_thread(int n)
+{
+    const int half = n / 2;
+    return (half * (half + 1) / 2 + kThreads - 1) / kThreads;
+}
+
+// Where element (s, t) of the slot-ordered matrix lands after the round's permutation:
+// (its index in Gb) << 1, with bit 0 set when it lands below the diagonal and is stored
+// conjugated.
+__device__ __forceinline__ int moved_loc(int s, int t, int h)
+{
+    const int s2 = slot_next(s, h), t2 = slot_next(t, h);
+    const int lo = min(s2, t2), hi = max(s2, t2);
+    const int idx = 4 * tri_idx(lo >> 1, hi >> 1, h) + ((lo & 1) << 1) + (hi & 1);
+    return (idx << 1) | (s2 > t2 ? 1 : 0);
+}
+
 // Jacobi eigensolver + W = V diag(g) V^H, one workgroup per block.
 //
-// G (n = F rounded up to even) lives in LDS as its upper triangle. Each round of the
-// round-robin ordering rotates n/2 disjoint pairs: the rotations are written to `rot`
-// and G <- J^H G J is applied to the upper 2x2 blocks only. V is never formed: after
-// convergence, the eigenvectors with g != 0 are rebuilt as v_k = J_1 J_2 ... J_R e_k by
-// replaying the recorded rotations backwards, one wave per vector (its lanes are the
-// pairs, so a round needs no workgroup barrier). The work of that pass scales with the
-// number of singular values that survive the threshold.
+// G (n = F rounded up to even) lives in LDS in slot order as upper 2x2 blocks; thread
+// blocks are fixed for the whole solve. A round: every thread loads its blocks and wave 0
+// forms the pair rotations from the diagonal blocks (written to `rot` as well); barrier;
+// each block becomes J_k^H G_kl J_l and its elements are stored where the next round's
+// slot order puts them (the destinations are the same every round); barrier. V is never
+// formed: after convergence, the eigenvectors with g != 0 are rebuilt as
+// v_k = J_1 J_2 ... J_R e_k by replaying the recorded rotations backwards, one wave per
+// vector (its lanes are the pairs, so a round needs no workgroup barrier); that pass
+// scales with the number of singular values that survive the threshold.
+template <int kBlkPerThread>
 __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restrict__ gpart, int nsplit, int F, int Fp,
                                                         float thresh, int soft, int rank, int max_sweeps,
                                                         float4* rot_all, float2* __restrict__ Wout,
@@ -258,71 +284,90 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = F + (F & 1);
     const int half = n / 2;
-    const int ntri = n * (n + 1) / 2;
-    const int nb = half * (half + 1) / 2;
     const EigLayout L(n);
-    float2* U = reinterpret_cast<float2*>(smem_raw);
+    float2* Gs = reinterpret_cast<float2*>(smem_raw);
     float4* rcs = reinterpret_cast<float4*>(smem_raw + L.rc);
-    int* pq = reinterpret_cast<int*>(smem_raw + L.pq);
     float* gk = reinterpret_cast<float*>(smem_raw + L.gk);
     int* kl = reinterpret_cast<int*>(smem_raw + L.kl);
-    unsigned short* tbl = reinterpret_cast<unsigned short*>(smem_raw + L.tbl);
     __shared__ double red[kThreads];
     __shared__ int nk_s;
     const int blk = blockIdx.x;
     const int tid = threadIdx.x;
     float4* rot = rot_all + (int64_t)blk * max_sweeps * (n - 1) * half;
 
-    // Upper triangle of G (sum of the partial Grams in split order); block table.
+    // G in slot order (sum of the partial Grams in split order; the partials are full
+    // Hermitian matrices).
     const float2* Gb = gpart + (int64_t)blk * nsplit * Fp * Fp;
     for (int e = tid; e < n * n; e += kThreads) {
-        const int i = e / n, j = e - (e / n) * n;
-        if (j < i) continue;
+        const int s = e / n, t = e - (e / n) * n;
+        if (t < s) continue;
+        const int i = slot0_index(s, n), j = slot0_index(t, n);
         float2 v = float2{0.f, 0.f};
-        if (j < F) {
-            for (int s = 0; s < nsplit; ++s) {
-                const float2 t = Gb[(int64_t)s * Fp * Fp + (int64_t)i * Fp + j];
-                v.x += t.x;
-                v.y += t.y;
+        if (i < F && j < F) {
+            for (int sp = 0; sp < nsplit; ++sp) {
+                const float2 w = Gb[(int64_t)sp * Fp * Fp + (int64_t)i * Fp + j];
+                v.x += w.x;
+                v.y += w.y;
             }
-            if (i == j) v.y = 0.f;
+            if (s == t) v.y = 0.f;
         }
-        U[tri_idx(i, j, n)] = v;
+        Gs[4 * tri_idx(s >> 1, t >> 1, half) + ((s & 1) << 1) + (t & 1)] = v;
     }
-    for (int e = tid; e < half * half; e += kThreads) {
-        const int k = e / half, l = e - (e / half) * half;
-        if (l >= k) tbl[tri_idx(k, l, half)] = (unsigned short)(k | (l << 8));
+
+    // This thread's blocks: e = tid + kThreads i in row-major order over the upper
+    // triangle of pairs (k, l), k <= l; and where their elements go after a round.
+    int bk[kBlkPerThread], bl[kBlkPerThread], src[kBlkPerThread], dst[kBlkPerThread][4];
+    {
+        int k = 0, rem = tid;
+#pragma unroll
+        for (int i = 0; i < kBlkPerThread; ++i) {
+            if (i > 0) rem += kThreads;
+            while (k < half && rem >= half - k) {
+                rem -= half - k;
+                ++k;
+            }
+            bk[i] = k < half ? k : -1;
+            bl[i] = k < half ? k + rem : -1;
+            const int kk = max(bk[i], 0), ll = max(bl[i], 0);
+            src[i] = 4 * tri_idx(kk, ll, half);
+#pragma unroll
+            for (int el = 0; el < 4; ++el) dst[i][el] = moved_loc(2 * kk + (el >> 1), 2 * ll + (el & 1), half);
+        }
     }
     __syncthreads();
 
-    // |G|_F^2 = 2 sum(triangle) - sum(diagonal); the off-diagonal mass is 2 (sum(triangle) - sum(diagonal))
-    auto tri_minus_diag = [&](double wt) {
+    // |G|_F^2 and the off-diagonal mass from the kept elements (off-diagonal ones count twice)
+    auto mass = [&](bool off_only) {
         double a = 0.0;
-        for (int e = tid; e < ntri; e += kThreads) a += wt * ((double)U[e].x * U[e].x + (double)U[e].y * U[e].y);
-        for (int x = tid; x < n; x += kThreads) {
-            const float2 d = U[tri_idx(x, x, n)];
-            a -= (double)d.x * d.x + (double)d.y * d.y;
+        for (int e = tid; e < n * n; e += kThreads) {
+            const int s = e / n, t = e - (e / n) * n;
+            if (t < s || (off_only && s == t)) continue;
+            const float2 v = Gs[4 * tri_idx(s >> 1, t >> 1, half) + ((s & 1) << 1) + (t & 1)];
+            a += (s == t ? 1.0 : 2.0) * ((double)v.x * v.x + (double)v.y * v.y);
         }
-        return a;
+        return block_sum(a, red);
     };
-    const double total = block_sum(tri_minus_diag(2.0), red);
+    const double total = mass(false);
 
     int sweeps = 0;
     for (; sweeps < max_sweeps; ++sweeps) {
-        const double offn = 2.0 * block_sum(tri_minus_diag(1.0), red);
+        const double offn = mass(true);
         // off-diagonal mass below (1e-5 |G|_F)^2: at FP32 rounding (n eps |G| ~ 6e-6 |G|
         // for n = 100) further sweeps stop paying
         if (offn <= 1e-10 * total || total == 0.0) break;
 
         for (int round = 0; round < n - 1; ++round) {
-            // round-robin pairing: position 0 fixed, positions 1..n-1 rotate
+            float4 va[kBlkPerThread], vb[kBlkPerThread];
+#pragma unroll
+            for (int i = 0; i < kBlkPerThread; ++i) {
+                if (bk[i] < 0) continue;
+                va[i] = *reinterpret_cast<const float4*>(Gs + src[i]);      // (2k, 2l), (2k, 2l+1)
+                vb[i] = *reinterpret_cast<const float4*>(Gs + src[i] + 2);  // (2k+1, 2l), (2k+1, 2l+1)
+            }
             if (tid < half) {
-                const int a = tid == 0 ? 0 : 1 + (tid - 1 + round) % (n - 1);
-                const int b = 1 + (n - 2 - tid + round) % (n - 1);
-                const int p = min(a, b), q = max(a, b);
-                pq[tid] = p | (q << 8);
-                const float ap = U[tri_idx(p, p, n)].x, dq = U[tri_idx(q, q, n)].x;
-                const float2 bpq = U[tri_idx(p, q, n)];
+                const float4 d0 = *reinterpret_cast<const float4*>(Gs + 4 * tri_idx(tid, tid, half));
+                const float ap = d0.x, dq = Gs[4 * tri_idx(tid, tid, half) + 3].x;
+                const float2 bpq = float2{d0.z, d0.w};
                 const float ab = hypotf(bpq.x, bpq.y);
                 float c = 1.f, s = 0.f, cf = 1.f, sf = 0.f;
                 if (ab > 1e-30f && ab > 1e-9f * (fabsf(ap) + fabsf(dq))) {
@@ -340,51 +385,55 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
             __syncthreads();
             // G <- J^H G J, block by block: the rows of pair k and the columns of pair l
             // form a 2x2 block that maps to J_k^H G_kl J_l (rows: r_p = c r_p - s e^{i phi} r_q,
-            // r_q = s r_p + c e^{i phi} r_q; columns the same with e^{-i phi}). Only the blocks
-            // with k <= l are formed; each element of the triangle belongs to exactly one of
-            // them, and the diagonal blocks end exactly diagonal and real.
-            for (int e = tid; e < nb; e += kThreads) {
-                const int t = tbl[e];
-                const int k = t & 255, l = t >> 8;
-                const int pqk = pq[k], pql = pq[l];
-                const int pk = pqk & 255, qk = pqk >> 8, pl = pql & 255, ql = pql >> 8;
-                const float4 rk = rcs[k], rl = rcs[l];
+            // r_q = s r_p + c e^{i phi} r_q; columns the same with e^{-i phi}). Each kept
+            // element belongs to exactly one block k <= l, and the diagonal blocks end exactly
+            // diagonal and real. Every block was read before the barrier, so the stores into
+            // the next round's slot order cannot overwrite an unread element.
+#pragma unroll
+            for (int i = 0; i < kBlkPerThread; ++i) {
+                if (bk[i] < 0) continue;
+                const bool diag = bk[i] == bl[i];
+                const float4 rk = rcs[bk[i]], rl = rcs[bl[i]];
                 const float ck = rk.x, sk = rk.y, cl = rl.x, sl = rl.y;
                 const float2 phk = float2{rk.z, rk.w};
                 const float2 phl = float2{rl.z, -rl.w};
-                const float2 a = herm_get(U, pk, pl, n), bb = herm_get(U, pk, ql, n);
-                const float2 cc = herm_get(U, qk, pl, n), d = herm_get(U, qk, ql, n);
+                const float2 a = float2{va[i].x, va[i].y}, bb = float2{va[i].z, va[i].w};
+                const float2 cc = diag ? float2{bb.x, -bb.y} : float2{vb[i].x, vb[i].y};
+                const float2 d = float2{vb[i].z, vb[i].w};
                 const float2 ea = cmul(phk, cc), eb = cmul(phk, d);
                 const float2 a1 = float2{ck * a.x - sk * ea.x, ck * a.y - sk * ea.y};
                 const float2 b1 = float2{ck * bb.x - sk * eb.x, ck * bb.y - sk * eb.y};
                 const float2 c1 = float2{sk * a.x + ck * ea.x, sk * a.y + ck * ea.y};
                 const float2 d1 = float2{sk * bb.x + ck * eb.x, sk * bb.y + ck * eb.y};
                 const float2 fb = cmul(phl, b1), fd = cmul(phl, d1);
-                float2 na = float2{cl * a1.x - sl * fb.x, cl * a1.y - sl * fb.y};
-                float2 nbv = float2{sl * a1.x + cl * fb.x, sl * a1.y + cl * fb.y};
-                const float2 nc = float2{cl * c1.x - sl * fd.x, cl * c1.y - sl * fd.y};
-                float2 nd = float2{sl * c1.x + cl * fd.x, sl * c1.y + cl * fd.y};
-                if (k == l) {
-                    na.y = 0.f;
-                    nd.y = 0.f;
-                    if (sk != 0.f) nbv = float2{0.f, 0.f};
-                    U[tri_idx(pk, pk, n)] = na;
-                    U[tri_idx(pk, qk, n)] = nbv;
-                    U[tri_idx(qk, qk, n)] = nd;
-                } else {
-                    herm_put(U, pk, pl, n, na);
-                    herm_put(U, pk, ql, n, nbv);
-                    herm_put(U, qk, pl, n, nc);
-                    herm_put(U, qk, ql, n, nd);
+                float2 nv[4];
+                nv[0] = float2{cl * a1.x - sl * fb.x, cl * a1.y - sl * fb.y};
+                nv[1] = float2{sl * a1.x + cl * fb.x, sl * a1.y + cl * fb.y};
+                nv[2] = float2{cl * c1.x - sl * fd.x, cl * c1.y - sl * fd.y};
+                nv[3] = float2{sl * c1.x + cl * fd.x, sl * c1.y + cl * fd.y};
+                if (diag) {
+                    nv[0].y = 0.f;
+                    nv[3].y = 0.f;
+                    if (sk != 0.f) nv[1] = float2{0.f, 0.f};
+                }
+#pragma unroll
+                for (int el = 0; el < 4; ++el) {
+                    if (el == 2 && diag) continue;  // the mirror of element 1
+                    const int loc = dst[i][el];
+                    Gs[loc >> 1] = float2{nv[el].x, (loc & 1) ? -nv[el].y : nv[el].y};
                 }
             }
             __syncthreads();
         }
     }
 
-    // g_k from the eigenvalues (the diagonal)
+    // g_k from the eigenvalues: index k sits in slot index_slot0(k) again (whole sweeps)
+    auto eig_of = [&](int k) {
+        const int s = index_slot0(k, n);
+        return Gs[4 * tri_idx(s >> 1, s >> 1, half) + 3 * (s & 1)].x;
+    };
     for (int k = tid; k < n; k += kThreads) {
-        const float lam = U[tri_idx(k, k, n)].x;
+        const float lam = eig_of(k);
         float gv = 0.f;
         if (k < F) {
             if (soft) {
@@ -393,7 +442,7 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
             } else {
                 int cnt = 0;
                 for (int j = 0; j < F; ++j) {
-                    const float lj = U[tri_idx(j, j, n)].x;
+                    const float lj = eig_of(j);
                     if (lj > lam || (lj == lam && j < k)) ++cnt;
                 }
                 gv = cnt < rank ? 1.f : 0.f;
@@ -419,8 +468,8 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
     __syncthreads();
     const int nk = nk_s;
 
-    // The triangle is free now: it holds up to n/2 eigenvectors X[v][n] at a time.
-    float2* X = U;
+    // G is no longer needed: its storage holds up to n/2 eigenvectors X[v][n] at a time.
+    float2* X = Gs;
     float2* W = Wout + (int64_t)blk * Fp * Fp;
     const int R = sweeps * (n - 1);
     const int cap = half;
@@ -432,10 +481,9 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
         }
         __syncthreads();
         if (wave < nkc) {
-            // lane m < n/2 applies the rotation of pair m to the wave's vectors, round R-1
-            // down to 0; rotations are prefetched 8 rounds ahead. With rr = r mod (n-1) the
-            // pair is (1 + (m - 1 + rr) mod (n-1), 1 + (n - 2 - m + rr) mod (n-1)), m = 0
-            // taking position 0 for the first; both residues step down by one per round.
+            // lane m < n/2 applies the rotation of pair m = (a_m, b_m) to the wave's
+            // vectors, round R-1 down to 0; rotations are prefetched 8 rounds ahead. The
+            // residues of a_m - 1 and b_m - 1 step down by one per round.
             const bool act = lane < half;
             int A = act && lane > 0 ? (lane - 1 + n - 2) % (n - 1) : 0;
             int B = act ? (2 * n - 4 - lane) % (n - 1) : 0;
@@ -456,9 +504,8 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
                 for (int j = 0; j < 8; ++j) {
                     if (r0 - j < 0) break;
                     if (act) {
-                        const int a = lane == 0 ? 0 : 1 + A;
-                        const int b = 1 + B;
-                        const int p = min(a, b), q = max(a, b);
+                        const int p = lane == 0 ? 0 : 1 + A;
+                        const int q = 1 + B;
                         const float c = cur[j].x, s = cur[j].y;
                         const float2 eph = float2{cur[j].z, -cur[j].w};
                         for (int v = wave; v < nkc; v += kThreads / 64) {
@@ -638,6 +685,17 @@ void launch_apply(float2* data, const LlrGeom& g, const int3* corners, int64_t n
     check_launch("apply_kernel");
 }
 
+template <int B>
+void launch_eig(const float2* gpart, int nsplit, const LlrGeom& g, int Fp, float thresh, int soft, int rank,
+                float4* rot, float2* W, float* eig_out, int64_t nblk, hipStream_t s)
+{
+    const size_t smem = (size_t)EigLayout((int)(g.F + (g.F & 1))).bytes;
+    allow_lds(eig_w_kernel<B>, smem);
+    hipLaunchKernelGGL(eig_w_kernel<B>, dim3((unsigned)nblk), dim3(kThreads), smem, s, gpart, nsplit, (int)g.F, Fp,
+                       thresh, soft, rank, g_max_sweeps, rot, W, eig_out);
+    check_launch("eig_w_kernel");
+}
+
 }  // namespace
 
 int llr_padded(int64_t F) { return (int)(((F + 15) / 16) * 16); }
@@ -658,6 +716,12 @@ size_t llr_workspace_bytes(int64_t nblk, int64_t F, int nsplit)
     const int64_t Fp = llr_padded(F);
     return sizeof(float2) * (size_t)nblk * (size_t)(nsplit + 1) * Fp * Fp +
            sizeof(float4) * (size_t)nblk * (size_t)rot_per_block(F);
+}
+
+void llr_set_max_sweeps(int sweeps)
+{
+    HASTY_REQUIRE(sweeps >= 1 && sweeps <= kMaxSweeps, "llr_max_sweeps must be in 1..15");
+    g_max_sweeps = sweeps;
 }
 
 void llr_check(int64_t F)
@@ -696,12 +760,20 @@ void llr_blocks(float2* data, const LlrGeom& g, const int3* corners_dev, int64_t
         }
     }
     {
-        const size_t smem = llr_eig_smem(g.F);
-        allow_lds(eig_w_kernel, smem);
         KernelTimer t("llr_eig", s);
-        hipLaunchKernelGGL(eig_w_kernel, dim3((unsigned)nblk), dim3(kThreads), smem, s, (const float2*)gpart, nsplit,
-                           (int)g.F, Fp, thresh, soft, rank, kMaxSweeps, rot, W, eig_out);
-        check_launch("eig_w_kernel");
+        const int n = (int)(g.F + (g.F & 1));
+        const float2* gp = gpart;
+        switch (eig_blk_per_thread(n)) {
+        case 1: launch_eig<1>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
+        case 2: launch_eig<2>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
+        case 3: launch_eig<3>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
+        case 4: launch_eig<4>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
+        case 5: launch_eig<5>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
+        case 6: launch_eig<6>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
+        case 7: launch_eig<7>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
+        case 8: launch_eig<8>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
+        default: launch_eig<kMaxBlkPerThread>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
+        }
     }
     {
         const int64_t rows_per_wg = 16 * (kApplyThreads / 64) * kApplySlabsPerWave;

@@ -115,8 +115,13 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
                                    const int64_t* nmodes3, void* workspace, size_t workspace_bytes, int coil_batch,
                                    void* stream);
 
-/* Library options: "sense_fused" (default 1) lets hasty_sense_normal_prepare choose the
- * fused layout. Returns 0, or 1 for an unknown name. */
+/* Library options (no counterpart in the reference; tuning switches):
+ *   "sense_fused" (default 1): hasty_sense_normal_prepare may choose the fused layout;
+ *   "sense_xy_variant" 0..3 (default 2): plane-kernel family of the fused normal operator
+ *     (0 LDS Stockham, 1 coset lane-DFT, 2 register radix-16 at 128 x 128 planes and
+ *     coset elsewhere, 3 coset with an LDS-DMA inverse), set before preparing kernels;
+ *   "llr_max_sweeps" 1..15 (default 15): cap on the Jacobi sweeps of the LLR eigensolver.
+ * Returns 0, or 1 for an unknown name or a value out of range. */
 int hasty_set_option(const char* name, int64_t value);
 
 /* Kernel timing (no counterpart in the reference; a measurement hook for benchmarks).
