@@ -224,10 +224,12 @@ __host__ __device__ inline int slot0_index(int s, int n) { return s == 0 ? 0 : (
 __host__ __device__ inline int index_slot0(int i, int n) { return i < n / 2 ? 2 * i : 2 * (n - 1 - i) + 1; }
 
 // Byte offsets of the eigensolver's LDS arrays for an n x n problem (n = 2 h):
-//   Gb  [h (h + 1) / 2][4] complex  G in slot order as upper 2x2 blocks (s, t) =
+//   Gb  [h (h + 1) / 2 + 1][4] complex  G in slot order as upper 2x2 blocks (s, t) =
 //                                   (2k + i, 2l + j), element 2 i + j; in a diagonal
-//                                   block element 2 (the mirror of 1) is not kept.
-//                                   Later: up to h eigenvectors [h][n].
+//                                   block element 2 (the mirror of 1) is not kept. The
+//                                   extra block is a scratch target for threads with
+//                                   fewer blocks than the others. Later: up to h
+//                                   eigenvectors [h][n].
 //   rc  [h] float4                  rotation of each pair: c, s, cos(phi), sin(phi)
 //   gk  [n] float                   g of each eigenvalue
 //   kl  [n] int                     kept columns
@@ -236,7 +238,7 @@ struct EigLayout {
     __host__ __device__ explicit EigLayout(int n)
     {
         const int h = n / 2;
-        rc = 8 * 4 * (h * (h + 1) / 2);
+        rc = 8 * 4 * (h * (h + 1) / 2 + 1);
         gk = rc + 16 * h;
         kl = gk + 4 * n;
         bytes = kl + 4 * n;
@@ -326,12 +328,18 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
                 rem -= half - k;
                 ++k;
             }
-            bk[i] = k < half ? k : -1;
-            bl[i] = k < half ? k + rem : -1;
-            const int kk = max(bk[i], 0), ll = max(bl[i], 0);
-            src[i] = 4 * tri_idx(kk, ll, half);
+            if (k < half) {
+                bk[i] = k;
+                bl[i] = k + rem;
+                src[i] = 4 * tri_idx(k, k + rem, half);
 #pragma unroll
-            for (int el = 0; el < 4; ++el) dst[i][el] = moved_loc(2 * kk + (el >> 1), 2 * ll + (el & 1), half);
+                for (int el = 0; el < 4; ++el) dst[i][el] = moved_loc(2 * k + (el >> 1), 2 * (k + rem) + (el & 1), half);
+            } else {  // no block: read and write the scratch block, uniformly with the rest of the wave
+                bk[i] = bl[i] = 0;
+                src[i] = 4 * (half * (half + 1) / 2);
+#pragma unroll
+                for (int el = 0; el < 4; ++el) dst[i][el] = (src[i] + el) << 1;
+            }
         }
     }
     __syncthreads();
@@ -360,7 +368,6 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
             float4 va[kBlkPerThread], vb[kBlkPerThread];
 #pragma unroll
             for (int i = 0; i < kBlkPerThread; ++i) {
-                if (bk[i] < 0) continue;
                 va[i] = *reinterpret_cast<const float4*>(Gs + src[i]);      // (2k, 2l), (2k, 2l+1)
                 vb[i] = *reinterpret_cast<const float4*>(Gs + src[i] + 2);  // (2k+1, 2l), (2k+1, 2l+1)
             }
@@ -391,7 +398,6 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
             // the next round's slot order cannot overwrite an unread element.
 #pragma unroll
             for (int i = 0; i < kBlkPerThread; ++i) {
-                if (bk[i] < 0) continue;
                 const bool diag = bk[i] == bl[i];
                 const float4 rk = rcs[bk[i]], rl = rcs[bl[i]];
                 const float ck = rk.x, sk = rk.y, cl = rl.x, sl = rl.y;
@@ -411,14 +417,16 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
                 nv[1] = float2{sl * a1.x + cl * fb.x, sl * a1.y + cl * fb.y};
                 nv[2] = float2{cl * c1.x - sl * fd.x, cl * c1.y - sl * fd.y};
                 nv[3] = float2{sl * c1.x + cl * fd.x, sl * c1.y + cl * fd.y};
-                if (diag) {
-                    nv[0].y = 0.f;
-                    nv[3].y = 0.f;
-                    if (sk != 0.f) nv[1] = float2{0.f, 0.f};
-                }
+                // a diagonal block is Hermitian: real diagonal, the rotated pair element
+                // exactly zero, and element 2 (the mirror of 1, with the same destination
+                // and the opposite conjugation) written as conj of element 1: the two
+                // stores agree
+                nv[0].y = diag ? 0.f : nv[0].y;
+                nv[3].y = diag ? 0.f : nv[3].y;
+                nv[1] = (diag && sk != 0.f) ? float2{0.f, 0.f} : nv[1];
+                nv[2] = diag ? float2{nv[1].x, -nv[1].y} : nv[2];
 #pragma unroll
                 for (int el = 0; el < 4; ++el) {
-                    if (el == 2 && diag) continue;  // the mirror of element 1
                     const int loc = dst[i][el];
                     Gs[loc >> 1] = float2{nv[el].x, (loc & 1) ? -nv[el].y : nv[el].y};
                 }
