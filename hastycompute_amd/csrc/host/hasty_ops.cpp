@@ -879,6 +879,73 @@ void OldNormal3DBlocksSVT::apply(at::Tensor in, const std::array<int64_t, 3>& bl
 
 namespace op {
 
+SenseNOp::SenseNOp(const at::Tensor& coords, const std::vector<int64_t>& nmodes, const at::Tensor& smaps,
+                   const std::vector<int64_t>& coils, const at::optional<fft::NufftOptions>& forward_opts,
+                   const at::optional<fft::NufftOptions>& backward_opts)
+    : _h(std::make_shared<Holder>())
+{
+    TORCH_CHECK(coords.is_cuda(), "Coordinates did not reside in cuda device");
+    TORCH_CHECK(!nmodes.empty() && nmodes[0] >= 1, "nmodes must start with the transform count");
+    // tolerances from the options; types and signs fixed so that apply_backward is the
+    // adjoint of apply_forward
+    const fft::NufftOptions fwd = fft::NufftOptions::type2(forward_opts.has_value() ? forward_opts->tol : 1e-5);
+    const fft::NufftOptions bwd = fft::NufftOptions::type1(backward_opts.has_value() ? backward_opts->tol : 1e-5);
+    _h->coords = coords;
+    _h->nmodes = nmodes;
+    _h->smaps = smaps;
+    _h->coils = coils;
+    _h->sense = std::make_unique<mri::CUDASenseNormal>(coords, nmodes, fwd, bwd);
+}
+
+at::Tensor SenseNOp::apply(const at::Tensor& in) const
+{
+    at::Tensor out = at::empty_like(in);
+    _h->sense->apply(in, out, _h->smaps, _h->coils);
+    return out;
+}
+
+at::Tensor SenseNOp::apply_forward(const at::Tensor& in) const
+{
+    at::Tensor out = fft::allocate_out(_h->coords, (int)_h->coils.size());
+    _h->sense->apply_forward(in, out, _h->smaps, _h->coils);
+    return out;
+}
+
+at::Tensor SenseNOp::apply_backward(const at::Tensor& in) const
+{
+    std::vector<int64_t> img = _h->nmodes;
+    img[0] = 1;
+    at::Tensor out = fft::allocate_adjoint_out(_h->coords, img);
+    _h->sense->apply_backward(in, out, _h->smaps, _h->coils);
+    return out;
+}
+
+std::vector<at::Tensor> SenseNOp::apply(const std::vector<at::Tensor>& in) const
+{
+    std::vector<at::Tensor> out;
+    out.reserve(in.size());
+    for (const auto& t : in) out.push_back(apply(t));
+    return out;
+}
+
+std::vector<at::Tensor> SenseNOp::apply_forward(const std::vector<at::Tensor>& in) const
+{
+    std::vector<at::Tensor> out;
+    out.reserve(in.size());
+    for (const auto& t : in) out.push_back(apply_forward(t));
+    return out;
+}
+
+std::vector<at::Tensor> SenseNOp::apply_backward(const std::vector<at::Tensor>& in) const
+{
+    std::vector<at::Tensor> out;
+    out.reserve(in.size());
+    for (const auto& t : in) out.push_back(apply_backward(t));
+    return out;
+}
+
+std::shared_ptr<SenseNOp> SenseNOp::adjoint() const { return std::shared_ptr<SenseNOp>(new SenseNOp(_h)); }
+
 ConjugateGradient::ConjugateGradient(Op A, at::Tensor b, at::optional<Op> P)
     : _A(std::move(A)), _b(std::move(b)), _P(std::move(P))
 {

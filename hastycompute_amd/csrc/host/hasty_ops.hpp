@@ -367,6 +367,43 @@ using Normal3DBlocksSVT = OldNormal3DBlocksSVT;
 
 namespace op {
 
+// lib/op/mriop.cpp:156-266: the SENSE normal operator as an operator object that holds
+// its trajectory, coil maps and coil list. apply = A^H A x (same shape as x),
+// apply_forward = A x (k-space [coils.size(), M]), apply_backward = A^H y (image
+// [1, N...], summed over coils); the list forms map over the children, as the reference's
+// nested Vector does. The reference's holder (mriop.cpp:165-188) never stores the
+// coordinates or mode counts and forces both NUFFTs to type 2; here they are kept and the
+// backward NUFFT is type 1 (the options give the tolerances; types and signs are fixed,
+// forward type 2 with e^{-i}, backward type 1 with e^{+i}). The reference sizes the forward output by nmodes[0] and the backward output
+// by nmodes, which is the same when one transform slot is used; here the forward output
+// has one row per listed coil and the backward output is the coil-combined image.
+class SenseNOp {
+public:
+    SenseNOp(const at::Tensor& coords, const std::vector<int64_t>& nmodes, const at::Tensor& smaps,
+             const std::vector<int64_t>& coils, const at::optional<fft::NufftOptions>& forward_opts = at::nullopt,
+             const at::optional<fft::NufftOptions>& backward_opts = at::nullopt);
+    at::Tensor apply(const at::Tensor& in) const;
+    at::Tensor apply_forward(const at::Tensor& in) const;
+    at::Tensor apply_backward(const at::Tensor& in) const;
+    std::vector<at::Tensor> apply(const std::vector<at::Tensor>& in) const;
+    std::vector<at::Tensor> apply_forward(const std::vector<at::Tensor>& in) const;
+    std::vector<at::Tensor> apply_backward(const std::vector<at::Tensor>& in) const;
+    // the normal operator is self-adjoint: the adjoint shares this operator's state
+    std::shared_ptr<SenseNOp> adjoint() const;
+    const std::vector<int64_t>& nmodes() const { return _h->nmodes; }
+
+private:
+    struct Holder {
+        at::Tensor coords;
+        std::vector<int64_t> nmodes;
+        at::Tensor smaps;
+        std::vector<int64_t> coils;
+        std::unique_ptr<mri::CUDASenseNormal> sense;
+    };
+    explicit SenseNOp(std::shared_ptr<Holder> h) : _h(std::move(h)) {}
+    std::shared_ptr<Holder> _h;
+};
+
 // lib/op/opalgs.cpp:50-96 on tensors: A is a callable, P an optional preconditioner.
 class ConjugateGradient {
 public:

@@ -5,6 +5,7 @@
 //                                                                      python/cpp/py_batched_sense.cpp:171-190
 //   torch.classes.HastySVT.{Random3DBlocksSVT, Normal3DBlocksSVT}      python/cpp/py_svt.cpp:64-73
 //   torch.classes.HastyInterface.FunctionLambda                        python/cpp/py_interface.cpp:37-42
+//   torch.classes.HastyOp.SenseNOp                                     lib/op/mriop.cpp:156-266 (not bound there)
 // Errors surface as RuntimeError carrying the operator's message (torch_util.cpp:181-200).
 #include <torch/custom_class.h>
 #include <torch/jit.h>
@@ -264,6 +265,50 @@ private:
     std::unique_ptr<svt::OldNormal3DBlocksSVT> _op;
 };
 
+// op::SenseNOp (lib/op/mriop.cpp:156-266); the reference has no Python binding for it.
+class SenseNOp : public torch::CustomClassHolder {
+public:
+    SenseNOp(const at::Tensor& coords, const std::vector<int64_t>& nmodes, const at::Tensor& smaps,
+             const std::vector<int64_t>& coils, const c10::optional<c10::intrusive_ptr<NufftOptions>>& forward_opts,
+             const c10::optional<c10::intrusive_ptr<NufftOptions>>& backward_opts)
+    {
+        require_cuda_coords(coords);
+        at::optional<fft::NufftOptions> fwd, bwd;
+        if (forward_opts.has_value()) fwd = (*forward_opts)->get();
+        if (backward_opts.has_value()) bwd = (*backward_opts)->get();
+        catcher([&] { _op = std::make_shared<op::SenseNOp>(coords, nmodes, smaps, coils, fwd, bwd); });
+    }
+    explicit SenseNOp(std::shared_ptr<op::SenseNOp> o) : _op(std::move(o)) {}
+    at::Tensor apply(const at::Tensor& in) const
+    {
+        at::Tensor r;
+        catcher([&] { r = _op->apply(in); });
+        return r;
+    }
+    at::Tensor apply_forward(const at::Tensor& in) const
+    {
+        at::Tensor r;
+        catcher([&] { r = _op->apply_forward(in); });
+        return r;
+    }
+    at::Tensor apply_backward(const at::Tensor& in) const
+    {
+        at::Tensor r;
+        catcher([&] { r = _op->apply_backward(in); });
+        return r;
+    }
+    std::vector<at::Tensor> apply_list(const std::vector<at::Tensor>& in) const
+    {
+        std::vector<at::Tensor> r;
+        catcher([&] { r = _op->apply(in); });
+        return r;
+    }
+    c10::intrusive_ptr<SenseNOp> adjoint() const { return c10::make_intrusive<SenseNOp>(_op->adjoint()); }
+
+private:
+    std::shared_ptr<op::SenseNOp> _op;
+};
+
 }  // namespace ffi
 }  // namespace hasty
 
@@ -348,5 +393,23 @@ TORCH_LIBRARY(HastySVT, m)
     m.def("doc", []() -> std::string {
         return "Random3DBlocksSVT(streams?).apply(in [F,E,X,Y,Z], nblocks, block_shape, thresh, soft); "
                "Normal3DBlocksSVT(streams?).apply(in, block_strides, block_shape, block_iter, thresh, soft).";
+    });
+}
+
+TORCH_LIBRARY(HastyOp, m)
+{
+    m.class_<SenseNOp>("SenseNOp")
+        .def(torch::init<const at::Tensor&, const std::vector<int64_t>&, const at::Tensor&, const std::vector<int64_t>&,
+                         const c10::optional<c10::intrusive_ptr<NufftOptions>>&,
+                         const c10::optional<c10::intrusive_ptr<NufftOptions>>&>())
+        .def("apply", &SenseNOp::apply)
+        .def("apply_forward", &SenseNOp::apply_forward)
+        .def("apply_backward", &SenseNOp::apply_backward)
+        .def("apply_list", &SenseNOp::apply_list)
+        .def("adjoint", &SenseNOp::adjoint);
+    m.def("doc", []() -> std::string {
+        return "SenseNOp(coords, nmodes, smaps, coils, forward_opts?, backward_opts?): apply(x) = A^H A x, "
+               "apply_forward(x) = A x [coils, M], apply_backward(y) = A^H y [1, N...], apply_list maps apply over a "
+               "list, adjoint() is the operator itself.";
     });
 }

@@ -54,7 +54,7 @@ def test_invalid_arguments_rejected_on_host():
 
 @pytest.fixture(scope="module")
 def ffi():
-    from hastycompute_amd.ffi import hasty_interface, hasty_nufft, hasty_sense, hasty_svt
+    from hastycompute_amd.ffi import hasty_interface, hasty_nufft, hasty_op, hasty_sense, hasty_svt  # noqa: F401
 
     return hasty_nufft, hasty_sense, hasty_svt, hasty_interface
 
@@ -64,7 +64,8 @@ def test_torchscript_modules_registered(ffi):
                          "HastySense": ["Sense", "SenseAdjoint", "SenseNormal"],
                          "HastyBatchedSense": ["BatchedSense", "BatchedSenseAdjoint", "BatchedSenseNormal"],
                          "HastySVT": ["Random3DBlocksSVT", "Normal3DBlocksSVT"],
-                         "HastyInterface": ["FunctionLambda"]}.items():
+                         "HastyInterface": ["FunctionLambda"],
+                         "HastyOp": ["SenseNOp"]}.items():
         ns = getattr(torch.classes, mod)
         for c in classes:
             assert getattr(ns, c) is not None

@@ -140,6 +140,41 @@ def test_ffi_sense_family(F, coils):
     assert e_f < 1e-4 and e_a < 1e-4 and e_n < 1e-4
 
 
+@pytest.mark.parametrize("coils", [[0, 1, 2, 3], [2, 0, 3]])
+def test_ffi_sense_nop_vs_oracle(F, coils):
+    """op::SenseNOp (mriop.cpp:156-266): normal, forward and backward maps, the list form
+    and the adjoint, against the oracle's SENSE operators."""
+    from hastycompute_amd.ffi import hasty_nufft, hasty_op
+
+    rng = np.random.default_rng(7 + len(coils))
+    img = [16, 14, 12]
+    M = 1400
+    co, co_used = coords32(rng, 3, M)
+    S = rnd(rng, [4] + img) * 0.5
+    x = rnd(rng, [1] + img)
+    y = rnd(rng, (len(coils), M))
+    op = hasty_op.SenseNOp(g(co, torch.float32), [1] + img, g(S), coils,
+                           hasty_nufft.NufftOptions.type2(), hasty_nufft.NufftOptions.type2())
+    e_n = O.rel_l2(h(op.apply(g(x))), O.sense_normal_apply(co_used, x, S, coils, exact=True))
+    fwd = op.apply_forward(g(x))
+    assert tuple(fwd.shape) == (len(coils), M)
+    # row i is coil coils[i] (sense.ixx:581-620 multiplies smaps[coils[idx]]); the oracle's
+    # sense_apply follows CUDASense's list-position maps, so it gets the maps in list order
+    e_f = O.rel_l2(h(fwd), O.sense_apply(co_used, x, S[coils], list(range(len(coils))), exact=True))
+    bwd = op.apply_backward(g(y))
+    assert tuple(bwd.shape) == tuple([1] + img)
+    # the backward NUFFT is type 1 with the + sign even when type-2 options are passed
+    e_b = O.rel_l2(h(bwd), O.sense_adjoint_apply(co_used, y, S, coils, exact=True))
+    x2 = rnd(rng, [1] + img)
+    lst = op.apply([g(x), g(x2)])
+    e_l = max(O.rel_l2(h(lst[0]), O.sense_normal_apply(co_used, x, S, coils, exact=True)),
+              O.rel_l2(h(lst[1]), O.sense_normal_apply(co_used, x2, S, coils, exact=True)))
+    e_adj = O.rel_l2(h(op.adjoint().apply(g(x2))), h(lst[1]))
+    print(f"ffi SenseNOp coils={coils}: normal {e_n:.2e} fwd {e_f:.2e} bwd {e_b:.2e} list {e_l:.2e} "
+          f"adjoint {e_adj:.2e}", flush=True)
+    assert e_n < 1e-4 and e_f < 1e-4 and e_b < 1e-4 and e_l < 1e-4 and e_adj < 1e-6
+
+
 def _frames(rng, nframes, E, img, C, M):
     coords, kd, w = [], [], []
     for f in range(nframes):
