@@ -57,14 +57,15 @@ __device__ __forceinline__ void tile_ij(int t, int nt, int& ti, int& tj)
 // Partial Gram G = A^H A over rows [split * rows_per_split, ...) of each block.
 //
 // Rows are staged through LDS in stages of kGramStage rows, frame-major ([f][row], row
-// stride kGramStage + 2 complex, so both the staging stores and the MFMA operand reads
-// are free of bank conflicts), double-buffered: the next stage's loads (32 threads read
+// stride kGramStage + 1 complex: the LDS serves a 64-bit access 16 lanes at a time over
+// 32 banks, and the 16 frames one operand read touches then start 66 words apart, on
+// distinct bank pairs; the staging stores are contiguous), double-buffered: the next stage's loads (32 threads read
 // 32 rows of one frame: whole z-lines) are in flight while the current stage's MFMAs run,
 // one barrier per stage. Each wave owns up to kMaxTilesPerWave upper 16x16 tiles of G;
 // conj(a) b takes three real MFMAs:
 //   P1 = ar br, P2 = ai bi, P3 = (ar - ai)(br + bi);  Re = P1 + P2, Im = P3 - P1 + P2.
 constexpr int kGramStage = 32;
-constexpr int kGramRowStride = kGramStage + 2;
+constexpr int kGramRowStride = kGramStage + 1;
 
 template <int NT>
 __global__ void __launch_bounds__(kThreads) gram_kernel(const float2* __restrict__ data, LlrGeom g, const int3* corners,
@@ -224,8 +225,9 @@ __host__ __device__ inline int slot0_index(int s, int n) { return s == 0 ? 0 : (
 __host__ __device__ inline int index_slot0(int i, int n) { return i < n / 2 ? 2 * i : 2 * (n - 1 - i) + 1; }
 
 // Byte offsets of the eigensolver's LDS arrays for an n x n problem (n = 2 h):
-//   Gb  [h (h + 1) / 2 + 1][4] complex  G in slot order as upper 2x2 blocks (s, t) =
-//                                   (2k + i, 2l + j), element 2 i + j; in a diagonal
+//   Gb  2 x [h (h + 1) / 2 + 1][2] complex  G in slot order as upper 2x2 blocks (s, t) =
+//                                   (2k + i, 2l + j), element 2 i + j (rows i = 0 and
+//                                   i = 1 of all blocks in two arrays); in a diagonal
 //                                   block element 2 (the mirror of 1) is not kept. The
 //                                   extra block is a scratch target for threads with
 //                                   fewer blocks than the others. Later: up to h
@@ -255,14 +257,19 @@ __host__ __device__ inline int eig_blk_per_thread(int n)
     return (half * (half + 1) / 2 + kThreads - 1) / kThreads;
 }
 
+// Complex index of element el = 2 i + j of block b in Gb: the blocks' top rows (elements
+// 0, 1) form one array of nb1 float4 and their bottom rows (2, 3) a second one, so the
+// lanes of a wave that read consecutive blocks read consecutive 16-byte words.
+__device__ __forceinline__ int gel(int b, int el, int nb1) { return (el >> 1) * 2 * nb1 + 2 * b + (el & 1); }
+
 // Where element (s, t) of the slot-ordered matrix lands after the round's permutation:
 // (its index in Gb) << 1, with bit 0 set when it lands below the diagonal and is stored
 // conjugated.
-__device__ __forceinline__ int moved_loc(int s, int t, int h)
+__device__ __forceinline__ int moved_loc(int s, int t, int h, int nb1)
 {
     const int s2 = slot_next(s, h), t2 = slot_next(t, h);
     const int lo = min(s2, t2), hi = max(s2, t2);
-    const int idx = 4 * tri_idx(lo >> 1, hi >> 1, h) + ((lo & 1) << 1) + (hi & 1);
+    const int idx = gel(tri_idx(lo >> 1, hi >> 1, h), ((lo & 1) << 1) + (hi & 1), nb1);
     return (idx << 1) | (s2 > t2 ? 1 : 0);
 }
 
@@ -286,6 +293,7 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = F + (F & 1);
     const int half = n / 2;
+    const int nb1 = half * (half + 1) / 2 + 1;  // blocks of the triangle + the scratch block
     const EigLayout L(n);
     float2* Gs = reinterpret_cast<float2*>(smem_raw);
     float4* rcs = reinterpret_cast<float4*>(smem_raw + L.rc);
@@ -313,7 +321,7 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
             }
             if (s == t) v.y = 0.f;
         }
-        Gs[4 * tri_idx(s >> 1, t >> 1, half) + ((s & 1) << 1) + (t & 1)] = v;
+        Gs[gel(tri_idx(s >> 1, t >> 1, half), ((s & 1) << 1) + (t & 1), nb1)] = v;
     }
 
     // This thread's blocks: e = tid + kThreads i in row-major order over the upper
@@ -331,14 +339,15 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
             if (k < half) {
                 bk[i] = k;
                 bl[i] = k + rem;
-                src[i] = 4 * tri_idx(k, k + rem, half);
+                src[i] = gel(tri_idx(k, k + rem, half), 0, nb1);
 #pragma unroll
-                for (int el = 0; el < 4; ++el) dst[i][el] = moved_loc(2 * k + (el >> 1), 2 * (k + rem) + (el & 1), half);
+                for (int el = 0; el < 4; ++el)
+                    dst[i][el] = moved_loc(2 * k + (el >> 1), 2 * (k + rem) + (el & 1), half, nb1);
             } else {  // no block: read and write the scratch block, uniformly with the rest of the wave
                 bk[i] = bl[i] = 0;
-                src[i] = 4 * (half * (half + 1) / 2);
+                src[i] = gel(nb1 - 1, 0, nb1);
 #pragma unroll
-                for (int el = 0; el < 4; ++el) dst[i][el] = (src[i] + el) << 1;
+                for (int el = 0; el < 4; ++el) dst[i][el] = gel(nb1 - 1, el, nb1) << 1;
             }
         }
     }
@@ -350,7 +359,7 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
         for (int e = tid; e < n * n; e += kThreads) {
             const int s = e / n, t = e - (e / n) * n;
             if (t < s || (off_only && s == t)) continue;
-            const float2 v = Gs[4 * tri_idx(s >> 1, t >> 1, half) + ((s & 1) << 1) + (t & 1)];
+            const float2 v = Gs[gel(tri_idx(s >> 1, t >> 1, half), ((s & 1) << 1) + (t & 1), nb1)];
             a += (s == t ? 1.0 : 2.0) * ((double)v.x * v.x + (double)v.y * v.y);
         }
         return block_sum(a, red);
@@ -368,12 +377,13 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
             float4 va[kBlkPerThread], vb[kBlkPerThread];
 #pragma unroll
             for (int i = 0; i < kBlkPerThread; ++i) {
-                va[i] = *reinterpret_cast<const float4*>(Gs + src[i]);      // (2k, 2l), (2k, 2l+1)
-                vb[i] = *reinterpret_cast<const float4*>(Gs + src[i] + 2);  // (2k+1, 2l), (2k+1, 2l+1)
+                va[i] = *reinterpret_cast<const float4*>(Gs + src[i]);            // (2k, 2l), (2k, 2l+1)
+                vb[i] = *reinterpret_cast<const float4*>(Gs + src[i] + 2 * nb1);  // (2k+1, 2l), (2k+1, 2l+1)
             }
             if (tid < half) {
-                const float4 d0 = *reinterpret_cast<const float4*>(Gs + 4 * tri_idx(tid, tid, half));
-                const float ap = d0.x, dq = Gs[4 * tri_idx(tid, tid, half) + 3].x;
+                const int bd = tri_idx(tid, tid, half);
+                const float4 d0 = *reinterpret_cast<const float4*>(Gs + gel(bd, 0, nb1));
+                const float ap = d0.x, dq = Gs[gel(bd, 3, nb1)].x;
                 const float2 bpq = float2{d0.z, d0.w};
                 const float ab = hypotf(bpq.x, bpq.y);
                 float c = 1.f, s = 0.f, cf = 1.f, sf = 0.f;
@@ -438,7 +448,7 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
     // g_k from the eigenvalues: index k sits in slot index_slot0(k) again (whole sweeps)
     auto eig_of = [&](int k) {
         const int s = index_slot0(k, n);
-        return Gs[4 * tri_idx(s >> 1, s >> 1, half) + 3 * (s & 1)].x;
+        return Gs[gel(tri_idx(s >> 1, s >> 1, half), 3 * (s & 1), nb1)].x;
     };
     for (int k = tid; k < n; k += kThreads) {
         const float lam = eig_of(k);
