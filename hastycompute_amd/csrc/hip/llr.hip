@@ -41,42 +41,155 @@ __device__ __forceinline__ int64_t row_addr(const LlrGeom& g, int3 c, int64_t r)
     return (e * xyz + (int64_t)(c.x + ix) * g.Y * g.Z + (int64_t)(c.y + iy) * g.Z + (c.z + iz)) * g.rstride;
 }
 
-// Upper-triangle tile index t -> (ti, tj), ti <= tj, row-major over ti.
-__device__ __forceinline__ void tile_ij(int t, int nt, int& ti, int& tj)
-{
-    int i = 0;
-    int rem = t;
-    while (rem >= nt - i) {
-        rem -= nt - i;
-        ++i;
-    }
-    ti = i;
-    tj = i + rem;
-}
-
 // Partial Gram G = A^H A over rows [split * rows_per_split, ...) of each block.
 //
-// Rows are staged through LDS in stages of kGramStage rows, frame-major ([f][row], row
-// stride kGramStage + 1 complex: the LDS serves a 64-bit access 16 lanes at a time over
-// 32 banks, and the 16 frames one operand read touches then start 66 words apart, on
-// distinct bank pairs; the staging stores are contiguous), double-buffered: the next stage's loads (32 threads read
-// 32 rows of one frame: whole z-lines) are in flight while the current stage's MFMAs run,
-// one barrier per stage. Each wave owns up to kMaxTilesPerWave upper 16x16 tiles of G;
-// conj(a) b takes three real MFMAs:
-//   P1 = ar br, P2 = ai bi, P3 = (ar - ai)(br + bi);  Re = P1 + P2, Im = P3 - P1 + P2.
-constexpr int kGramStage = 32;
-constexpr int kGramRowStride = kGramStage + 1;
+// No LDS and no barriers: each wave streams every 16-row slab of the split and owns a
+// fixed set of upper 16x16 tiles of G (tile t = wave + 4 k, as compile-time lists). A
+// slab is loaded straight into the v_mfma_f32_16x16x4f32 operand layout: lane (m, kg)
+// holds A[row][16 t + m] for the four rows 4 kg + ks (ks = 0..3) of frame group t; the
+// same registers are the A operand (conjugated through the product below) of the tiles
+// in row group t and the B operand of those in column group t. Row order inside a slab
+// does not matter for A^H A, so each lane takes four consecutive rows (one row address).
+// The four waves of a workgroup read the same slab (cache hits after the first); the
+// next slab is loaded while the current one's MFMAs run. conj(a) b takes three real
+// MFMAs: P1 = ar br, P2 = ai bi, P3 = (ar - ai)(br + bi); Re = P1 + P2, Im = P3 - P1 + P2.
+// Frames past F read frame F - 1 (they only feed entries of G beyond F, which nothing
+// reads); rows past the split's end are zero.
+// row_addr for r < 2^31 (checked on the host) with 32-bit division.
+__device__ __forceinline__ int64_t row_addr32(const LlrGeom& g, int3 c, uint32_t r)
+{
+    const uint32_t bz = (uint32_t)g.bz, by = (uint32_t)g.by, bx = (uint32_t)g.bx;
+    const uint32_t t1 = r / bz, iz = r - t1 * bz;
+    const uint32_t t2 = t1 / by, iy = t1 - t2 * by;
+    const uint32_t e = t2 / bx, ix = t2 - e * bx;
+    return ((int64_t)e * g.X * g.Y * g.Z + (int64_t)(c.x + (int)ix) * g.Y * g.Z + (int64_t)(c.y + (int)iy) * g.Z +
+            (c.z + (int)iz)) * g.rstride;
+}
+
+__host__ __device__ constexpr int tile_ti(int t, int nt)
+{
+    int i = 0;
+    while (t >= nt - i) {
+        t -= nt - i;
+        ++i;
+    }
+    return i;
+}
+__host__ __device__ constexpr int tile_tj(int t, int nt)
+{
+    int i = 0;
+    while (t >= nt - i) {
+        t -= nt - i;
+        ++i;
+    }
+    return i + t;
+}
+
+// MFMAs of tiles K.. of a wave's list for one slab; the tile's frame groups are
+// constant expressions, so the operands stay in registers.
+template <int NT, int WAVE, int K, int NTW> struct GramTiles {
+    __device__ __forceinline__ static void mfma(const float2 (&a)[NT][4], floatx4* p1, floatx4* p2, floatx4* p3)
+    {
+        constexpr int t = WAVE + 4 * K;
+        constexpr int ti = tile_ti(t, NT), tj = tile_tj(t, NT);
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const float2 x = a[ti][ks], y = a[tj][ks];
+            p1[K] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, y.x, p1[K], 0, 0, 0);
+            p2[K] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, y.y, p2[K], 0, 0, 0);
+            p3[K] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x - x.y, y.x + y.y, p3[K], 0, 0, 0);
+        }
+        GramTiles<NT, WAVE, K + 1, NTW>::mfma(a, p1, p2, p3);
+    }
+};
+template <int NT, int WAVE, int NTW> struct GramTiles<NT, WAVE, NTW, NTW> {
+    __device__ __forceinline__ static void mfma(const float2 (&)[NT][4], floatx4*, floatx4*, floatx4*) {}
+};
+
+template <int NT, int WAVE>
+__device__ __forceinline__ void gram_wave(const float2* __restrict__ data, const LlrGeom& g, int3 c, int64_t r_begin,
+                                          int64_t r_end, float2* __restrict__ G)
+{
+    constexpr int Fp = 16 * NT;
+    constexpr int NTILE = NT * (NT + 1) / 2;
+    constexpr int NTW = NTILE > WAVE ? (NTILE - WAVE + 3) / 4 : 0;
+    if constexpr (NTW > 0) {
+        const int lane = threadIdx.x & 63;
+        const int m = lane & 15, kg = lane >> 4;
+        const int F = (int)g.F;
+        floatx4 p1[NTW], p2[NTW], p3[NTW];
+#pragma unroll
+        for (int k = 0; k < NTW; ++k) {
+            p1[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+            p2[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+            p3[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+        auto load_slab = [&](int64_t r0, float2 (&a)[NT][4]) {
+            int64_t fs = g.fstride;
+            int fm = F - 1;
+            asm volatile("" : "+s"(fs), "+s"(fm));  // opaque per slab: nothing is hoisted out of the slab loop
+            // the lane's rows rr0 .. rr0 + 3: one address when they lie in one z-line
+            const int64_t rr0 = r0 + 4 * kg;
+            int64_t addr[4];
+            bool ok[4];
+            if (rr0 + 3 < r_end && (uint32_t)rr0 % (uint32_t)g.bz + 3 < (uint32_t)g.bz) {
+                const int64_t base = row_addr32(g, c, (uint32_t)rr0);
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks) {
+                    addr[ks] = base + ks * g.rstride;
+                    ok[ks] = true;
+                }
+            } else {
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks) {
+                    ok[ks] = rr0 + ks < r_end;
+                    addr[ks] = row_addr32(g, c, (uint32_t)(ok[ks] ? rr0 + ks : r_begin));
+                }
+            }
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const float2 v = data[addr[ks] + (int64_t)min(16 * t + m, fm) * fs];
+                    a[t][ks] = ok[ks] ? v : float2{0.f, 0.f};
+                }
+            }
+        };
+        auto mfma_slab = [&](const float2 (&a)[NT][4]) { GramTiles<NT, WAVE, 0, NTW>::mfma(a, p1, p2, p3); };
+        float2 a0[NT][4], a1[NT][4];
+        int64_t r0 = r_begin;
+        if (r0 < r_end) load_slab(r0, a0);
+        while (r0 < r_end) {
+            if (r0 + 16 < r_end) load_slab(r0 + 16, a1);
+            mfma_slab(a0);
+            r0 += 16;
+            if (r0 >= r_end) break;
+            if (r0 + 16 < r_end) load_slab(r0 + 16, a0);
+            mfma_slab(a1);
+            r0 += 16;
+        }
+#pragma unroll
+        for (int k = 0; k < NTW; ++k) {
+            const int t = WAVE + 4 * k;
+            const int ti = tile_ti(t, NT), tj = tile_tj(t, NT);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = ti * 16 + 4 * kg + v;
+                const int j = tj * 16 + m;
+                const float re = p1[k][v] + p2[k][v];
+                const float im = p3[k][v] - p1[k][v] + p2[k][v];
+                G[(int64_t)i * Fp + j] = float2{re, im};
+                if (ti != tj) G[(int64_t)j * Fp + i] = float2{re, -im};
+            }
+        }
+    }
+}
 
 template <int NT>
 __global__ void __launch_bounds__(kThreads) gram_kernel(const float2* __restrict__ data, LlrGeom g, const int3* corners,
                                                        int64_t rows_per_split, float2* __restrict__ gpart)
 {
     constexpr int Fp = 16 * NT;
-    constexpr int PER = kGramStage * Fp / kThreads;  // staged elements per thread
-    static_assert(kGramStage * Fp % kThreads == 0, "stage must split evenly over the threads");
-    static_assert(kThreads % kGramStage == 0, "a thread keeps one row of every stage");
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    float2* slab = reinterpret_cast<float2*>(smem_raw);  // [2][Fp][kGramRowStride]
     const int blk = blockIdx.x;
     const int split = blockIdx.y;
     const int nsplit = gridDim.y;
@@ -84,100 +197,20 @@ __global__ void __launch_bounds__(kThreads) gram_kernel(const float2* __restrict
     const int64_t R = g.E * g.bx * g.by * g.bz;
     const int64_t r_begin = split * rows_per_split;
     const int64_t r_end = min(R, r_begin + rows_per_split);
-    const int F = (int)g.F;
-    const int ntile = NT * (NT + 1) / 2;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int m = lane & 15, kg = lane >> 4;
-
-    floatx4 p1[kMaxTilesPerWave], p2[kMaxTilesPerWave], p3[kMaxTilesPerWave];
-    int tI[kMaxTilesPerWave], tJ[kMaxTilesPerWave];
-#pragma unroll
-    for (int k = 0; k < kMaxTilesPerWave; ++k) {
-        p1[k] = floatx4{0.f, 0.f, 0.f, 0.f};
-        p2[k] = floatx4{0.f, 0.f, 0.f, 0.f};
-        p3[k] = floatx4{0.f, 0.f, 0.f, 0.f};
-        const int t = wave + 4 * k;
-        if (t < ntile)
-            tile_ij(t, NT, tI[k], tJ[k]);
-        else
-            tI[k] = tJ[k] = -1;
-    }
-
-    // staging: this thread's row of every stage and its frames f0 + (kThreads / kGramStage) i
-    const int srow = tid % kGramStage;
-    const int f0 = tid / kGramStage;
-    constexpr int fstep = kThreads / kGramStage;
-    float2 st[PER];
-    auto load_stage = [&](int64_t r0) {
-        const int64_t r = r0 + srow;
-        const bool ok = r < r_end;
-        const int64_t base = row_addr(g, c, ok ? r : r_begin);
-#pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int f = f0 + fstep * i;
-            st[i] = (ok && f < F) ? data[base + (int64_t)f * g.fstride] : float2{0.f, 0.f};
-        }
-    };
-    auto store_stage = [&](int buf) {
-        float2* s = slab + buf * Fp * kGramRowStride;
-#pragma unroll
-        for (int i = 0; i < PER; ++i) s[(f0 + fstep * i) * kGramRowStride + srow] = st[i];
-    };
-
-    const int64_t nst = r_end > r_begin ? (r_end - r_begin + kGramStage - 1) / kGramStage : 0;
-    if (nst > 0) {
-        load_stage(r_begin);
-        store_stage(0);
-    }
-    __syncthreads();
-    for (int64_t s = 0; s < nst; ++s) {
-        const int buf = (int)(s & 1);
-        if (s + 1 < nst) load_stage(r_begin + (s + 1) * kGramStage);
-        const float2* sb = slab + buf * Fp * kGramRowStride + m * kGramRowStride + kg;
-#pragma unroll
-        for (int k = 0; k < kMaxTilesPerWave; ++k) {
-            if (tI[k] < 0) continue;
-            const float2* pa = sb + 16 * tI[k] * kGramRowStride;
-            const float2* pb = sb + 16 * tJ[k] * kGramRowStride;
-#pragma unroll
-            for (int ks = 0; ks < kGramStage / 4; ++ks) {
-                const float2 a = pa[4 * ks], b = pb[4 * ks];
-                p1[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, p1[k], 0, 0, 0);
-                p2[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, p2[k], 0, 0, 0);
-                p3[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x - a.y, b.x + b.y, p3[k], 0, 0, 0);
-            }
-        }
-        if (s + 1 < nst) store_stage(buf ^ 1);
-        __syncthreads();
-    }
-
     float2* G = gpart + ((int64_t)blk * nsplit + split) * Fp * Fp;
-#pragma unroll
-    for (int k = 0; k < kMaxTilesPerWave; ++k) {
-        if (tI[k] < 0) continue;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int i = tI[k] * 16 + 4 * kg + v;
-            const int j = tJ[k] * 16 + m;
-            const float re = p1[k][v] + p2[k][v];
-            const float im = p3[k][v] - p1[k][v] + p2[k][v];
-            G[(int64_t)i * Fp + j] = float2{re, im};
-            if (tI[k] != tJ[k]) G[(int64_t)j * Fp + i] = float2{re, -im};
-        }
+    switch (threadIdx.x >> 6) {
+    case 0: gram_wave<NT, 0>(data, g, c, r_begin, r_end, G); break;
+    case 1: gram_wave<NT, 1>(data, g, c, r_begin, r_end, G); break;
+    case 2: gram_wave<NT, 2>(data, g, c, r_begin, r_end, G); break;
+    default: gram_wave<NT, 3>(data, g, c, r_begin, r_end, G); break;
     }
 }
-
-size_t gram_smem(int64_t F) { return sizeof(float2) * 2 * (size_t)llr_padded(F) * kGramRowStride; }
 
 template <int NT>
 void launch_gram(const float2* data, const LlrGeom& g, const int3* corners, int64_t nblk, int nsplit, int64_t rps,
                  float2* gpart, hipStream_t s)
 {
-    const size_t smem = gram_smem(g.F);
-    allow_lds(gram_kernel<NT>, smem);
-    hipLaunchKernelGGL(gram_kernel<NT>, dim3((unsigned)nblk, (unsigned)nsplit), dim3(kThreads), smem, s, data, g,
+    hipLaunchKernelGGL(gram_kernel<NT>, dim3((unsigned)nblk, (unsigned)nsplit), dim3(kThreads), 0, s, data, g,
                        corners, rps, gpart);
     check_launch("gram_kernel");
 }
@@ -756,6 +789,7 @@ void llr_blocks(float2* data, const LlrGeom& g, const int3* corners_dev, int64_t
     llr_check(g.F);
     if (nblk <= 0) return;
     HASTY_REQUIRE(nblk < 65536, "too many blocks in one LLR launch");
+    HASTY_REQUIRE(g.E * g.bx * g.by * g.bz < (int64_t(1) << 31), "LLR block has too many rows");
     HASTY_REQUIRE(ws_bytes >= llr_workspace_bytes(nblk, g.F, nsplit), "LLR workspace too small");
     const int Fp = llr_padded(g.F);
     const int64_t R = g.E * g.bx * g.by * g.bz;
