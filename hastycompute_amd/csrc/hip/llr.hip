@@ -604,7 +604,7 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
 //   P1 = Wr Ar, P2 = Wi Ai, P3 = (Wr + Wi)(Ar + Ai);  Re = P1 - P2, Im = P3 - P1 - P2.
 // K runs over F rounded up to 4 only. A wave reads a slab whole before writing it and
 // slabs are disjoint, so the update is in place; the next slab is loaded during the
-// current one's MFMAs. The main loop has no workgroup barrier.
+// current one's MFMAs, half by half. The main loop has no workgroup barrier.
 constexpr int kApplyThreads = 512;
 constexpr int kApplySlabsPerWave = 4;
 
@@ -645,15 +645,21 @@ __global__ void __launch_bounds__(kApplyThreads) apply_kernel(float2* __restrict
     // one k-step per register: lane holds A[row r0 + n][frame 4 ks + kg]. Frames past F
     // (the last k-step) read frame F - 1 and rows past the end read the last row: both
     // meet zero rows of W or are not stored, so no per-lane load masks are needed.
-    auto load_slab = [&](int64_t r0, float2* a, int64_t& addr, bool& valid) {
+    // k-steps [ks0, ks1) of one slab: lane holds A[row][frame 4 ks + kg]. Frames past F
+    // (the last k-step) read frame F - 1 and rows past the end read the last row: both
+    // meet zero rows of W or are not stored, so no per-lane load masks are needed.
+    auto slab_row = [&](int64_t r0, int64_t& addr, bool& valid) {
         const int64_t r = r0 + n;
         valid = r < r_end;
         addr = row_addr(g, c, valid ? r : r_end - 1);
+    };
+    auto load_ks = [&](float2* a, int64_t addr, int ks0, int ks1) {
         int64_t fs = g.fstride;
         int fm = F - 1, k4 = K4;
         asm volatile("" : "+s"(fs), "+s"(fm), "+s"(k4));  // opaque per slab: nothing here is hoisted out of the slab loop
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
+            if (ks < ks0 || ks >= ks1) continue;
             if (ks < k4) {
                 const int k = min(4 * ks + kg, fm);
                 a[ks] = data[addr + (int64_t)k * fs];
@@ -663,13 +669,18 @@ __global__ void __launch_bounds__(kApplyThreads) apply_kernel(float2* __restrict
         }
     };
 
+    // The next slab's operands are loaded in two halves into registers that the current
+    // slab no longer needs: the first half after the current slab's first-half MFMAs (so
+    // its latency hides behind the second half), the second half before the stores.
+    constexpr int H = KS / 2;
     const int waves = kApplyThreads / 64;
     float2 a[KS];
     int64_t addr;
     bool valid;
     int64_t r0 = r_begin + 16 * wave;
     if (r0 >= r_end) return;  // no barrier follows
-    load_slab(r0, a, addr, valid);
+    slab_row(r0, addr, valid);
+    load_ks(a, addr, 0, KS);
     for (; r0 < r_end; r0 += 16 * waves) {
         floatx4 p1[NT], p2[NT], p3[NT];
 #pragma unroll
@@ -680,29 +691,38 @@ __global__ void __launch_bounds__(kApplyThreads) apply_kernel(float2* __restrict
         }
         int woff = kg * S + n, k4 = K4;
         asm volatile("" : "+v"(woff), "+s"(k4));  // per-k-step LDS bases and tests are formed here, not hoisted
+        auto mfma_ks = [&](int ks0, int ks1) {
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            if (ks < k4) {
-                const float ar = a[ks].x, ai = a[ks].y, as = ar + ai;
-                const float2* wrow = sW + woff + 4 * ks * S;
+            for (int ks = 0; ks < KS; ++ks) {
+                if (ks < ks0 || ks >= ks1) continue;
+                if (ks < k4) {
+                    const float ar = a[ks].x, ai = a[ks].y, as = ar + ai;
+                    const float2* wrow = sW + woff + 4 * ks * S;
 #pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    const float2 w = wrow[16 * t];
-                    p1[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.x, ar, p1[t], 0, 0, 0);
-                    p2[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.y, ai, p2[t], 0, 0, 0);
-                    p3[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.x + w.y, as, p3[t], 0, 0, 0);
+                    for (int t = 0; t < NT; ++t) {
+                        const float2 w = wrow[16 * t];
+                        p1[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.x, ar, p1[t], 0, 0, 0);
+                        p2[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.y, ai, p2[t], 0, 0, 0);
+                        p3[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.x + w.y, as, p3[t], 0, 0, 0);
+                    }
                 }
             }
+        };
+        const bool more = r0 + 16 * waves < r_end;
+        int64_t addr_n = addr;
+        bool valid_n = false;
+        mfma_ks(0, H);
+        if (more) {
+            slab_row(r0 + 16 * waves, addr_n, valid_n);
+            load_ks(a, addr_n, 0, H);
         }
-        // the operand registers are free: start the next slab's loads before the stores
-        const int64_t addr_out = addr;
-        const bool valid_out = valid;
-        if (r0 + 16 * waves < r_end) load_slab(r0 + 16 * waves, a, addr, valid);
+        mfma_ks(H, KS);
+        if (more) load_ks(a, addr_n, H, KS);
         // lane holds out[row n][col 16 t + 4 kg + v]; only the last tile can pass F
-        if (valid_out) {
+        if (valid) {
             int64_t fs = g.fstride;
             asm volatile("" : "+s"(fs));
-            float2* q = data + addr_out + (int64_t)(4 * kg) * fs;
+            float2* q = data + addr + (int64_t)(4 * kg) * fs;
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
 #pragma unroll
@@ -716,6 +736,8 @@ __global__ void __launch_bounds__(kApplyThreads) apply_kernel(float2* __restrict
                 }
             }
         }
+        addr = addr_n;
+        valid = valid_n;
     }
 }
 
