@@ -134,6 +134,20 @@ __device__ __forceinline__ void gram_wave(const float2* __restrict__ data, const
             bool ok[4];
             if (rr0 + 3 < r_end && (uint32_t)rr0 % (uint32_t)g.bz + 3 < (uint32_t)g.bz) {
                 const int64_t base = row_addr32(g, c, (uint32_t)rr0);
+                if (g.rstride == 1 && ((base | fs) & 1) == 0) {
+                    // four consecutive complex values per frame, 16-byte aligned: two 16-byte loads
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        const float2* p = data + base + (int64_t)min(16 * t + m, fm) * fs;
+                        const float4 v0 = *reinterpret_cast<const float4*>(p);
+                        const float4 v1 = *reinterpret_cast<const float4*>(p + 2);
+                        a[t][0] = float2{v0.x, v0.y};
+                        a[t][1] = float2{v0.z, v0.w};
+                        a[t][2] = float2{v1.x, v1.y};
+                        a[t][3] = float2{v1.z, v1.w};
+                    }
+                    return;
+                }
 #pragma unroll
                 for (int ks = 0; ks < 4; ++ks) {
                     addr[ks] = base + ks * g.rstride;
