@@ -435,11 +435,16 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
                 const float ab = hypotf(bpq.x, bpq.y);
                 float c = 1.f, s = 0.f, cf = 1.f, sf = 0.f;
                 if (ab > 1e-30f && ab > 1e-9f * (fabsf(ap) + fabsf(dq))) {
-                    cf = bpq.x / ab;
-                    sf = bpq.y / ab;
-                    const float tau = (dq - ap) / (2.f * ab);
-                    const float t = (tau >= 0.f ? 1.f : -1.f) / (fabsf(tau) + sqrtf(1.f + tau * tau));
-                    c = 1.f / sqrtf(1.f + t * t);
+                    // hardware reciprocal / square root (1 ulp): this chain gates every round,
+                    // and the rotation applied is the one recorded, with |e^{i phi}| and
+                    // c^2 + s^2 equal to 1 to FP32 rounding (tau overflowing gives t = 0)
+                    const float inv = __builtin_amdgcn_rcpf(ab);
+                    cf = bpq.x * inv;
+                    sf = bpq.y * inv;
+                    const float tau = (dq - ap) * 0.5f * inv;
+                    const float t = copysignf(
+                        __builtin_amdgcn_rcpf(fabsf(tau) + __builtin_amdgcn_sqrtf(fmaf(tau, tau, 1.f))), tau);
+                    c = __builtin_amdgcn_rsqf(fmaf(t, t, 1.f));
                     s = t * c;
                 }
                 const float4 r4 = make_float4(c, s, cf, sf);
