@@ -282,6 +282,22 @@ def test_llr_normal_blocks_vs_oracle(N, F, E, X, blk, it, soft):
     assert e < 1e-4
 
 
+@pytest.mark.parametrize("F,soft", [(100, True), (20, False)])
+def test_llr_repeatable(N, F, soft):
+    """The LLR prox is deterministic: partial Grams are summed in a fixed order, the
+    eigensolver and A W have no atomics, so two runs on the same input agree bit for bit."""
+    rng = np.random.default_rng(11 + F)
+    x = _lowrank(rng, F, 1, 64, 5, 0.05).astype(np.complex64)
+    A = O.extract_block(x.astype(np.complex128), (0, 0, 0), (16,) * 3)
+    thresh = 0.05 * np.linalg.svd(A, compute_uv=False)[0] if soft else 3
+    x1, x2 = g(x), g(x)
+    N.llr_normal_blocks(x1, (16,) * 3, (16,) * 3, 2, thresh, soft)
+    N.llr_normal_blocks(x2, (16,) * 3, (16,) * 3, 2, thresh, soft)
+    torch.cuda.synchronize()
+    assert not torch.equal(x1, g(x))
+    assert torch.equal(x1, x2)
+
+
 def test_svt_matrix_vs_numpy(N):
     rng = np.random.default_rng(3)
     for R, F in [(500, 20), (4096, 100), (64, 100)]:
