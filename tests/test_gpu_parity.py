@@ -314,6 +314,27 @@ def test_svt_matrix_vs_numpy(N):
         assert e < 1e-4 and e2 < 1e-4
 
 
+@pytest.mark.parametrize("F", [1, 2, 3])
+def test_svt_matrix_smallest(N, F):
+    """One to three columns: n = 2 or 4 in the eigensolver (a single pair, or the
+    padding column), soft and hard thresholds."""
+    rng = np.random.default_rng(40 + F)
+    A = rnd(rng, (50, F)).astype(np.complex64)
+    s = np.linalg.svd(A.astype(np.complex128), compute_uv=False)
+    for lam in (0.5 * s[-1], 0.5 * (s[0] + s[-1]), 2.0 * s[0]):
+        Ag = g(A)
+        N.svt_matrix(Ag, lam, True)
+        e = O.rel_l2(h(Ag), O.svt_soft(A.astype(np.complex128), lam)) if lam < s[0] else float(np.abs(h(Ag)).max())
+        print(f"svt F={F} lam={lam:.3g}: {e:.2e}", flush=True)
+        assert e < 1e-4
+    for rank in range(F + 1):
+        Ah = g(A)
+        N.svt_matrix(Ah, rank, False)
+        ref = O.svt_hard(A.astype(np.complex128), rank)
+        e2 = O.rel_l2(h(Ah), ref) if rank > 0 else float(np.abs(h(Ah)).max())
+        assert e2 < 1e-4, (rank, e2)
+
+
 @pytest.mark.parametrize("R,F", [(300, 33), (400, 128), (250, 100), (40, 7)])
 def test_svt_matrix_many_kept(N, R, F):
     """Thresholds that keep all, about half, or one singular value: the eigenvectors are
