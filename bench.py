@@ -307,7 +307,7 @@ def copy_calibration(torch):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (how the path would shard over "
+        print(json.dumps({"error": "bench.py runs on one GPU; --gpus must be 1 (how the path would shard over "
                                    "several GPUs is described, not launched)", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch

@@ -83,7 +83,7 @@ class FivePointFULL:
             self.sensenormalop = BatchedSenseNormal(self.coord_vec, self.smaps, self.kdata_vec, self.weights_vec,
                                                     self.streams)
         else:
-            raise RuntimeError(f"solver {solver!r} is not part of this build (GD is)")
+            raise RuntimeError(f"solver {solver!r} is not supported (GD is)")
         self.lamda = lamda / self.stepsize
         if float(self.lamda) != 0.0:
             self._dctprox = prox.ProximalL1DCT((5, 1) + tuple(self.smaps.shape[1:]), (True, True, True),
@@ -131,7 +131,7 @@ class FivePointLLR:
         elif self.solver == "Gridding":
             pass
         else:
-            raise RuntimeError(f"solver {solver!r} is not part of this build (GD and Gridding are)")
+            raise RuntimeError(f"solver {solver!r} is not supported (GD and Gridding are)")
 
         sp = tuple(self.smaps.shape[1:])
         self.svtshape = (self.nframes, 5) + sp

@@ -194,7 +194,7 @@ void CUDANormalNufftToeplitz::build_diagonal(const at::Tensor& coords, std::vect
     if (!diagonal.defined() || diagonal.sizes() != at::IntArrayRef(ns))
         throw std::runtime_error("diagonal must have the doubled nmodes shape");
     if (diagonal.scalar_type() != at::ScalarType::ComplexFloat)
-        throw std::runtime_error("this build keeps the Toeplitz diagonal in complex64");
+        throw std::runtime_error("the Toeplitz diagonal is kept in complex64");
     std::vector<int64_t> img(nmodes.begin() + 1, nmodes.end());
     int64_t per = 1;
     for (size_t i = 1; i < ns.size(); ++i) per *= ns[i];
@@ -781,7 +781,7 @@ void insert_block(at::Tensor& in, const Block<3>& block, at::Tensor& block_tenso
         view_lens[view_lens.size() - 3 + i] = block.second_corner[i] - block.first_corner[i];
     at::Tensor bt = block_tensor.contiguous();
     if (transpose) bt = bt.transpose(0, 1).contiguous();
-    if (perms.has_value()) throw std::runtime_error("permuted insert_block is not part of this build");
+    if (perms.has_value()) throw std::runtime_error("permuted insert_block is not supported");
     in.index_put_({"...", Slice(block.first_corner[0], block.second_corner[0]),
                    Slice(block.first_corner[1], block.second_corner[1]),
                    Slice(block.first_corner[2], block.second_corner[2])},
