@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import platform
 import sys
@@ -61,9 +62,10 @@ def parse():
     ap.add_argument("--no-kernel-events", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (the known-size kernel for counter calibration)")
-    ap.add_argument("--mode", choices=["normal", "llr", "serve"], default="normal",
+    ap.add_argument("--mode", choices=["normal", "llr", "serve", "nufft"], default="normal",
                     help="normal: the batched normal operator (default); llr: the LLR prox on 128^3 x 100 frames; "
-                         "serve: the Poisson request stream through the continuous-batching server")
+                         "serve: the Poisson request stream through the continuous-batching server; "
+                         "nufft: type-2 and type-1 NUFFT at 128^3 with one frame's trajectory (1150 x 128)")
     ap.add_argument("--llr-frames", type=int, default=100)
     ap.add_argument("--llr-max-sweeps", type=int, default=None, help="cap on the LLR Jacobi sweeps (1..15)")
     ap.add_argument("--requests", type=int, default=200)
@@ -73,6 +75,76 @@ def parse():
 
 
 MFMA_FP32_PEAK_TFLOPS = 157.3
+
+
+def nufft_main(args, torch, native):
+    """Type-2 and type-1 NUFFT at 128^3 (fine grid 256^3, tol 1e-6, ns = 7) on one
+    4D-flow frame's kooshball (1150 spokes x 128 samples, M = 147 200). A step is one
+    transform of each type; the value is transforms per second. Roofline: HBM, with the
+    algorithmic bytes B_U = 20 M + 8 N^3 + 32 (2N)^3 per transform (coordinates and
+    samples, the image, four fine-grid volumes)."""
+    import numpy as np
+
+    from hastycompute_amd import synthetic
+
+    dev = torch.device("cuda:0")
+    n, spokes, samp = args.n, 1150, 128
+    co = synthetic.kooshball_torch(spokes, samp, 0, dev).contiguous()
+    M = co.shape[1]
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    f = torch.randn((1, n, n, n), dtype=torch.complex64, device=dev, generator=g)
+    c = torch.randn((1, M), dtype=torch.complex64, device=dev, generator=g)
+    scale = 1.0 / math.sqrt(n ** 3)
+    p2 = native.NufftPlan(co, [n, n, n], 2, -1, 1, 1e-6, scale)
+    p1 = native.NufftPlan(co, [n, n, n], 1, +1, 1, 1e-6, scale)
+    out2 = torch.empty((1, M), dtype=torch.complex64, device=dev)
+    out1 = torch.empty((1, n, n, n), dtype=torch.complex64, device=dev)
+    for _ in range(args.warmup):
+        p2.execute(out2, f)
+        p1.execute(c, out1)
+    B = 20.0 * M + 8.0 * n ** 3 + 32.0 * (2 * n) ** 3
+    res = {}
+    for name, fn in (("type2", lambda: p2.execute(out2, f)), ("type1", lambda: p1.execute(c, out1))):
+        torch.cuda.synchronize()
+        native.timing_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        kt = native.timing_read()
+        native.timing_enable(False)
+        res[name] = {"ms": dt * 1e3, "achieved_GBs": B / dt / 1e9,
+                     "kernels_ms": {k: v[0] / v[1] for k, v in kt.items()}}
+    p1.close()
+    p2.close()
+    # CPU: the FP64 oracle's ES-kernel type 2 on a bounded sample of the points
+    from oracle import oracle as O
+
+    Ms = 8192
+    co_s = co[:, :Ms].double().cpu().numpy()
+    f_s = f[0].cpu().numpy().astype(np.complex128)
+    t = time.perf_counter()
+    O.nufft_type2(co_s, f_s, -1, 1e-6)
+    cdt = time.perf_counter() - t
+    tr_s = 1e3 / sum(r["ms"] for r in res.values())
+    dom = max(res, key=lambda k: res[k]["ms"])
+    line = {"metric": "NUFFT type-2 + type-1 pairs/s @128^3, M=147200 (kooshball 1150x128), tol 1e-6",
+            "value": tr_s, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 / tr_s, "higher_is_better": True, "dtype": "c64 (fp32)",
+            "data": "synthetic (seeded kooshball, random image and samples)",
+            "config": {"workload": f"NUFFT {n}^3, fine {2 * n}^3, ns 7, M={M}", "M": M},
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": res[dom]["achieved_GBs"], "peak": 8000.0,
+                         "unit": "GB/s", "frac": res[dom]["achieved_GBs"] / 8000.0, "traffic": None,
+                         "alg_bytes_per_transform": B},
+            "transforms": res,
+            "cpu_baseline": {"value": 1.0 / (cdt * M / Ms), "unit": "type-2 transforms/s", "cores": O.default_threads(),
+                             "kind": "port",
+                             "sample": f"FP64 oracle ES-kernel type 2 (scipy.fft) with {Ms} of the {M} points, "
+                                       f"scaled to M by the point count (the FFT part is not scaled)",
+                             "seconds": cdt}}
+    print(json.dumps(line), flush=True)
 
 
 def llr_main(args, torch, native):
@@ -320,6 +392,9 @@ def main():
         return
     if args.mode == "llr":
         llr_main(args, torch, native)
+        return
+    if args.mode == "nufft":
+        nufft_main(args, torch, native)
         return
     if args.mode == "serve":
         serve_main(args, torch)

@@ -35,10 +35,14 @@ __device__ __forceinline__ double fine_pos(double x, int64_t nf)
     return t;
 }
 
+// i mod n for the indices the kernels form (tile cells and kernel taps lie within a
+// kernel width plus a tile of the grid, so one step usually suffices; the loops keep any
+// argument correct without a 64-bit division)
 __device__ __forceinline__ int64_t wrapi(int64_t i, int64_t n)
 {
-    i %= n;
-    return i < 0 ? i + n : i;
+    while (i < 0) i += n;
+    while (i >= n) i -= n;
+    return i;
 }
 
 template <typename R> __device__ __forceinline__ R es_eval(R z, R beta, R c)
