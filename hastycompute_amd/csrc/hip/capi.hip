@@ -310,6 +310,10 @@ int hasty_set_option(const char* name, int64_t value)
         sense_fused_set_xy_variant((int)value);
         return 0;
     }
+    if (name && std::strcmp(name, "nufft_spread_variant") == 0 && value >= 0 && value <= 3) {
+        nufft_set_spread_variant((int)value);
+        return 0;
+    }
     if (name && std::strcmp(name, "llr_max_sweeps") == 0 && value >= 1 && value <= 15) {
         llr_set_max_sweeps((int)value);
         return 0;

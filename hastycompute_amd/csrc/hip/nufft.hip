@@ -9,9 +9,14 @@
 // setpts bins points into tiles of the fine grid with a device counting sort; the
 // spreader gives each (tile, chunk of <= kMaxSub points) one workgroup that
 // accumulates into an LDS copy of the tile plus its halo with LDS atomics and then
-// flushes the tile to HBM with one global atomic per non-zero cell. Interpolation
+// flushes the tile to HBM with one global atomic per non-zero cell. In 3-D each lane
+// walks the ns x ns rows of its point's window from its own starting row, and the LDS
+// tile's row and plane pitches are padded so that those rows fall into different
+// banks: points at one position (every spoke of a radial trajectory crosses k = 0)
+// then update different words at any instant instead of serialising on one. Interpolation
 // walks points in tile order so the w^3 gathers hit L2.
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <vector>
 
@@ -22,7 +27,8 @@ namespace hasty_hip {
 
 namespace {
 
-constexpr int kMaxSub = 1024;
+constexpr int kMaxSub = 256;
+int g_spread_variant = 3;  // 3-D complex64 spreader, values as nufft_set_spread_variant
 constexpr int kThreads = 256;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
@@ -157,7 +163,21 @@ template <typename R> struct SpreadArgs {
     cplx<R>* fine;
     R beta;
     R cfac;  // 4 / ns^2
+    int rp;  // LDS tile pitches (complex elements) between rows (last axis) and planes
+    int pp;
 };
+
+// LDS tile pitches for tile extents T and kernel width ns. With e complex elements
+// spanning the 32 banks, the row pitch rp is odd modulo e and the plane pitch is
+// ns * rp modulo e, so row k = ia * ns + ib of a point's window starts at bank group
+// rp * k mod e and consecutive rows cycle through all e groups.
+inline void tile_pitches(const int T[3], int ns, int esize, int& rp, int& pp)
+{
+    const int e = 128 / esize;
+    rp = T[2] + ((T[2] % e) % 2 == 0 ? 1 : 0);
+    const int plane = T[1] * rp;
+    pp = plane + (((ns * rp - plane) % e) + e) % e;
+}
 
 template <typename R, int NS, int DIM>
 __global__ void __launch_bounds__(kThreads) spread_kernel(SpreadArgs<R> a)
@@ -187,7 +207,8 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(SpreadArgs<R> a)
         o[d] = act ? bb[d] * pa.bs[d] - (NS + 1) / 2 : 0;
     }
     const int cells = T[0] * T[1] * T[2];
-    for (int i = threadIdx.x; i < cells; i += kThreads) tile[i] = cplx<R>{(R)0, (R)0};
+    const int rp = a.rp, pp = a.pp;
+    for (int i = threadIdx.x; i < T[0] * pp; i += kThreads) tile[i] = cplx<R>{(R)0, (R)0};
     __syncthreads();
 
     for (int p = start + threadIdx.x; p < end; p += kThreads) {
@@ -195,7 +216,7 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(SpreadArgs<R> a)
         const cplx<R> cj = a.c[j];
         R w[2][NS];
         int rel[3] = {0, 0, 0};
-        double t0 = 0.0, l00 = 0.0;
+        double t0 = 0.0, l00 = 0.0, t1 = 0.0, l01 = 0.0;
 #pragma unroll
         for (int d = 3 - DIM; d < 3; ++d) {
             const double t = fine_pos((double)pa.pts[d][j], pa.nf[d]);
@@ -204,31 +225,43 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(SpreadArgs<R> a)
             if (d == 0) {
                 t0 = t;
                 l00 = l0;
+            } else if (DIM == 3 && d == 1) {
+                t1 = t;
+                l01 = l0;
             } else {
 #pragma unroll
                 for (int q = 0; q < NS; ++q) w[d - 1][q] = es_eval<R>((R)(l0 + q - t), a.beta, a.cfac);
             }
         }
         if (DIM == 3) {
-            for (int ia = 0; ia < NS; ++ia) {
-                const R wa = es_eval<R>((R)(l00 + ia - t0), a.beta, a.cfac);
+            // rows (ia, ib) of the window in lane-rotated order
+            int ia, ib;
+            {
+                const int r0 = (int)(threadIdx.x & 63) % (NS * NS);
+                ia = r0 / NS;
+                ib = r0 - ia * NS;
+            }
+            cplx<R>* base = tile + rel[0] * pp + rel[1] * rp + rel[2];
+            for (int k = 0; k < NS * NS; ++k) {
+                const R wab = es_eval<R>((R)(l00 + ia - t0), a.beta, a.cfac) *
+                              es_eval<R>((R)(l01 + ib - t1), a.beta, a.cfac);
+                cplx<R>* row = base + ia * pp + ib * rp;
 #pragma unroll
-                for (int ib = 0; ib < NS; ++ib) {
-                    const R wab = wa * w[0][ib];
-                    cplx<R>* row = tile + ((rel[0] + ia) * T[1] + rel[1] + ib) * T[2] + rel[2];
-#pragma unroll
-                    for (int q = 0; q < NS; ++q) {
-                        const R ww = wab * w[1][q];
-                        atomicAdd(&row[q].x, ww * cj.x);
-                        atomicAdd(&row[q].y, ww * cj.y);
-                    }
+                for (int q = 0; q < NS; ++q) {
+                    const R ww = wab * w[1][q];
+                    atomicAdd(&row[q].x, ww * cj.x);
+                    atomicAdd(&row[q].y, ww * cj.y);
+                }
+                if (++ib == NS) {
+                    ib = 0;
+                    if (++ia == NS) ia = 0;
                 }
             }
         } else if (DIM == 2) {
 #pragma unroll
             for (int ib = 0; ib < NS; ++ib) {
                 const R wb = w[0][ib];
-                cplx<R>* row = tile + (rel[1] + ib) * T[2] + rel[2];
+                cplx<R>* row = tile + (rel[1] + ib) * rp + rel[2];
 #pragma unroll
                 for (int q = 0; q < NS; ++q) {
                     const R ww = wb * w[1][q];
@@ -248,15 +281,128 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(SpreadArgs<R> a)
     __syncthreads();
 
     for (int i = threadIdx.x; i < cells; i += kThreads) {
-        const cplx<R> v = tile[i];
-        if (v.x == (R)0 && v.y == (R)0) continue;
         const int i2 = i % T[2];
         const int i1 = (i / T[2]) % T[1];
         const int i0 = i / (T[2] * T[1]);
+        const cplx<R> v = tile[i0 * pp + i1 * rp + i2];
+        if (v.x == (R)0 && v.y == (R)0) continue;
         const int64_t g0 = wrapi(o[0] + i0, pa.nf[0]);
         const int64_t g1 = wrapi(o[1] + i1, pa.nf[1]);
         const int64_t g2 = wrapi(o[2] + i2, pa.nf[2]);
         cplx<R>* dst = a.fine + (g0 * pa.nf[1] + g1) * pa.nf[2] + g2;
+        atomicAdd(&dst->x, v.x);
+        atomicAdd(&dst->y, v.y);
+    }
+}
+
+// 3-D spreading by window rows (complex64 tiles). A batch of kRowBatch points first
+// stages its base offset, strength and 3 x NS kernel weights in LDS (one lane per
+// point); then the NW waves of the group take the batch's points in turn, one point at a
+// time per wave, lane ia * NS + ib updating the NS cells of window row (ia, ib). The
+// lanes of one instruction touch NS * NS distinct rows, which the padded pitches place
+// in distinct bank groups. With RMW the first wave alone accumulates, updating the tile
+// with plain loads and stores (LDS executes one wave's operations in order), and the NW
+// waves only clear and flush it; otherwise the waves share the points and the tile
+// through LDS atomics.
+constexpr int kRowBatch = 32;
+
+template <int NS, int NW, bool RMW>
+__global__ void __launch_bounds__(64 * NW) spread_rows_kernel(SpreadArgs<float> a)
+{
+    constexpr int S = 3 * NS + 3;  // staged words per point
+    __shared__ float stage[kRowBatch * S];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<float>* tile = reinterpret_cast<cplx<float>*>(smem_raw);
+    const PtsArgs<float>& pa = a.pa;
+    const int sp = blockIdx.x;
+    const int bin = a.subbin[sp];
+    const int start = a.binstart[bin] + a.suboff[sp];
+    const int end = min(a.binstart[bin + 1], start + kMaxSub);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+
+    int64_t bb[3];
+    {
+        int64_t r = bin;
+        bb[2] = r % pa.nbins[2];
+        r /= pa.nbins[2];
+        bb[1] = r % pa.nbins[1];
+        bb[0] = r / pa.nbins[1];
+    }
+    int T[3];
+    int64_t o[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        T[d] = pa.bs[d] + NS + 1;
+        o[d] = bb[d] * pa.bs[d] - (NS + 1) / 2;
+    }
+    const int rp = a.rp, pp = a.pp;
+    for (int i = tid; i < T[0] * pp; i += 64 * NW) tile[i] = cplx<float>{0.f, 0.f};
+
+    const int ia = lane / NS;
+    const int ib = lane - ia * NS;
+    const bool rowlane = lane < NS * NS;
+    const int rowoff = ia * pp + ib * rp;
+    for (int b0 = start; b0 < end; b0 += kRowBatch) {
+        const int np = min(kRowBatch, end - b0);
+        __syncthreads();
+        if (tid < np) {
+            const int j = a.sortidx[b0 + tid];
+            const cplx<float> cj = a.c[j];
+            float* st = stage + tid * S;
+            int off = 0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double t = fine_pos((double)pa.pts[d][j], pa.nf[d]);
+                const double l0 = ceil(t - 0.5 * NS);
+                const int rel = (int)((int64_t)l0 - o[d]);
+                off += rel * (d == 0 ? pp : (d == 1 ? rp : 1));
+#pragma unroll
+                for (int q = 0; q < NS; ++q) st[3 + d * NS + q] = es_eval<float>((float)(l0 + q - t), a.beta, a.cfac);
+            }
+            st[0] = __int_as_float(off);
+            st[1] = cj.x;
+            st[2] = cj.y;
+        }
+        __syncthreads();
+        for (int pt = RMW ? 0 : wave; pt < (RMW && wave > 0 ? 0 : np); pt += RMW ? 1 : NW) {
+            const float* st = stage + pt * S;
+            const int off = __float_as_int(st[0]);
+            const float cx = st[1], cy = st[2];
+            // lanes past NS * NS read in-range stage words and store nothing
+            const float wab = st[3 + ia] * st[3 + NS + ib];
+            if (rowlane) {
+                cplx<float>* row = tile + off + rowoff;
+#pragma unroll
+                for (int q = 0; q < NS; ++q) {
+                    const float ww = wab * st[3 + 2 * NS + q];
+                    if (RMW) {
+                        cplx<float> v = row[q];
+                        v.x += ww * cx;
+                        v.y += ww * cy;
+                        row[q] = v;
+                    } else {
+                        atomicAdd(&row[q].x, ww * cx);
+                        atomicAdd(&row[q].y, ww * cy);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    const int cells = T[0] * T[1] * T[2];
+    for (int i = tid; i < cells; i += 64 * NW) {
+        const int i2 = i % T[2];
+        const int i1 = (i / T[2]) % T[1];
+        const int i0 = i / (T[2] * T[1]);
+        const cplx<float> v = tile[i0 * pp + i1 * rp + i2];
+        if (v.x == 0.f && v.y == 0.f) continue;
+        const int64_t g0 = wrapi(o[0] + i0, pa.nf[0]);
+        const int64_t g1 = wrapi(o[1] + i1, pa.nf[1]);
+        const int64_t g2 = wrapi(o[2] + i2, pa.nf[2]);
+        cplx<float>* dst = a.fine + (g0 * pa.nf[1] + g1) * pa.nf[2] + g2;
         atomicAdd(&dst->x, v.x);
         atomicAdd(&dst->y, v.y);
     }
@@ -335,6 +481,26 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(SpreadArgs<R> a)
 
 template <typename R, int NS, int DIM> void launch_spread_ns(const SpreadArgs<R>& a, int64_t nsub, size_t smem, hipStream_t s)
 {
+    if constexpr (DIM == 3 && std::is_same<R, float>::value && NS * NS <= 64) {
+        if (g_spread_variant == 1) {
+            allow_lds(spread_rows_kernel<NS, 1, true>, smem);
+            KernelTimer t("nufft_spread", s);
+            hipLaunchKernelGGL((spread_rows_kernel<NS, 1, true>), dim3((unsigned)nsub), dim3(64), smem, s, a);
+            return;
+        }
+        if (g_spread_variant == 2) {
+            allow_lds(spread_rows_kernel<NS, 4, false>, smem);
+            KernelTimer t("nufft_spread", s);
+            hipLaunchKernelGGL((spread_rows_kernel<NS, 4, false>), dim3((unsigned)nsub), dim3(256), smem, s, a);
+            return;
+        }
+        if (g_spread_variant == 3) {
+            allow_lds(spread_rows_kernel<NS, 4, true>, smem);
+            KernelTimer t("nufft_spread", s);
+            hipLaunchKernelGGL((spread_rows_kernel<NS, 4, true>), dim3((unsigned)nsub), dim3(256), smem, s, a);
+            return;
+        }
+    }
     allow_lds(spread_kernel<R, NS, DIM>, smem);
     KernelTimer t("nufft_spread", s);
     hipLaunchKernelGGL((spread_kernel<R, NS, DIM>), dim3((unsigned)nsub), dim3(kThreads), smem, s, a);
@@ -462,6 +628,8 @@ template <typename R> PtsArgs<R> pts_args(const NufftPlan* p)
 }
 
 }  // namespace
+
+void nufft_set_spread_variant(int v) { g_spread_variant = v; }
 
 void es_params(double tol, int* ns, double* beta)
 {
@@ -624,6 +792,9 @@ template <typename R> SpreadArgs<R> spread_args(const NufftPlan* p)
     a.cfac = (R)(4.0 / ((double)p->ns * p->ns));
     a.c = nullptr;
     a.cout = nullptr;
+    int T[3];
+    for (int d = 0; d < 3; ++d) T[d] = (d >= 3 - p->dim) ? p->bs[d] + p->ns + 1 : 1;
+    tile_pitches(T, p->ns, (int)sizeof(cplx<R>), a.rp, a.pp);
     return a;
 }
 
@@ -703,7 +874,9 @@ template <typename R> void execute_type1(NufftPlan* p, const cplx<R>* c, cplx<R>
     const int64_t nvox = n0 * n1 * n2;
     int T[3];
     for (int d = 0; d < 3; ++d) T[d] = (d >= 3 - p->dim) ? p->bs[d] + p->ns + 1 : 1;
-    const size_t smem = sizeof(cplx<R>) * (size_t)T[0] * T[1] * T[2];
+    int rp = 0, pp = 0;
+    tile_pitches(T, p->ns, (int)sizeof(cplx<R>), rp, pp);
+    const size_t smem = sizeof(cplx<R>) * (size_t)T[0] * pp;
     HASTY_REQUIRE(smem <= 160 * 1024, "spreading tile exceeds LDS");
     for (int t = 0; t < p->ntransf; ++t) {
         const cplx<R>* ct = c + t * p->M;

@@ -121,6 +121,10 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
  *     (0 LDS Stockham, 1 coset lane-DFT, 2 register radix-16 at 128 x 128 planes and
  *     coset elsewhere, 3 coset with an LDS-DMA inverse), set before preparing kernels;
  *   "llr_max_sweeps" 1..15 (default 15): cap on the Jacobi sweeps of the LLR eigensolver.
+ *   "nufft_spread_variant" 0..3 (default 3): 3-D complex64 type-1 spreader; 0 one lane per
+ *     point with LDS atomics, 1 window rows with one wave per tile, 2 window rows with four
+ *     waves and LDS atomics, 3 four waves clearing and flushing the tile while one wave
+ *     accumulates without atomics.
  * Returns 0, or 1 for an unknown name or a value out of range. */
 int hasty_set_option(const char* name, int64_t value);
 

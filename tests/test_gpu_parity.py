@@ -118,6 +118,39 @@ def test_nufft_cfg1_sizes_vs_oracle(N):
     assert e2 < 1e-5 and e1 < 1e-5
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("traj", ["kooshball", "uniform", "pileup"])
+@pytest.mark.parametrize("tol,bound", [(1e-6, 1e-5), (1e-3, 1e-2)])
+def test_nufft_type1_spread_variants(N, variant, traj, tol, bound):
+    # Every 3-D complex64 spreader against the FP64 direct sum. "pileup" puts 1500 points
+    # on one position plus 500 in its tile, so the tile splits into several chunks whose
+    # points all update the same cells; the kooshball has every spoke's k = 0 sample at
+    # the origin.
+    from hastycompute_amd import _capi
+
+    rng = np.random.default_rng(11)
+    n = 24
+    if traj == "kooshball":
+        co = O.kooshball(300, 24)
+    elif traj == "uniform":
+        co = rng.uniform(-np.pi, np.pi, (3, 6000))
+    else:
+        p0 = rng.uniform(-np.pi, np.pi, (3, 1))
+        co = np.concatenate([np.repeat(p0, 1500, axis=1), p0 + rng.uniform(-0.3, 0.3, (3, 500)),
+                             rng.uniform(-np.pi, np.pi, (3, 2000))], axis=1)
+    co32 = co.astype(np.float32).astype(np.float64)
+    y = rnd(rng, co.shape[1])
+    print(f"spread variant {variant} {traj} tol {tol}", flush=True)
+    _capi.call("hasty_set_option", b"nufft_spread_variant", variant)
+    try:
+        img = h(N.nufft(g(co, torch.float32), g(y[None]), [n, n, n], 1, +1, tol))[0]
+    finally:
+        _capi.call("hasty_set_option", b"nufft_spread_variant", 3)
+    e = O.rel_l2(img, O.nufft_type1(co32, y, (n, n, n), +1, 1e-12))
+    print(f"  rel err {e:.2e}", flush=True)
+    assert e < bound
+
+
 @pytest.mark.parametrize("nm", [[1, 8, 6, 10], [2, 6, 6, 6], [1, 12, 9], [3, 20]])
 def test_toeplitz_diag_and_apply_modes(N, nm):
     rng = np.random.default_rng(sum(nm))
