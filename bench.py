@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--generic", action="store_true", help="use the generic five-pass path")
     ap.add_argument("--xy-variant", type=int, default=None,
                     help="plane kernels of the fused path (library option sense_xy_variant)")
+    ap.add_argument("--zconv-batch", type=int, default=None,
+                    help="problems per z-convolution launch at N0 = 128 (library option sense_zconv_batch, 1..16)")
     ap.add_argument("--cg-iters", type=int, default=15)
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--plain", type=int, default=3, help="problems solved by the plain per-frame path (0: skip)")
@@ -402,6 +404,9 @@ def main():
     if args.xy_variant is not None:
         from hastycompute_amd import _capi
         _capi.call("hasty_set_option", b"sense_xy_variant", args.xy_variant)
+    if args.zconv_batch is not None:
+        from hastycompute_amd import _capi
+        _capi.call("hasty_set_option", b"sense_zconv_batch", args.zconv_batch)
     K, smaps, x, kidx, npts, spokes, offsets = make_workload(args, torch, native)
     P = x.shape[0]
     op = native.SenseNormalToeplitz(smaps, K, coil_batch=args.coil_batch, problem_batch=args.problem_batch,

@@ -314,6 +314,10 @@ int hasty_set_option(const char* name, int64_t value)
         nufft_set_spread_variant((int)value);
         return 0;
     }
+    if (name && std::strcmp(name, "sense_zconv_batch") == 0 && value >= 1 && value <= 16) {
+        sense_fused_set_zconv_batch((int)value);
+        return 0;
+    }
     if (name && std::strcmp(name, "llr_max_sweeps") == 0 && value >= 1 && value <= 15) {
         llr_set_max_sweeps((int)value);
         return 0;

@@ -7,11 +7,11 @@
 // ceil(-log10(tol/10)) fine-grid cells wide, fine grid 2x oversampled.
 //
 // setpts bins points into tiles of the fine grid with a device counting sort; the
-// spreader gives each (tile, chunk of <= kMaxSub points) one workgroup that
+// spreader gives each (tile, chunk of <= chunk_cap points) one workgroup that
 // accumulates into an LDS copy of the tile plus its halo with LDS atomics and then
 // flushes the tile to HBM with one global atomic per non-zero cell. In 3-D each lane
-// walks the ns x ns rows of its point's window from its own starting row, and the LDS
-// tile's row and plane pitches are padded so that those rows fall into different
+// walks the ns planes of its point's window from its own starting plane, and the LDS
+// tile's row and plane pitches are padded so that those planes fall into different
 // banks: points at one position (every spoke of a radial trajectory crosses k = 0)
 // then update different words at any instant instead of serialising on one. Interpolation
 // walks points in tile order so the w^3 gathers hit L2.
@@ -27,7 +27,10 @@ namespace hasty_hip {
 
 namespace {
 
-constexpr int kMaxSub = 256;
+// points per spreader workgroup: complex64 chunks stay small so that a crowded tile
+// (the k-space centre of a radial trajectory) spreads over many CUs; complex128 tiles
+// (20^3 cells, one workgroup per CU) are costlier to clear and flush, so they take more
+inline int chunk_cap(bool fp64) { return fp64 ? 1024 : 256; }
 int g_spread_variant = 3;  // 3-D complex64 spreader, values as nufft_set_spread_variant
 constexpr int kThreads = 256;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
@@ -49,6 +52,17 @@ __device__ __forceinline__ int64_t wrapi(int64_t i, int64_t n)
     while (i < 0) i += n;
     while (i >= n) i -= n;
     return i;
+}
+
+// Tile cell i = (i0 * T1 + i1) * T2 + i2 without integer division: (i + 1/2) / T lies at
+// least 1/(2T) from an integer, far above the float rounding of the product for the
+// cell counts of a tile (i < 2^20), so the truncated product is the exact quotient.
+__device__ __forceinline__ void split_cell(int i, int T1, int T2, float r1, float r2, int& i0, int& i1, int& i2)
+{
+    const int q = (int)(((float)i + 0.5f) * r2);
+    i2 = i - q * T2;
+    i0 = (int)(((float)q + 0.5f) * r1);
+    i1 = q - i0 * T1;
 }
 
 template <typename R> __device__ __forceinline__ R es_eval(R z, R beta, R c)
@@ -102,13 +116,14 @@ __global__ void fill_int_kernel(int* p, int64_t n, int v)
     if (i < n) p[i] = v;
 }
 
-__global__ void subcount_kernel(const int* counts, int* nsub, int64_t nbins)
+__global__ void subcount_kernel(const int* counts, int* nsub, int64_t nbins, int cap)
 {
     const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (b < nbins) nsub[b] = (counts[b] + kMaxSub - 1) / kMaxSub;
+    if (b < nbins) nsub[b] = (counts[b] + cap - 1) / cap;
 }
 
-__global__ void subfill_kernel(const int* counts, const int* nsubstart, int* subbin, int* suboff, int64_t nbins)
+__global__ void subfill_kernel(const int* counts, const int* nsubstart, int* subbin, int* suboff, int64_t nbins,
+                               int cap)
 {
     const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (b >= nbins) return;
@@ -116,7 +131,7 @@ __global__ void subfill_kernel(const int* counts, const int* nsubstart, int* sub
     const int ns = nsubstart[b + 1] - s0;
     for (int s = 0; s < ns; ++s) {
         subbin[s0 + s] = (int)b;
-        suboff[s0 + s] = s * kMaxSub;
+        suboff[s0 + s] = s * cap;
     }
 }
 
@@ -165,14 +180,20 @@ template <typename R> struct SpreadArgs {
     R cfac;  // 4 / ns^2
     int rp;  // LDS tile pitches (complex elements) between rows (last axis) and planes
     int pp;
+    int maxsub;  // points per subproblem at most
 };
 
 // LDS tile pitches for tile extents T and kernel width ns. With e complex elements
 // spanning the 32 banks, the row pitch rp is odd modulo e and the plane pitch is
 // ns * rp modulo e, so row k = ia * ns + ib of a point's window starts at bank group
-// rp * k mod e and consecutive rows cycle through all e groups.
+// rp * k mod e and consecutive rows cycle through all e groups (complex64 tiles).
 inline void tile_pitches(const int T[3], int ns, int esize, int& rp, int& pp)
 {
+    if (esize != 8) {  // complex128 tiles keep the natural layout (measured faster)
+        rp = T[2];
+        pp = T[1] * T[2];
+        return;
+    }
     const int e = 128 / esize;
     rp = T[2] + ((T[2] % e) % 2 == 0 ? 1 : 0);
     const int plane = T[1] * rp;
@@ -188,7 +209,7 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(SpreadArgs<R> a)
     const int sp = blockIdx.x;
     const int bin = a.subbin[sp];
     const int start = a.binstart[bin] + a.suboff[sp];
-    const int end = min(a.binstart[bin + 1], start + kMaxSub);
+    const int end = min(a.binstart[bin + 1], start + a.maxsub);
 
     int64_t bb[3];
     {
@@ -216,7 +237,7 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(SpreadArgs<R> a)
         const cplx<R> cj = a.c[j];
         R w[2][NS];
         int rel[3] = {0, 0, 0};
-        double t0 = 0.0, l00 = 0.0, t1 = 0.0, l01 = 0.0;
+        double t0 = 0.0, l00 = 0.0;
 #pragma unroll
         for (int d = 3 - DIM; d < 3; ++d) {
             const double t = fine_pos((double)pa.pts[d][j], pa.nf[d]);
@@ -225,37 +246,30 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(SpreadArgs<R> a)
             if (d == 0) {
                 t0 = t;
                 l00 = l0;
-            } else if (DIM == 3 && d == 1) {
-                t1 = t;
-                l01 = l0;
             } else {
 #pragma unroll
                 for (int q = 0; q < NS; ++q) w[d - 1][q] = es_eval<R>((R)(l0 + q - t), a.beta, a.cfac);
             }
         }
         if (DIM == 3) {
-            // rows (ia, ib) of the window in lane-rotated order
-            int ia, ib;
-            {
-                const int r0 = (int)(threadIdx.x & 63) % (NS * NS);
-                ia = r0 / NS;
-                ib = r0 - ia * NS;
-            }
+            // planes ia of the window in lane-rotated order (complex64; complex128 tiles
+            // measured faster in natural order)
+            int ia = std::is_same<R, float>::value ? (int)(threadIdx.x & 63) % NS : 0;
             cplx<R>* base = tile + rel[0] * pp + rel[1] * rp + rel[2];
-            for (int k = 0; k < NS * NS; ++k) {
-                const R wab = es_eval<R>((R)(l00 + ia - t0), a.beta, a.cfac) *
-                              es_eval<R>((R)(l01 + ib - t1), a.beta, a.cfac);
-                cplx<R>* row = base + ia * pp + ib * rp;
+            for (int k = 0; k < NS; ++k) {
+                const R wa = es_eval<R>((R)(l00 + ia - t0), a.beta, a.cfac);
 #pragma unroll
-                for (int q = 0; q < NS; ++q) {
-                    const R ww = wab * w[1][q];
-                    atomicAdd(&row[q].x, ww * cj.x);
-                    atomicAdd(&row[q].y, ww * cj.y);
+                for (int ib = 0; ib < NS; ++ib) {
+                    const R wab = wa * w[0][ib];
+                    cplx<R>* row = base + ia * pp + ib * rp;
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) {
+                        const R ww = wab * w[1][q];
+                        atomicAdd(&row[q].x, ww * cj.x);
+                        atomicAdd(&row[q].y, ww * cj.y);
+                    }
                 }
-                if (++ib == NS) {
-                    ib = 0;
-                    if (++ia == NS) ia = 0;
-                }
+                if (++ia == NS) ia = 0;
             }
         } else if (DIM == 2) {
 #pragma unroll
@@ -280,10 +294,10 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(SpreadArgs<R> a)
     }
     __syncthreads();
 
+    const float r1 = 1.0f / (float)T[1], r2 = 1.0f / (float)T[2];
     for (int i = threadIdx.x; i < cells; i += kThreads) {
-        const int i2 = i % T[2];
-        const int i1 = (i / T[2]) % T[1];
-        const int i0 = i / (T[2] * T[1]);
+        int i0, i1, i2;
+        split_cell(i, T[1], T[2], r1, r2, i0, i1, i2);
         const cplx<R> v = tile[i0 * pp + i1 * rp + i2];
         if (v.x == (R)0 && v.y == (R)0) continue;
         const int64_t g0 = wrapi(o[0] + i0, pa.nf[0]);
@@ -317,7 +331,7 @@ __global__ void __launch_bounds__(64 * NW) spread_rows_kernel(SpreadArgs<float> 
     const int sp = blockIdx.x;
     const int bin = a.subbin[sp];
     const int start = a.binstart[bin] + a.suboff[sp];
-    const int end = min(a.binstart[bin + 1], start + kMaxSub);
+    const int end = min(a.binstart[bin + 1], start + a.maxsub);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -393,10 +407,10 @@ __global__ void __launch_bounds__(64 * NW) spread_rows_kernel(SpreadArgs<float> 
     __syncthreads();
 
     const int cells = T[0] * T[1] * T[2];
+    const float r1 = 1.0f / (float)T[1], r2 = 1.0f / (float)T[2];
     for (int i = tid; i < cells; i += 64 * NW) {
-        const int i2 = i % T[2];
-        const int i1 = (i / T[2]) % T[1];
-        const int i0 = i / (T[2] * T[1]);
+        int i0, i1, i2;
+        split_cell(i, T[1], T[2], r1, r2, i0, i1, i2);
         const cplx<float> v = tile[i0 * pp + i1 * rp + i2];
         if (v.x == 0.f && v.y == 0.f) continue;
         const int64_t g0 = wrapi(o[0] + i0, pa.nf[0]);
@@ -751,11 +765,11 @@ template <typename R> void setpts_typed(NufftPlan* p, hipStream_t s)
                            (const int*)p->binstart, (const int*)p->rank, p->sortidx);
         check_launch("bin_scatter_kernel");
     }
-    // Spreader subproblems: ceil(count / kMaxSub) per tile. The per-tile counts go into
+    // Spreader subproblems: ceil(count / chunk_cap) per tile. The per-tile counts go into
     // `rank` (free again after the scatter) to avoid another buffer.
     int* nsub = p->rank;
     hipLaunchKernelGGL(subcount_kernel, dim3((unsigned)ceil_div(nb, kThreads)), dim3(kThreads), 0, s,
-                       (const int*)p->counts, nsub, nb);
+                       (const int*)p->counts, nsub, nb, chunk_cap(p->fp64 != 0));
     check_launch("subcount_kernel");
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, (const int*)nsub, p->nsubbin, nb);
     check_launch("scan_kernel");
@@ -774,7 +788,8 @@ template <typename R> void setpts_typed(NufftPlan* p, hipStream_t s)
     }
     if (total > 0) {
         hipLaunchKernelGGL(subfill_kernel, dim3((unsigned)ceil_div(nb, kThreads)), dim3(kThreads), 0, s,
-                           (const int*)p->counts, (const int*)p->nsubbin, p->subbin, p->suboff, nb);
+                           (const int*)p->counts, (const int*)p->nsubbin, p->subbin, p->suboff, nb,
+                           chunk_cap(p->fp64 != 0));
         check_launch("subfill_kernel");
     }
 }
@@ -795,6 +810,7 @@ template <typename R> SpreadArgs<R> spread_args(const NufftPlan* p)
     int T[3];
     for (int d = 0; d < 3; ++d) T[d] = (d >= 3 - p->dim) ? p->bs[d] + p->ns + 1 : 1;
     tile_pitches(T, p->ns, (int)sizeof(cplx<R>), a.rp, a.pp);
+    a.maxsub = chunk_cap(p->fp64 != 0);
     return a;
 }
 

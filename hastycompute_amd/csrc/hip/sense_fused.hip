@@ -205,14 +205,29 @@ template <int S> __device__ __forceinline__ void dft4(float2* v)
     v[1] = cadd(t1, t3);
     v[3] = csub(t1, t3);
 }
+// dft4 of (v[0], v[1], 0, 0): the first stage of a DFT whose upper half of inputs is zero
+template <int S> __device__ __forceinline__ void dft4_half(float2* v)
+{
+    const float2 a = v[0], b = v[1], mb = mis<S>(v[1]);
+    v[0] = cadd(a, b);
+    v[2] = csub(a, b);
+    v[1] = cadd(a, mb);
+    v[3] = csub(a, mb);
+}
+// with HALF the upper half of a DFT's inputs is zero and is not read
+template <int S, bool HALF> __device__ __forceinline__ void dft4_first(float2* v)
+{
+    if constexpr (HALF) dft4_half<S>(v);
+    else dft4<S>(v);
+}
 template <> __device__ __forceinline__ void dftR<4, -1>(float2* v) { dft4<-1>(v); }
 template <> __device__ __forceinline__ void dftR<4, 1>(float2* v) { dft4<1>(v); }
-template <int S> __device__ __forceinline__ void dft8(float2* v)
+template <int S, bool HALF = false> __device__ __forceinline__ void dft8(float2* v)
 {
-    float2 e[4] = {v[0], v[2], v[4], v[6]};
-    float2 o[4] = {v[1], v[3], v[5], v[7]};
-    dft4<S>(e);
-    dft4<S>(o);
+    float2 e[4] = {v[0], v[2], HALF ? float2{0.f, 0.f} : v[4], HALF ? float2{0.f, 0.f} : v[6]};
+    float2 o[4] = {v[1], v[3], HALF ? float2{0.f, 0.f} : v[5], HALF ? float2{0.f, 0.f} : v[7]};
+    dft4_first<S, HALF>(e);
+    dft4_first<S, HALF>(o);
     const float h = 0.70710678118654752440f;
     float2 w1 = float2{h * (o[1].x - (float)S * o[1].y), h * (o[1].y + (float)S * o[1].x)};
     float2 w2 = mis<S>(o[2]);
@@ -686,7 +701,7 @@ __global__ void __launch_bounds__(kZT) z_conv_ct_kernel(FusedArgs a, int LL, int
 
 // 16-point DFT in registers: n = 4a + b, k = c + 4d,
 // X[c + 4d] = sum_b w16^{bc} w4^{bd} sum_a v[4a + b] w4^{ac}.
-template <int S> __device__ __forceinline__ void dft16(float2* v)
+template <int S, bool HALF = false> __device__ __forceinline__ void dft16(float2* v)
 {
     // w16^m = exp(S 2 pi i m / 16), m = 0..9
     const float C[10] = {1.f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f, 0.f,
@@ -698,8 +713,8 @@ template <int S> __device__ __forceinline__ void dft16(float2* v)
     float2 t[16];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-        float2 q[4] = {v[b], v[4 + b], v[8 + b], v[12 + b]};
-        dft4<S>(q);
+        float2 q[4] = {v[b], v[4 + b], HALF ? float2{0.f, 0.f} : v[8 + b], HALF ? float2{0.f, 0.f} : v[12 + b]};
+        dft4_first<S, HALF>(q);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int m = b * c;
@@ -1213,7 +1228,7 @@ __global__ void __launch_bounds__(kQT) fwd_xy_r16_kernel(FusedArgs a)
             if (i < 3) load_row(sr, i + 1);
 #pragma unroll
             for (int x2 = 8; x2 < 16; ++x2) u[x2] = float2{0.f, 0.f};
-            dft16<-1>(u);
+            dft16<-1, true>(u);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int k1 = 1; k1 < 16; ++k1) u[k1] = cmul(u[k1], tq[q * 17 + k1]);
@@ -1236,7 +1251,7 @@ __global__ void __launch_bounds__(kQT) fwd_xy_r16_kernel(FusedArgs a)
                 for (int i = 0; i < 4; ++i) u[i] = X[i][4 * h + c];
 #pragma unroll
                 for (int i = 4; i < 8; ++i) u[i] = float2{0.f, 0.f};
-                dft8<-1>(u);
+                dft8<-1, true>(u);
 #pragma unroll
                 for (int ka = 0; ka < 8; ++ka)
                     yb[g * kYS + ka * 64 + 16 * c + q] = ka ? cmul(u[ka], tgt[g * 8 + ka]) : u[0];
@@ -1350,24 +1365,24 @@ __global__ void __launch_bounds__(kQT) inv_xy_r16_kernel(FusedArgs a)
         for (int i = 0; i < 4; ++i) {
             // inverse DFT over k2 -> x1 (thread k1 = q), transpose, twiddle w256^{-x1 k1},
             // inverse DFT over k1 -> x2 (thread x1 = q), pixels x2 < 8 kept
-            float2 sn[8];
-            if (i < 3) load_srow(sn, i + 1);
+            // per-thread LDS offsets, opaque here so their address forms are built per row
+            // instead of being held in registers across the column phase
+            int xo = g * kXS, qo = q * 17;
+            asm volatile("" : "+v"(xo), "+v"(qo));
             dft16<1>(X[i]);
             wave_lds_sync();
 #pragma unroll
-            for (int x1 = 0; x1 < 16; ++x1) xs[x1 * 17 + q] = X[i][x1];
+            for (int x1 = 0; x1 < 16; ++x1) yb[xo + x1 * 17 + q] = X[i][x1];
             wave_lds_sync();
             float2 u[16];
-            u[0] = xs[q * 17];
+            u[0] = yb[xo + qo];
 #pragma unroll
-            for (int k1 = 1; k1 < 16; ++k1) u[k1] = cmulc(tq[q * 17 + k1], xs[q * 17 + k1]);
+            for (int k1 = 1; k1 < 16; ++k1) u[k1] = cmulc(tq[qo + k1], yb[xo + qo + k1]);
             dft16<1>(u);
 #pragma unroll
             for (int x2 = 0; x2 < 8; ++x2) acc[i][x2] = cfma_conj(acc[i][x2], sv[x2], u[x2]);
-            if (i < 3) {
-#pragma unroll
-                for (int x2 = 0; x2 < 8; ++x2) sv[x2] = sn[x2];
-            }
+            // the next row's maps load during that row's transforms
+            if (i < 3) load_srow(sv, i + 1);
             __builtin_amdgcn_sched_barrier(0);
         }
         phase_barrier();
@@ -1395,8 +1410,16 @@ constexpr int kR16T = 256;  // threads
 constexpr int kR16S1 = 17;  // LDS stride of the inner index (padded)
 constexpr int kR16S2 = 280; // LDS stride of the outer index (padded)
 
-__global__ void __launch_bounds__(kR16T) z_conv_r16_kernel(FusedArgs a, int LL, int pl, const float* __restrict__ K)
+// Toeplitz kernels of the problems of one launch (problem pl0 + blockIdx.y uses k[blockIdx.y])
+constexpr int kZMaxProblems = 16;
+struct ZKernels {
+    const float* k[kZMaxProblems];
+};
+
+__global__ void __launch_bounds__(kR16T) z_conv_r16_kernel(FusedArgs a, int LL, int pl0, ZKernels zk)
 {
+    const int pl = pl0 + (int)blockIdx.y;
+    const float* __restrict__ K = zk.k[blockIdx.y];
     __shared__ float2 lds[16 * kR16S2];
     const int t = threadIdx.x;
     const int c = t & 15;
@@ -1435,8 +1458,8 @@ __global__ void __launch_bounds__(kR16T) z_conv_r16_kernel(FusedArgs a, int LL, 
 #pragma unroll
             for (int n1 = 0; n1 < 8; ++n1) pf[n1] = bload<true>(rn, voff, n1 * zstep);
         }
-        // forward, step A (n2 = g): DFT over n1, twiddle w256^{g k1}, transpose
-        dft16<-1>(v);
+        // forward, step A (n2 = g): DFT over n1 (n1 < 8 nonzero), twiddle w256^{g k1}, transpose
+        dft16<-1, true>(v);
 #pragma unroll
         for (int k1 = 1; k1 < 16; ++k1) v[k1] = cmul(v[k1], tw[k1]);
 #pragma unroll
@@ -1509,11 +1532,13 @@ namespace {
 // with the inverse fed by direct-to-LDS loads. The kernel permutation follows the
 // variant, so it is set before kernels are prepared.
 int g_xy_variant = 2;
+int g_zconv_batch = 16;  // problems per z-convolution launch (N0 = 128), 1..16
 
 bool use_r16(int N) { return g_xy_variant == 2 && N == 128; }
 }  // namespace
 
 void sense_fused_set_xy_variant(int v) { g_xy_variant = v; }
+void sense_fused_set_zconv_batch(int v) { g_zconv_batch = v; }
 
 bool sense_fused_supported(const int64_t* n3)
 {
@@ -1606,13 +1631,23 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
                 hipLaunchKernelGGL(fwd, dim3(N0, np), dim3(nxy), smem_xy, s, a);
                 check_launch("fwd_xy_kernel");
             }
-            for (int pl = 0; pl < np; ++pl) {
+            if (L0 == 256) {
+                // one launch per up to g_zconv_batch problems of the chunk
+                for (int q0 = 0; q0 < np; q0 += g_zconv_batch) {
+                    const int nq = std::min(g_zconv_batch, np - q0);
+                    ZKernels zk{};
+                    for (int j = 0; j < nq; ++j)
+                        zk.k[j] = kernels + (int64_t)kidx[p0 + q0 + j] * L0 * (int64_t)L * L;
+                    KernelTimer t("sense_z_conv", s);
+                    hipLaunchKernelGGL(z_conv_r16_kernel, dim3((unsigned)(L * L / kR16C), (unsigned)nq), dim3(kR16T), 0,
+                                       s, a, L * L, q0, zk);
+                    check_launch("z_conv_kernel");
+                }
+            }
+            for (int pl = 0; pl < np && L0 != 256; ++pl) {
                 const float* K = kernels + (int64_t)kidx[p0 + pl] * L0 * (int64_t)L * L;
                 KernelTimer t("sense_z_conv", s);
-                if (L0 == 256) {
-                    hipLaunchKernelGGL(z_conv_r16_kernel, dim3((unsigned)(L * L / kR16C)), dim3(kR16T), 0, s, a,
-                                       L * L, pl, K);
-                } else if (zct) {
+                if (zct) {
                     hipLaunchKernelGGL(zct, dim3((unsigned)(L * L / kZC)), dim3(kZT), smem_zct, s, a, L * L, pl, K);
                 } else {
                     hipLaunchKernelGGL(z_conv_kernel, dim3((unsigned)(L * L / kW)), dim3(kZ), smem_z, s, a, L * L, pl,

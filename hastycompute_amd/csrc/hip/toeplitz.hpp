@@ -61,6 +61,8 @@ void circulant_precond_apply(const float2* in, float2* out, const float2* diag, 
 bool sense_fused_supported(const int64_t* n3);
 // 1 (default): plane kernels with register four-step y transforms; 0: LDS Stockham passes.
 void sense_fused_set_xy_variant(int v);
+// problems per z-convolution launch of the fused operator at N0 = 128 (1..16)
+void sense_fused_set_zconv_batch(int v);
 void sense_fused_permute_kernel(const float* natural, float* permuted, const int64_t* n3, hipStream_t s);
 size_t sense_fused_workspace_bytes(const int64_t* n3, int pchunk, int ccount);
 void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const int* coils_dev, int ncoil,

@@ -120,6 +120,8 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
  *   "sense_xy_variant" 0..3 (default 2): plane-kernel family of the fused normal operator
  *     (0 LDS Stockham, 1 coset lane-DFT, 2 register radix-16 at 128 x 128 planes and
  *     coset elsewhere, 3 coset with an LDS-DMA inverse), set before preparing kernels;
+ *   "sense_zconv_batch" 1..16 (default 16): problems per z-convolution launch of the fused
+ *     normal operator at N0 = 128.
  *   "llr_max_sweeps" 1..15 (default 15): cap on the Jacobi sweeps of the LLR eigensolver.
  *   "nufft_spread_variant" 0..3 (default 3): 3-D complex64 type-1 spreader; 0 one lane per
  *     point with LDS atomics, 1 window rows with one wave per tile, 2 window rows with four
