@@ -31,7 +31,7 @@ namespace {
 // (the k-space centre of a radial trajectory) spreads over many CUs; complex128 tiles
 // (20^3 cells, one workgroup per CU) are costlier to clear and flush, so they take more
 inline int chunk_cap(bool fp64) { return fp64 ? 1024 : 256; }
-int g_spread_variant = 3;  // 3-D complex64 spreader, values as nufft_set_spread_variant
+int g_spread_variant = 3;  // 3-D spreader, values as nufft_set_spread_variant
 constexpr int kThreads = 256;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
@@ -186,10 +186,12 @@ template <typename R> struct SpreadArgs {
 // LDS tile pitches for tile extents T and kernel width ns. With e complex elements
 // spanning the 32 banks, the row pitch rp is odd modulo e and the plane pitch is
 // ns * rp modulo e, so row k = ia * ns + ib of a point's window starts at bank group
-// rp * k mod e and consecutive rows cycle through all e groups (complex64 tiles).
-inline void tile_pitches(const int T[3], int ns, int esize, int& rp, int& pp)
+// rp * k mod e and consecutive rows cycle through all e groups. Complex128 tiles are
+// padded only for the window-row spreader (the lane-per-point one measured faster on
+// the natural layout).
+inline void tile_pitches(const int T[3], int ns, int esize, bool pad, int& rp, int& pp)
 {
-    if (esize != 8) {  // complex128 tiles keep the natural layout (measured faster)
+    if (!pad) {
         rp = T[2];
         pp = T[1] * T[2];
         return;
@@ -309,7 +311,7 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(SpreadArgs<R> a)
     }
 }
 
-// 3-D spreading by window rows (complex64 tiles). A batch of kRowBatch points first
+// 3-D spreading by window rows. A batch of kRowBatch points first
 // stages its base offset, strength and 3 x NS kernel weights in LDS (one lane per
 // point); then the NW waves of the group take the batch's points in turn, one point at a
 // time per wave, lane ia * NS + ib updating the NS cells of window row (ia, ib). The
@@ -320,14 +322,15 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(SpreadArgs<R> a)
 // through LDS atomics.
 constexpr int kRowBatch = 32;
 
-template <int NS, int NW, bool RMW>
-__global__ void __launch_bounds__(64 * NW) spread_rows_kernel(SpreadArgs<float> a)
+template <typename R, int NS, int NW, bool RMW>
+__global__ void __launch_bounds__(64 * NW) spread_rows_kernel(SpreadArgs<R> a)
 {
-    constexpr int S = 3 * NS + 3;  // staged words per point
-    __shared__ float stage[kRowBatch * S];
+    constexpr int S = 3 * NS + 2;  // staged values per point: strength, 3 x NS weights
+    __shared__ R stage[kRowBatch * S];
+    __shared__ int stage_off[kRowBatch];
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    cplx<float>* tile = reinterpret_cast<cplx<float>*>(smem_raw);
-    const PtsArgs<float>& pa = a.pa;
+    cplx<R>* tile = reinterpret_cast<cplx<R>*>(smem_raw);
+    const PtsArgs<R>& pa = a.pa;
     const int sp = blockIdx.x;
     const int bin = a.subbin[sp];
     const int start = a.binstart[bin] + a.suboff[sp];
@@ -352,19 +355,15 @@ __global__ void __launch_bounds__(64 * NW) spread_rows_kernel(SpreadArgs<float> 
         o[d] = bb[d] * pa.bs[d] - (NS + 1) / 2;
     }
     const int rp = a.rp, pp = a.pp;
-    for (int i = tid; i < T[0] * pp; i += 64 * NW) tile[i] = cplx<float>{0.f, 0.f};
+    for (int i = tid; i < T[0] * pp; i += 64 * NW) tile[i] = cplx<R>{(R)0, (R)0};
 
-    const int ia = lane / NS;
-    const int ib = lane - ia * NS;
-    const bool rowlane = lane < NS * NS;
-    const int rowoff = ia * pp + ib * rp;
     for (int b0 = start; b0 < end; b0 += kRowBatch) {
         const int np = min(kRowBatch, end - b0);
         __syncthreads();
         if (tid < np) {
             const int j = a.sortidx[b0 + tid];
-            const cplx<float> cj = a.c[j];
-            float* st = stage + tid * S;
+            const cplx<R> cj = a.c[j];
+            R* st = stage + tid * S;
             int off = 0;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
@@ -373,32 +372,38 @@ __global__ void __launch_bounds__(64 * NW) spread_rows_kernel(SpreadArgs<float> 
                 const int rel = (int)((int64_t)l0 - o[d]);
                 off += rel * (d == 0 ? pp : (d == 1 ? rp : 1));
 #pragma unroll
-                for (int q = 0; q < NS; ++q) st[3 + d * NS + q] = es_eval<float>((float)(l0 + q - t), a.beta, a.cfac);
+                for (int q = 0; q < NS; ++q) st[2 + d * NS + q] = es_eval<R>((R)(l0 + q - t), a.beta, a.cfac);
             }
-            st[0] = __int_as_float(off);
-            st[1] = cj.x;
-            st[2] = cj.y;
+            stage_off[tid] = off;
+            st[0] = cj.x;
+            st[1] = cj.y;
         }
         __syncthreads();
         for (int pt = RMW ? 0 : wave; pt < (RMW && wave > 0 ? 0 : np); pt += RMW ? 1 : NW) {
-            const float* st = stage + pt * S;
-            const int off = __float_as_int(st[0]);
-            const float cx = st[1], cy = st[2];
-            // lanes past NS * NS read in-range stage words and store nothing
-            const float wab = st[3 + ia] * st[3 + NS + ib];
-            if (rowlane) {
-                cplx<float>* row = tile + off + rowoff;
+            const R* st = stage + pt * S;
+            const int off = stage_off[pt];
+            const R cx = st[0], cy = st[1];
+            // lane l updates rows l, l + 64, ... of the NS x NS window rows (ia, ib)
 #pragma unroll
-                for (int q = 0; q < NS; ++q) {
-                    const float ww = wab * st[3 + 2 * NS + q];
-                    if (RMW) {
-                        cplx<float> v = row[q];
-                        v.x += ww * cx;
-                        v.y += ww * cy;
-                        row[q] = v;
-                    } else {
-                        atomicAdd(&row[q].x, ww * cx);
-                        atomicAdd(&row[q].y, ww * cy);
+            for (int r0 = 0; r0 < NS * NS; r0 += 64) {
+                const int r = r0 + lane;
+                if (r < NS * NS) {
+                    const int ia = r / NS;
+                    const int ib = r - ia * NS;
+                    const R wab = st[2 + ia] * st[2 + NS + ib];
+                    cplx<R>* row = tile + off + ia * pp + ib * rp;
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) {
+                        const R ww = wab * st[2 + 2 * NS + q];
+                        if (RMW) {
+                            cplx<R> v = row[q];
+                            v.x += ww * cx;
+                            v.y += ww * cy;
+                            row[q] = v;
+                        } else {
+                            atomicAdd(&row[q].x, ww * cx);
+                            atomicAdd(&row[q].y, ww * cy);
+                        }
                     }
                 }
             }
@@ -411,12 +416,12 @@ __global__ void __launch_bounds__(64 * NW) spread_rows_kernel(SpreadArgs<float> 
     for (int i = tid; i < cells; i += 64 * NW) {
         int i0, i1, i2;
         split_cell(i, T[1], T[2], r1, r2, i0, i1, i2);
-        const cplx<float> v = tile[i0 * pp + i1 * rp + i2];
-        if (v.x == 0.f && v.y == 0.f) continue;
+        const cplx<R> v = tile[i0 * pp + i1 * rp + i2];
+        if (v.x == (R)0 && v.y == (R)0) continue;
         const int64_t g0 = wrapi(o[0] + i0, pa.nf[0]);
         const int64_t g1 = wrapi(o[1] + i1, pa.nf[1]);
         const int64_t g2 = wrapi(o[2] + i2, pa.nf[2]);
-        cplx<float>* dst = a.fine + (g0 * pa.nf[1] + g1) * pa.nf[2] + g2;
+        cplx<R>* dst = a.fine + (g0 * pa.nf[1] + g1) * pa.nf[2] + g2;
         atomicAdd(&dst->x, v.x);
         atomicAdd(&dst->y, v.y);
     }
@@ -495,23 +500,23 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(SpreadArgs<R> a)
 
 template <typename R, int NS, int DIM> void launch_spread_ns(const SpreadArgs<R>& a, int64_t nsub, size_t smem, hipStream_t s)
 {
-    if constexpr (DIM == 3 && std::is_same<R, float>::value && NS * NS <= 64) {
+    if constexpr (DIM == 3) {
         if (g_spread_variant == 1) {
-            allow_lds(spread_rows_kernel<NS, 1, true>, smem);
+            allow_lds(spread_rows_kernel<R, NS, 1, true>, smem);
             KernelTimer t("nufft_spread", s);
-            hipLaunchKernelGGL((spread_rows_kernel<NS, 1, true>), dim3((unsigned)nsub), dim3(64), smem, s, a);
+            hipLaunchKernelGGL((spread_rows_kernel<R, NS, 1, true>), dim3((unsigned)nsub), dim3(64), smem, s, a);
             return;
         }
         if (g_spread_variant == 2) {
-            allow_lds(spread_rows_kernel<NS, 4, false>, smem);
+            allow_lds(spread_rows_kernel<R, NS, 4, false>, smem);
             KernelTimer t("nufft_spread", s);
-            hipLaunchKernelGGL((spread_rows_kernel<NS, 4, false>), dim3((unsigned)nsub), dim3(256), smem, s, a);
+            hipLaunchKernelGGL((spread_rows_kernel<R, NS, 4, false>), dim3((unsigned)nsub), dim3(256), smem, s, a);
             return;
         }
         if (g_spread_variant == 3) {
-            allow_lds(spread_rows_kernel<NS, 4, true>, smem);
+            allow_lds(spread_rows_kernel<R, NS, 4, true>, smem);
             KernelTimer t("nufft_spread", s);
-            hipLaunchKernelGGL((spread_rows_kernel<NS, 4, true>), dim3((unsigned)nsub), dim3(256), smem, s, a);
+            hipLaunchKernelGGL((spread_rows_kernel<R, NS, 4, true>), dim3((unsigned)nsub), dim3(256), smem, s, a);
             return;
         }
     }
@@ -794,6 +799,9 @@ template <typename R> void setpts_typed(NufftPlan* p, hipStream_t s)
     }
 }
 
+// padded LDS tiles: complex64 always, complex128 for the 3-D window-row spreader
+bool pad_tile(const NufftPlan* p) { return !p->fp64 || (p->dim == 3 && g_spread_variant != 0); }
+
 template <typename R> SpreadArgs<R> spread_args(const NufftPlan* p)
 {
     SpreadArgs<R> a;
@@ -809,7 +817,7 @@ template <typename R> SpreadArgs<R> spread_args(const NufftPlan* p)
     a.cout = nullptr;
     int T[3];
     for (int d = 0; d < 3; ++d) T[d] = (d >= 3 - p->dim) ? p->bs[d] + p->ns + 1 : 1;
-    tile_pitches(T, p->ns, (int)sizeof(cplx<R>), a.rp, a.pp);
+    tile_pitches(T, p->ns, (int)sizeof(cplx<R>), pad_tile(p), a.rp, a.pp);
     a.maxsub = chunk_cap(p->fp64 != 0);
     return a;
 }
@@ -891,7 +899,7 @@ template <typename R> void execute_type1(NufftPlan* p, const cplx<R>* c, cplx<R>
     int T[3];
     for (int d = 0; d < 3; ++d) T[d] = (d >= 3 - p->dim) ? p->bs[d] + p->ns + 1 : 1;
     int rp = 0, pp = 0;
-    tile_pitches(T, p->ns, (int)sizeof(cplx<R>), rp, pp);
+    tile_pitches(T, p->ns, (int)sizeof(cplx<R>), pad_tile(p), rp, pp);
     const size_t smem = sizeof(cplx<R>) * (size_t)T[0] * pp;
     HASTY_REQUIRE(smem <= 160 * 1024, "spreading tile exceeds LDS");
     for (int t = 0; t < p->ntransf; ++t) {

@@ -49,7 +49,7 @@ void nufft_setpts(NufftPlan* p, int64_t M, const void* x, const void* y, const v
 // type 1: c [ntransf][M] -> f [ntransf][modes]; type 2: f -> c.
 void nufft_execute(NufftPlan* p, void* c, void* f, hipStream_t s);
 void nufft_destroy(NufftPlan* p);
-// 3-D complex64 spreader: 0 one lane per point with LDS atomics; window rows with
+// 3-D spreader: 0 one lane per point with LDS atomics; window rows with
 // 1 one wave per tile, 2 four waves and LDS atomics, 3 (default) four waves clearing and
 // flushing the tile while the first wave accumulates without atomics
 void nufft_set_spread_variant(int v);

@@ -123,7 +123,7 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
  *   "sense_zconv_batch" 1..16 (default 16): problems per z-convolution launch of the fused
  *     normal operator at N0 = 128.
  *   "llr_max_sweeps" 1..15 (default 15): cap on the Jacobi sweeps of the LLR eigensolver.
- *   "nufft_spread_variant" 0..3 (default 3): 3-D complex64 type-1 spreader; 0 one lane per
+ *   "nufft_spread_variant" 0..3 (default 3): 3-D type-1 spreader; 0 one lane per
  *     point with LDS atomics, 1 window rows with one wave per tile, 2 window rows with four
  *     waves and LDS atomics, 3 four waves clearing and flushing the tile while one wave
  *     accumulates without atomics.
