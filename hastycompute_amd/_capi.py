@@ -44,6 +44,7 @@ _SIGS = {
     "hasty_sense_normal_toeplitz_ex": ([c_vp, c_vp, c_vp, c_i64, c_vp, c_int, c_int, P_i32, c_vp, c_vp, c_int, c_int,
                                         P_i32, c_vp, c_int, P_i64, c_vp, c_sz, c_int, c_vp], c_int),
     "hasty_set_option": ([ctypes.c_char_p, c_i64], c_int),
+    "hasty_get_option": ([ctypes.c_char_p, P_i64], c_int),
     "hasty_timing_enable": ([c_int], c_int),
     "hasty_timing_entries": ([], c_int),
     "hasty_timing_read": ([c_int, ctypes.c_char_p, c_int, P_dbl, P_i64], c_int),
@@ -133,3 +134,13 @@ def stream_ptr(stream=None) -> int:
 
 def call(name: str, *args) -> None:
     check(getattr(lib(), name)(*args))
+
+
+def get_option(name: str) -> int:
+    v = ctypes.c_int64()
+    call("hasty_get_option", name.encode(), ctypes.byref(v))
+    return int(v.value)
+
+
+def set_option(name: str, value: int) -> None:
+    call("hasty_set_option", name.encode(), int(value))

@@ -326,6 +326,24 @@ int hasty_set_option(const char* name, int64_t value)
     return 1;
 }
 
+int hasty_get_option(const char* name, int64_t* value)
+{
+    if (!value) {
+        set_last_error("value must not be NULL");
+        return 1;
+    }
+    if (name && std::strcmp(name, "sense_fused") == 0) *value = g_opt_sense_fused;
+    else if (name && std::strcmp(name, "sense_xy_variant") == 0) *value = sense_fused_get_xy_variant();
+    else if (name && std::strcmp(name, "nufft_spread_variant") == 0) *value = nufft_get_spread_variant();
+    else if (name && std::strcmp(name, "sense_zconv_batch") == 0) *value = sense_fused_get_zconv_batch();
+    else if (name && std::strcmp(name, "llr_max_sweeps") == 0) *value = llr_get_max_sweeps();
+    else {
+        set_last_error(std::string("unknown option ") + (name ? name : "(null)"));
+        return 1;
+    }
+    return 0;
+}
+
 int hasty_timing_enable(int on)
 {
     return guarded([&] { timing_enable(on != 0); });

@@ -816,6 +816,8 @@ void llr_set_max_sweeps(int sweeps)
     g_max_sweeps = sweeps;
 }
 
+int llr_get_max_sweeps() { return g_max_sweeps; }
+
 void llr_check(int64_t F)
 {
     HASTY_REQUIRE(F >= 1 && F <= 128, "LLR supports 1..128 columns (frames)");

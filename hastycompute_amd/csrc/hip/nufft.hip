@@ -649,6 +649,7 @@ template <typename R> PtsArgs<R> pts_args(const NufftPlan* p)
 }  // namespace
 
 void nufft_set_spread_variant(int v) { g_spread_variant = v; }
+int nufft_get_spread_variant() { return g_spread_variant; }
 
 void es_params(double tol, int* ns, double* beta)
 {

@@ -53,6 +53,7 @@ void nufft_destroy(NufftPlan* p);
 // 1 one wave per tile, 2 four waves and LDS atomics, 3 (default) four waves clearing and
 // flushing the tile while the first wave accumulates without atomics
 void nufft_set_spread_variant(int v);
+int nufft_get_spread_variant();
 
 // ES kernel parameters for upsampling factor 2.
 void es_params(double tol, int* ns, double* beta);

@@ -1539,6 +1539,8 @@ bool use_r16(int N) { return g_xy_variant == 2 && N == 128; }
 
 void sense_fused_set_xy_variant(int v) { g_xy_variant = v; }
 void sense_fused_set_zconv_batch(int v) { g_zconv_batch = v; }
+int sense_fused_get_xy_variant() { return g_xy_variant; }
+int sense_fused_get_zconv_batch() { return g_zconv_batch; }
 
 bool sense_fused_supported(const int64_t* n3)
 {

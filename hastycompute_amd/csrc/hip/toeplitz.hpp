@@ -63,6 +63,8 @@ bool sense_fused_supported(const int64_t* n3);
 void sense_fused_set_xy_variant(int v);
 // problems per z-convolution launch of the fused operator at N0 = 128 (1..16)
 void sense_fused_set_zconv_batch(int v);
+int sense_fused_get_xy_variant();
+int sense_fused_get_zconv_batch();
 void sense_fused_permute_kernel(const float* natural, float* permuted, const int64_t* n3, hipStream_t s);
 size_t sense_fused_workspace_bytes(const int64_t* n3, int pchunk, int ccount);
 void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const int* coils_dev, int ncoil,

@@ -151,30 +151,59 @@ class SenseNormalToeplitz:
         self.coils = torch.tensor(coils, dtype=torch.int32, device=smaps.device)
         self.ncoil = len(coils)
         self.coil_batch = max(1, min(coil_batch, self.ncoil))
-        if fused is not None:
-            C.call("hasty_set_option", b"sense_fused", int(bool(fused)))
+        _require(kernels.dtype == torch.float32, "kernels must be float32")
         self.kernels = torch.empty_like(kernels)
         layout = ctypes.c_int()
+        prev = C.get_option("sense_fused")
+        if fused is not None:
+            C.set_option("sense_fused", int(bool(fused)))
         try:
             C.call("hasty_sense_normal_prepare", kernels.data_ptr(), self.kernels.data_ptr(), kernels.shape[0],
                    C.i64arr(self.n3), ctypes.byref(layout), C.stream_ptr())
         finally:
-            if fused is not None:
-                C.call("hasty_set_option", b"sense_fused", 1)
+            C.set_option("sense_fused", prev)
         self.layout = layout.value
-        ws = C.lib().hasty_sense_normal_workspace_size(C.i64arr(self.n3), self.coil_batch, max(1, problem_batch))
+        self.nvox = self.n3[0] * self.n3[1] * self.n3[2]
+        self.problem_batch = max(1, problem_batch)
+        ws = C.lib().hasty_sense_normal_workspace_size(C.i64arr(self.n3), self.coil_batch, self.problem_batch)
         self.workspace = torch.empty(ws, dtype=torch.uint8, device=smaps.device)
 
+    def _check_image(self, t: torch.Tensor, name: str, P: int):
+        _cuda(t, name)
+        _require(t.dtype == torch.complex64, f"{name} must be complex64")
+        _require(t.numel() == P * self.nvox, f"{name} must hold {P} images of {self.n3}")
+
     def apply(self, x: torch.Tensor, out: Optional[torch.Tensor] = None, kernel_index: Optional[Sequence[int]] = None,
-              sub: Optional[torch.Tensor] = None) -> torch.Tensor:
+              sub: Optional[torch.Tensor] = None, problem_ncoil: Optional[Sequence[int]] = None,
+              coil_problem_stride: int = 0) -> torch.Tensor:
+        """out[p] = sum_{i < ncoil_p} conj(S_{c_i + p s}) T_{k(p)}(S_{c_i + p s} x[p]) - sub[p] with
+        c_i the operator's coil list, s = coil_problem_stride and ncoil_p = problem_ncoil[p]
+        (default: every listed coil); hasty_sense_normal_toeplitz(_ex)."""
         _cuda(x, "x")
-        P = x.numel() // int(torch.tensor(self.n3).prod())
+        _require(x.dtype == torch.complex64, "x must be complex64")
+        _require(x.numel() % self.nvox == 0 and x.numel() > 0, f"x must hold whole images of {self.n3}")
+        P = x.numel() // self.nvox
         if out is None:
             out = torch.empty_like(x)
-        kidx = [0] * P if kernel_index is None else list(kernel_index)
-        C.call("hasty_sense_normal_toeplitz", x.data_ptr(), out.data_ptr(), self.smaps.data_ptr(),
-               self.coils.data_ptr(), self.ncoil, self.kernels.data_ptr(), self.kernels.shape[0], self.layout,
-               C.i32arr(kidx), C.ptr(sub), P, C.i64arr(self.n3), self.workspace.data_ptr(), self.workspace.numel(), self.coil_batch,
+        self._check_image(out, "out", P)
+        if sub is not None:
+            self._check_image(sub, "sub", P)
+        kidx = [0] * P if kernel_index is None else [int(k) for k in kernel_index]
+        _require(len(kidx) == P and all(0 <= k < self.kernels.shape[0] for k in kidx),
+                 "kernel_index must give a valid kernel for every problem")
+        if problem_ncoil is None and coil_problem_stride == 0:
+            C.call("hasty_sense_normal_toeplitz", x.data_ptr(), out.data_ptr(), self.smaps.data_ptr(),
+                   self.coils.data_ptr(), self.ncoil, self.kernels.data_ptr(), self.kernels.shape[0], self.layout,
+                   C.i32arr(kidx), C.ptr(sub), P, C.i64arr(self.n3), self.workspace.data_ptr(),
+                   self.workspace.numel(), self.coil_batch, C.stream_ptr())
+            return out
+        nc = [self.ncoil] * P if problem_ncoil is None else [int(c) for c in problem_ncoil]
+        _require(len(nc) == P and all(1 <= c <= self.ncoil for c in nc), "problem_ncoil must be in 1..ncoil")
+        nc_dev = torch.tensor(nc, dtype=torch.int32, device=x.device)
+        C.call("hasty_sense_normal_toeplitz_ex", x.data_ptr(), out.data_ptr(), self.smaps.data_ptr(),
+               int(self.smaps.shape[0]), self.coils.data_ptr(), self.ncoil, int(coil_problem_stride), C.i32arr(nc),
+               nc_dev.data_ptr(), self.kernels.data_ptr(), self.kernels.shape[0], self.layout, C.i32arr(kidx),
+               C.ptr(sub), P, C.i64arr(self.n3), self.workspace.data_ptr(), self.workspace.numel(), self.coil_batch,
                C.stream_ptr())
         return out
 

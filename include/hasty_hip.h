@@ -129,6 +129,8 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
  *     accumulates without atomics.
  * Returns 0, or 1 for an unknown name or a value out of range. */
 int hasty_set_option(const char* name, int64_t value);
+/* Current value of an option of hasty_set_option (0, or 1 for an unknown name). */
+int hasty_get_option(const char* name, int64_t* value);
 
 /* Kernel timing (no counterpart in the reference; a measurement hook for benchmarks).
  * While enabled, each instrumented launch is bracketed by two HIP events on its own
