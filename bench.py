@@ -12,11 +12,14 @@ Also reported:
 - per-kernel durations from HIP events recorded on the launch stream over the timed
   region, and the roofline of the dominant kernel (algorithmic bytes per launch over
   its average duration);
-- CG-SENSE recons/s for the batched form (15 iterations, all 100 problems in one
-  device-CG batch) and for the plain per-frame drop-in (the explicit NUFFT
+- CG-SENSE recons/s (15 iterations on the l2-regularised system A + 0.1 lambda_max I)
+  for the batched form (all 100 problems in one device-CG batch, eager and as a replayed
+  HIP graph of one iteration) and for the plain per-frame drop-in (the explicit NUFFT
   SenseNormal per problem inside a host-synchronised CG, as the reference runs it),
-  each with p50/p99 latency, and the relative difference of their images;
-- the CPU baseline: the FP64 oracle's Toeplitz SENSE normal operator on host cores.
+  the plain path's p50/p99 latency, and the images of problem 0 against the FP64
+  oracle's CG;
+- the CPU baseline: the reference's complex64 ATen sequence for the Toeplitz SENSE
+  normal operator on torch-CPU with every host thread.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -49,8 +52,10 @@ def parse():
     ap.add_argument("--readout", type=int, default=128)
     ap.add_argument("--spokes-min", type=int, default=800)
     ap.add_argument("--spokes-max", type=int, default=1500)
-    ap.add_argument("--coil-batch", type=int, default=16)
-    ap.add_argument("--problem-batch", type=int, default=4)
+    ap.add_argument("--coil-batch", type=int, default=None, help="coils per workspace (default: autotuned)")
+    ap.add_argument("--problem-batch", type=int, default=None, help="problems per launch (default: autotuned)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (defaults: 16 coils and 4 problems per launch, library options as set)")
     ap.add_argument("--generic", action="store_true", help="use the generic five-pass path")
     ap.add_argument("--xy-variant", type=int, default=None,
                     help="plane kernels of the fused path (library option sense_xy_variant)")
@@ -60,7 +65,11 @@ def parse():
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--plain", type=int, default=3, help="problems solved by the plain per-frame path (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-applies", type=int, default=6)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="CPU baseline: repeat whole applies until this many seconds have passed")
+    ap.add_argument("--cg-l2", type=float, default=0.1, help="l2 regularisation of the CG system, x lambda_max")
+    ap.add_argument("--power-iters", type=int, default=20)
+    ap.add_argument("--no-oracle-cg", action="store_true", help="skip the FP64 oracle CG of problem 0")
     ap.add_argument("--no-kernel-events", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (the known-size kernel for counter calibration)")
@@ -102,6 +111,11 @@ def nufft_main(args, torch, native):
     p1 = native.NufftPlan(co, [n, n, n], 1, +1, 1, 1e-6, scale)
     out2 = torch.empty((1, M), dtype=torch.complex64, device=dev)
     out1 = torch.empty((1, n, n, n), dtype=torch.complex64, device=dev)
+    tune = None
+    if not args.no_autotune:
+        from hastycompute_amd import autotune
+
+        tune = autotune.nufft(p1, c, out1)
     for _ in range(args.warmup):
         p2.execute(out2, f)
         p1.execute(c, out1)
@@ -136,7 +150,7 @@ def nufft_main(args, torch, native):
             "value": tr_s, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": 1e3 / tr_s, "higher_is_better": True, "dtype": "c64 (fp32)",
             "data": "synthetic (seeded kooshball, random image and samples)",
-            "config": {"workload": f"NUFFT {n}^3, fine {2 * n}^3, ns 7, M={M}", "M": M},
+            "config": {"workload": f"NUFFT {n}^3, fine {2 * n}^3, ns 7, M={M}", "M": M, "autotune": tune},
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": res[dom]["achieved_GBs"], "peak": 8000.0,
                          "unit": "GB/s", "frac": res[dom]["achieved_GBs"] / 8000.0, "traffic": None,
                          "alg_bytes_per_transform": B},
@@ -172,6 +186,11 @@ def llr_main(args, torch, native):
     blk = x[:, 0, :b, :b, :b].reshape(F, -1).T.cpu().numpy().astype(np.complex128)
     lam = 0.01 * float(np.linalg.svd(blk, compute_uv=False)[0])
     nblk = (n // b - 1) ** 3
+    tune = None
+    if not args.no_autotune:
+        from hastycompute_amd import autotune
+
+        tune = autotune.llr(list(x.shape), (b, b, b), (b, b, b))
     for _ in range(args.warmup):
         native.llr_normal_blocks(x, (b, b, b), (b, b, b), 1, lam, True)
     torch.cuda.synchronize()
@@ -198,12 +217,41 @@ def llr_main(args, torch, native):
             "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
             "dtype": "c64 (fp32, MFMA fp32 accumulate)", "data": "synthetic rank-5 temporal phantom + noise",
             "config": {"workload": f"LLR prox, [{F},1,{n},{n},{n}], 16^3 blocks, stride 16, lambda=0.01 sigma_max",
-                       "blocks": nblk, "lambda": lam},
+                       "blocks": nblk, "lambda": lam, "autotune": tune},
             "roofline": {"bound": "mfma", "kernel": dom, "achieved": kernels[dom]["achieved_TFLOPs"],
                          "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": kernels[dom]["mfma_frac"],
                          "traffic": None, "flops_per_launch": flops[dom]},
             "kernels": kernels}
+    if not args.no_cpu:
+        line["cpu_baseline"] = llr_cpu_baseline(args, torch, x, lam, b)
     print(json.dumps(line), flush=True)
+
+
+def llr_cpu_baseline(args, torch, x, lam, b, nblocks=8):
+    """The reference's per-block SVT on host cores: svt_soft (lib/op/svt.cpp:117-144)
+    as torch.linalg.svd of the block's Casorati matrix (complex64, 4096 x F), the soft
+    threshold max(S - lam, 0) and U diag(S) Vh, on torch-CPU with every host thread;
+    `nblocks` blocks of the benchmark volume (gather/scatter not timed)."""
+    threads = cpu_threads()
+    prev = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    F = x.shape[0]
+    mats = []
+    for i in range(nblocks):
+        z0 = b * i
+        mats.append(x[:, 0, :b, :b, z0:z0 + b].reshape(F, -1).T.contiguous().cpu())
+    t = time.perf_counter()
+    with torch.inference_mode():
+        for A in mats:
+            U, S, Vh = torch.linalg.svd(A, full_matrices=False)
+            S = torch.clamp(S - lam, min=0.0)
+            A.copy_((U * S.to(A.dtype)) @ Vh)
+    dt = time.perf_counter() - t
+    torch.set_num_threads(prev)
+    return {"value": nblocks / dt, "unit": "blocks/s", "cores": threads, "kind": "port",
+            "sample": f"{nblocks} blocks of {b ** 3} x {F}: torch.linalg.svd + soft threshold + U diag(S) Vh "
+                      f"(svt.cpp:117-144), complex64, torch-CPU ({threads} threads)",
+            "seconds": dt}
 
 
 def serve_main(args, torch):
@@ -281,28 +329,75 @@ def make_workload(args, torch, native):
     return K, smaps, x, kidx, npts, spokes, offsets
 
 
-def cpu_baseline(args, K, smaps, x):
-    """The FP64 oracle's Toeplitz SENSE normal operator on host cores, one problem x
-    args.cpu_applies applies at the benchmark size (kernel values from the GPU build;
-    the timing does not depend on them)."""
+def cpu_threads() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(args, torch, K, smaps, x):
+    """The reference's CPU path for this operator: the ATen call sequence of
+    CUDANormalNufftToeplitz::apply_inplace (lib/fft/nufft.cpp:1188-1195: fft_fftn_out
+    with s = 2N, mul_ by the complex diagonal, fft_ifftn_out, copy of the cropped
+    corner) inside the coil loop of sense_normal_apply (lib/mri/sense.ixx:533-577:
+    imstore = in * S_c, the Toeplitz apply, imstore *= conj(S_c), out += imstore), run
+    in complex64 on torch-CPU with every host thread. One problem at the benchmark size;
+    applies are repeated until about args.cpu_seconds have passed (at least one)."""
+    threads = cpu_threads()
+    prev = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    N = args.n
+    L = 2 * N
+    diag = (K[0].cpu() * float(N) ** 3).to(torch.complex64).contiguous()  # the reference's complex diagonal
+    S = smaps.cpu()
+    xin = x[0:1].cpu()
+    out = torch.zeros_like(xin)
+    imstore = torch.empty_like(xin)
+    storage1 = torch.empty((1, L, L, L), dtype=torch.complex64)
+    storage2 = torch.empty_like(storage1)
+    dims = (1, 2, 3)
+    napply = 0
+    t = time.perf_counter()
+    with torch.inference_mode():
+        while True:
+            out.zero_()
+            for c in range(args.coils):
+                torch.mul(xin, S[c], out=imstore)
+                torch.fft.fftn(imstore, s=(L, L, L), dim=dims, out=storage1)
+                storage1.mul_(diag)
+                torch.fft.ifftn(storage1, dim=dims, out=storage2)
+                imstore.copy_(storage2[:, :N, :N, :N])
+                imstore.mul_(S[c].conj())
+                out.add_(imstore)
+            napply += 1
+            if time.perf_counter() - t >= args.cpu_seconds:
+                break
+    dt = time.perf_counter() - t
+    torch.set_num_threads(prev)
+    return {"value": napply / dt, "unit": "applies/s", "cores": threads, "kind": "port",
+            "sample": f"1 problem x {napply} applies at {N}^3 x {args.coils} coils: the reference's ATen sequence "
+                      f"(nufft.cpp:1188-1195 in the coil loop of sense.ixx:533-577), complex64, torch-CPU "
+                      f"({threads} threads) on {platform.processor() or platform.machine()}",
+            "seconds": dt}
+
+
+def oracle_cg_image(args, K, smaps, b, p, lam):
+    """The FP64 oracle's CG (15 iterations) on problem p's system A + lam I, A the Toeplitz
+    SENSE normal operator with the problem's GPU-built kernel (so the comparison pins the
+    arithmetic of the operator and the CG, not the kernel build, which the tests pin)."""
     import numpy as np
 
     from oracle import oracle as O
 
-    threads = O.default_threads()
     N = args.n
-    # the oracle takes the reference's complex diagonal D = kernel * N^d
-    diag = (K[0].double().cpu().numpy() * float(N) ** 3).astype(np.complex128)[None]
-    S = smaps.cpu().numpy().astype(np.complex128)
-    x0 = x[0:1].cpu().numpy().astype(np.complex128)
-    t = time.perf_counter()
-    for _ in range(args.cpu_applies):
-        O.sense_normal_toeplitz(diag, x0, S, list(range(args.coils)), workers=threads)
-    dt = time.perf_counter() - t
-    return {"value": args.cpu_applies / dt, "unit": "applies/s", "cores": threads, "kind": "port",
-            "sample": f"1 problem x {args.cpu_applies} applies at {N}^3 x {args.coils} coils, FP64 oracle "
-                      f"(scipy.fft, {threads} threads) on {platform.processor() or platform.machine()}",
-            "seconds": dt}
+    diag = (K[p // args.encodings].double().cpu().numpy() * float(N) ** 3).astype(np.complex128)[None]
+    S = smaps.cpu().numpy()
+    coils = list(range(args.coils))
+    w = O.default_threads()
+    A = lambda v: O.sense_normal_toeplitz(diag, v, S, coils, workers=w) + lam * v
+    bp = b[p:p + 1].cpu().numpy().astype(np.complex128)
+    return O.conjugate_gradient(A, bp, np.zeros_like(bp), args.cg_iters)
 
 
 def percentile(v, q):
@@ -311,7 +406,7 @@ def percentile(v, q):
     return float(np.percentile(np.asarray(v, dtype=np.float64), q))
 
 
-def plain_recons(args, torch, spokes, offsets, smaps, b_all, x_batched, toeplitz_apply):
+def plain_recons(args, torch, spokes, offsets, smaps, b_all, x_batched, toeplitz_apply, lam=0.0):
     """The plain per-frame drop-in: for each sampled problem, the explicit NUFFT
     SenseNormal (HastySense.SenseNormal, default NUFFT options) inside the host CG of
     ConjugateGradient::run (opalgs.cpp:50-96), one problem at a time. Also returns, per
@@ -323,7 +418,7 @@ def plain_recons(args, torch, spokes, offsets, smaps, b_all, x_batched, toeplitz
 
     N = args.n
     coils = list(range(args.coils))
-    lat, diffs, apply_diffs, res_plain, res_batched = [], [], [], [], []
+    lat, diffs, apply_diffs, res_plain, res_batched, images = [], [], [], [], [], []
     for j in range(args.plain):
         p = j * args.encodings  # encoding 0 of frames 0, 1, ...
         f = p // args.encodings
@@ -334,9 +429,11 @@ def plain_recons(args, torch, spokes, offsets, smaps, b_all, x_batched, toeplitz
         im = torch.empty((1, N, N, N), dtype=torch.complex64, device=smaps.device)
         ks = torch.empty((1, co.shape[1]), dtype=torch.complex64, device=smaps.device)
 
-        def A(v):
+        def A(v):  # l2_reg_linop(SenseNormal, lam) (python/python/hastypy/base/opalg.py:626-627)
             out = torch.empty_like(v)
             op.apply(v, out, smaps, coils, im, ks)
+            if lam:
+                out.add_(v, alpha=lam)
             return out
 
         b = b_all[p:p + 1]
@@ -364,8 +461,9 @@ def plain_recons(args, torch, spokes, offsets, smaps, b_all, x_batched, toeplitz
         apply_diffs.append(float(nrm(A(b) - ta) / nrm(ta)))
         res_plain.append(float(nrm(tr - b) / nrm(b)))
         res_batched.append(float(nrm(toeplitz_apply(ref, p) - b) / nrm(b)))
+        images.append(x)
         del op, co
-    return lat, diffs, apply_diffs, res_plain, res_batched
+    return lat, diffs, apply_diffs, res_plain, res_batched, images
 
 
 def copy_calibration(torch):
@@ -376,6 +474,82 @@ def copy_calibration(torch):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8}), flush=True)
+
+
+def cg_section(args, torch, native, op, x, K, kidx, smaps, spokes, offsets):
+    """CG-SENSE over the 100 problems: 15 iterations of ConjugateGradient::run
+    (lib/op/opalgs.cpp:50-96) on l2_reg_linop(A, lam) = A + lam I
+    (python/python/hastypy/base/opalg.py:626-627) with lam = 0.1 lambda_max, the
+    regularisation under which the 15th iterate is well conditioned in its inputs (on the
+    unregularised kooshball system the FP64 CG itself moves by ~1e-5 under a one-ulp
+    change of b). Batched: one device-CG batch of all problems, eager and replaying a HIP
+    graph of one iteration; plain: the per-frame drop-in. Images of problem 0 are
+    compared with the FP64 oracle CG outside the timed regions."""
+    from hastycompute_amd import synthetic
+
+    P = x.shape[0]
+    # right-hand sides b = T x_true with x_true a phantom whose global phase varies over
+    # the problems (the velocity encodings and frames)
+    ph = synthetic.ellipsoid_phantom_torch(args.n, x.device, seed=args.seed)
+    phase = torch.exp(1j * torch.linspace(0.0, 3.0, P, device=x.device, dtype=torch.float64)).to(torch.complex64)
+    b = torch.empty_like(x)
+    op.apply((ph[None] * phase[:, None, None, None]).contiguous(), out=b, kernel_index=kidx)
+    del ph
+    # lambda_max of problem 0 by power iteration
+    v = b[0:1].clone()
+    w = torch.empty_like(v)
+    for _ in range(args.power_iters):
+        op.apply(v, out=w, kernel_index=kidx[:1])
+        lmax = float(torch.linalg.vector_norm(w))
+        v = w / lmax
+    lam = args.cg_l2 * lmax
+    A = lambda v_, o_: op.apply(v_, out=o_, kernel_index=kidx)
+    res = {"iters": args.cg_iters, "system": f"A + lam I, lam = {args.cg_l2} lambda_max (l2_reg_linop)",
+           "lambda": lam, "lambda_max": lmax}
+    xs = {}
+    for mode in ("eager", "graph"):
+        x0 = torch.zeros_like(x)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        xs[mode], _ = native.cg_solve(A, b, x0, args.cg_iters, check_pd=False, l2=lam, graph=mode == "graph")
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        res[f"batched_{mode}_recons_per_s"] = P / dt
+        res[f"batched_{mode}_makespan_s"] = dt
+        del x0
+    res["graph_equals_eager"] = bool(torch.equal(xs["eager"], xs["graph"]))
+    best = max(("eager", "graph"), key=lambda m: res[f"batched_{m}_recons_per_s"])
+    res["batched_recons_per_s"] = res[f"batched_{best}_recons_per_s"]
+    res["batched_mode"] = best
+    res["batched_note"] = ("all problems submitted together and finishing with the batch, so the makespan is every "
+                           "problem's latency; latency distributions come from --mode serve")
+    xb = xs["graph"]
+    del xs
+    images = []
+    if args.plain > 0:
+        def t_apply(v, p):
+            o = torch.empty_like(v)
+            op.apply(v, out=o, kernel_index=[kidx[p]])
+            return o.add_(v, alpha=lam)
+
+        lat, diffs, adiff, rp, rb, images = plain_recons(args, torch, spokes, offsets, smaps, b, xb, t_apply, lam)
+        res.update({"plain_recons_per_s": len(lat) / sum(lat), "plain_latency_p50_s": percentile(lat, 50),
+                    "plain_latency_p99_s": percentile(lat, 99), "plain_sampled_problems": len(lat),
+                    "speedup_recons_per_s": res["batched_recons_per_s"] / (len(lat) / sum(lat)),
+                    "plain_vs_batched_apply_rel_diff_max": max(adiff),
+                    "plain_vs_batched_image_rel_diff_max": max(diffs),
+                    "residual_rel_plain_max": max(rp), "residual_rel_batched_max": max(rb)})
+    if not args.no_oracle_cg:
+        from oracle import oracle as O
+
+        t = time.perf_counter()
+        ref = oracle_cg_image(args, K, smaps, b, 0, lam)
+        res["oracle_cg_seconds"] = time.perf_counter() - t
+        res["image_rel_l2_vs_fp64_oracle"] = {"batched": O.rel_l2(xb[0:1].cpu().numpy(), ref)}
+        if images:
+            res["image_rel_l2_vs_fp64_oracle"]["plain"] = O.rel_l2(images[0].cpu().numpy(), ref)
+        res["image_rel_l2_vs_fp64_oracle"]["problem"] = 0
+    return res
 
 
 def main():
@@ -407,6 +581,16 @@ def main():
     if args.zconv_batch is not None:
         from hastycompute_amd import _capi
         _capi.call("hasty_set_option", b"sense_zconv_batch", args.zconv_batch)
+    tune = None
+    manual = any(v is not None for v in (args.xy_variant, args.zconv_batch, args.coil_batch, args.problem_batch))
+    if not (args.no_autotune or args.generic or manual):
+        from hastycompute_amd import autotune
+
+        # before any kernel is prepared: the tuner sets the plane-kernel option
+        tune = autotune.sense_normal([args.n] * 3, args.coils, args.frames * args.encodings)
+        args.coil_batch, args.problem_batch = tune["coil_batch"], tune["problem_batch"]
+    args.coil_batch = 16 if args.coil_batch is None else args.coil_batch
+    args.problem_batch = 4 if args.problem_batch is None else args.problem_batch
     K, smaps, x, kidx, npts, spokes, offsets = make_workload(args, torch, native)
     P = x.shape[0]
     op = native.SenseNormalToeplitz(smaps, K, coil_batch=args.coil_batch, problem_batch=args.problem_batch,
@@ -476,7 +660,8 @@ def main():
         "config": {"workload": "4D-flow batched normal op: 5 enc x 20 frames = 100 problems, 128^3, 16 coils, "
                                "grid 256^3, 800-1500 spokes x 128 per frame",
                    "problems": P, "n": args.n, "coils": args.coils, "coil_batch": args.coil_batch,
-                   "problem_batch": args.problem_batch, "path": "fused" if op.layout == 1 else "generic",
+                   "problem_batch": args.problem_batch, "path": "fused" if op.layout != 0 else "generic",
+                   "kernel_layout": op.layout, "autotune": tune,
                    "points_per_frame_mean": float(sum(npts) / len(npts))},
         "roofline": roofline,
         "op_roofline": {"achieved": op_achieved, "frac": op_achieved / HBM_PEAK_GBS, "alg_bytes_per_apply": alg_bytes,
@@ -486,43 +671,10 @@ def main():
     }
 
     if not args.no_cg:
-        from hastycompute_amd.native import cg_solve
-
-        from hastycompute_amd import synthetic
-
-        # right-hand sides b = T x_true with x_true a phantom whose global phase varies
-        # over the problems (the velocity encodings and frames)
-        ph = synthetic.ellipsoid_phantom_torch(args.n, x.device, seed=args.seed)
-        phase = torch.exp(1j * torch.linspace(0.0, 3.0, P, device=x.device, dtype=torch.float64)).to(torch.complex64)
-        b = torch.empty_like(x)
-        op.apply((ph[None] * phase[:, None, None, None]).contiguous(), out=b, kernel_index=kidx)
-        del ph
-        x0 = torch.zeros_like(x)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        xb, _ = cg_solve(lambda v, o: op.apply(v, out=o, kernel_index=kidx), b, x0, args.cg_iters, check_pd=False)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t
-        cg = {"iters": args.cg_iters, "batched_recons_per_s": P / dt,
-              "batched_latency_p50_s": dt, "batched_latency_p99_s": dt,
-              "batched_note": "all problems submitted together; each finishes with the batch"}
-        if args.plain > 0:
-            def t_apply(v, p):
-                o = torch.empty_like(v)
-                op.apply(v, out=o, kernel_index=[kidx[p]])
-                return o
-
-            lat, diffs, adiff, rp, rb = plain_recons(args, torch, spokes, offsets, smaps, b, xb, t_apply)
-            cg.update({"plain_recons_per_s": len(lat) / sum(lat), "plain_latency_p50_s": percentile(lat, 50),
-                       "plain_latency_p99_s": percentile(lat, 99), "plain_sampled_problems": len(lat),
-                       "speedup_recons_per_s": (P / dt) / (len(lat) / sum(lat)),
-                       "plain_vs_batched_apply_rel_diff_max": max(adiff),
-                       "plain_vs_batched_image_rel_diff_max": max(diffs),
-                       "residual_rel_plain_max": max(rp), "residual_rel_batched_max": max(rb)})
-        line["cg"] = cg
-        line["recons_per_s"] = cg["batched_recons_per_s"]
+        line["cg"] = cg_section(args, torch, native, op, x, K, kidx, smaps, spokes, offsets)
+        line["recons_per_s"] = line["cg"]["batched_recons_per_s"]
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args, K, smaps, x)
+        line["cpu_baseline"] = cpu_baseline(args, torch, K, smaps, x)
     print(json.dumps(line), flush=True)
 
 

@@ -45,12 +45,21 @@ _SIGS = {
                                         P_i32, c_vp, c_int, P_i64, c_vp, c_sz, c_int, c_vp], c_int),
     "hasty_set_option": ([ctypes.c_char_p, c_i64], c_int),
     "hasty_get_option": ([ctypes.c_char_p, P_i64], c_int),
+    "hasty_autotune_sense_normal": ([P_i64, c_int, c_int, P_i64, P_dbl, c_vp], c_int),
+    "hasty_nufft_autotune": ([c_vp, c_vp, c_vp, P_i64, P_dbl, c_vp], c_int),
+    "hasty_autotune_llr": ([P_i64, P_i64, P_i64, P_i64, P_dbl, c_vp], c_int),
     "hasty_timing_enable": ([c_int], c_int),
     "hasty_timing_entries": ([], c_int),
     "hasty_timing_read": ([c_int, ctypes.c_char_p, c_int, P_dbl, P_i64], c_int),
+    "hasty_graph_begin": ([c_vp], c_int),
+    "hasty_graph_end": ([c_vp, ctypes.POINTER(c_vp)], c_int),
+    "hasty_graph_launch": ([c_vp, c_vp], c_int),
+    "hasty_graph_nodes": ([c_vp], c_i64),
+    "hasty_graph_destroy": ([c_vp], c_int),
     "hasty_cg_workspace_size": ([c_i64, c_i64], c_sz),
     "hasty_cg_init": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "hasty_cg_iterate": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp], c_int),
+    "hasty_cg_iterate_l2": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl, c_vp], c_int),
     "hasty_cg_init_precond": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "hasty_cg_iterate_xr": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp], c_int),
     "hasty_cg_iterate_pz": ([c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),

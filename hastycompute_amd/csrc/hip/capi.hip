@@ -2,7 +2,10 @@
 // host-side drivers behind them (LLR block grid, workspace layout).
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <random>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -12,6 +15,7 @@
 #include "toeplitz.hpp"
 #include "cg.hpp"
 #include "llr.hpp"
+#include "tune.hpp"
 
 namespace hasty_hip {
 
@@ -66,17 +70,28 @@ std::vector<int32_t> grid_blocks(const int64_t* dims5, const int64_t* strides, c
     return out;
 }
 
-int choose_nsplit(int64_t nblk, int64_t R)
+// Row splits of the LLR Gram per (blocks per launch, rows, columns), set by
+// hasty_autotune_llr; other shapes use the rule in choose_nsplit.
+std::map<std::tuple<int64_t, int64_t, int64_t>, int> g_nsplit_tuned;
+std::mutex g_nsplit_mu;
+
+int max_nsplit(int64_t R) { return (int)std::min<int64_t>(8, std::max<int64_t>(1, R / 256)); }
+
+int choose_nsplit(int64_t nblk, int64_t R, int64_t F)
 {
+    {
+        std::lock_guard<std::mutex> lock(g_nsplit_mu);
+        auto it = g_nsplit_tuned.find(std::make_tuple(nblk, R, F));
+        if (it != g_nsplit_tuned.end()) return it->second;
+    }
     int64_t ns = ceil_div(512, std::max<int64_t>(1, nblk));
-    ns = std::min<int64_t>(ns, 8);
-    ns = std::min<int64_t>(ns, std::max<int64_t>(1, R / 256));
+    ns = std::min<int64_t>(ns, max_nsplit(R));
     return (int)std::max<int64_t>(1, ns);
 }
 
 size_t llr_ws_bytes_for(int64_t nblk, int64_t F, int64_t R)
 {
-    const int ns = choose_nsplit(nblk, R);
+    const int ns = choose_nsplit(nblk, R, F);
     return 256 + sizeof(int32_t) * 3 * (size_t)nblk + llr_workspace_bytes(nblk, F, ns);
 }
 
@@ -95,7 +110,7 @@ void run_llr_batch(float2* data, const LlrGeom& g, const std::vector<int32_t>& c
     const int64_t nblk = (int64_t)corners.size() / 3;
     if (nblk == 0) return;
     const int64_t R = g.E * g.bx * g.by * g.bz;
-    const int nsplit = choose_nsplit(nblk, R);
+    const int nsplit = choose_nsplit(nblk, R, g.F);
     const size_t cbytes = ((sizeof(int32_t) * 3 * (size_t)nblk + 255) / 256) * 256;
     HASTY_REQUIRE(ws_bytes >= cbytes + llr_workspace_bytes(nblk, g.F, nsplit), "LLR workspace too small");
     int3* cdev = reinterpret_cast<int3*>(ws);
@@ -162,6 +177,12 @@ using namespace hasty_hip;
 
 struct hasty_nufft_plan_s {
     NufftPlan* impl;
+};
+
+struct hasty_graph_s {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    int64_t nodes;
 };
 
 extern "C" {
@@ -393,6 +414,16 @@ int hasty_cg_iterate(int64_t nproblems, int64_t n, void* x, void* r, void* p, co
     });
 }
 
+int hasty_cg_iterate_l2(int64_t nproblems, int64_t n, void* x, void* r, void* p, const void* Ap, double* state,
+                        void* workspace, double tol, double lambda, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(x && r && p && Ap && state && workspace, "null argument");
+        cg_iterate(nproblems, n, (float2*)x, (float2*)r, (float2*)p, (const float2*)Ap, state, (double*)workspace, tol,
+                   as_stream(stream), lambda);
+    });
+}
+
 int hasty_cg_init_precond(int64_t nproblems, int64_t n, const void* r, const void* z, void* p, double* state,
                           void* workspace, void* stream)
 {
@@ -581,6 +612,150 @@ int hasty_svt_matrix(void* A, int64_t R, int64_t F, double thresh, int soft, voi
         g.rstride = F;
         std::vector<int32_t> c = {0, 0, 0};
         run_llr_batch((float2*)A, g, c, (float)thresh, soft, workspace, workspace_bytes, as_stream(stream));
+    });
+}
+
+int hasty_autotune_sense_normal(const int64_t* nmodes3, int ncoil, int nproblems, int64_t* choice,
+                                double* ms_per_problem, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(nmodes3 && choice, "null argument");
+        for (int d = 0; d < 3; ++d) HASTY_REQUIRE(nmodes3[d] >= 1, "nmodes must be >= 1");
+        const SenseChoice c = autotune_sense_normal(nmodes3, ncoil, nproblems, g_opt_sense_fused != 0,
+                                                    as_stream(stream));
+        choice[0] = c.xy_variant;
+        choice[1] = c.zconv_batch;
+        choice[2] = c.problem_batch;
+        choice[3] = c.coil_batch;
+        if (ms_per_problem) *ms_per_problem = c.ms_per_problem;
+    });
+}
+
+int hasty_nufft_autotune(hasty_nufft_plan plan, void* c, void* f, int64_t* spread_variant, double* ms, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(plan != nullptr, "null plan");
+        const int v = nufft_autotune(plan->impl, c, f, as_stream(stream), ms);
+        if (spread_variant) *spread_variant = v;
+    });
+}
+
+int hasty_autotune_llr(const int64_t* dims5, const int64_t* block_strides, const int64_t* block_shape,
+                       int64_t* nsplit, double* ms, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(dims5 && block_strides && block_shape, "null argument");
+        for (int d = 0; d < 3; ++d) HASTY_REQUIRE(block_strides[d] >= 1, "block strides must be >= 1");
+        check_dims5(dims5, block_shape);
+        hipStream_t s = as_stream(stream);
+        const LlrGeom g = image_geom(dims5, block_shape);
+        const std::vector<int32_t> blocks = grid_blocks(dims5, block_strides, block_shape, 0, 0, 0);
+        const int64_t nblk = (int64_t)blocks.size() / 3;
+        const int64_t R = g.E * g.bx * g.by * g.bz;
+        if (nblk == 0) {
+            if (nsplit) *nsplit = 1;
+            if (ms) *ms = 0.0;
+            return;
+        }
+        const auto key = std::make_tuple(nblk, R, g.F);
+        {
+            std::lock_guard<std::mutex> lock(g_nsplit_mu);
+            g_nsplit_tuned.erase(key);
+        }
+        const int ns0 = choose_nsplit(nblk, R, g.F);
+        const int64_t nval = dims5[0] * dims5[1] * dims5[2] * dims5[3] * dims5[4];
+        void* img = dev_alloc(sizeof(float2) * nval);
+        void* ws = nullptr;
+        int best = ns0;
+        double tbest = 1e300;
+        try {
+            fill_uniform((float*)img, 2 * nval, 7u, -1.f, 1.f, s);
+            std::vector<int> order = {ns0};
+            for (int v : {1, 2, 4, 8})
+                if (v != ns0 && v <= max_nsplit(R)) order.push_back(v);
+            for (int v : order) {
+                {
+                    std::lock_guard<std::mutex> lock(g_nsplit_mu);
+                    g_nsplit_tuned[key] = v;
+                }
+                const size_t wsb = llr_ws_bytes_for(nblk, g.F, R);
+                dev_free(ws);
+                ws = nullptr;
+                ws = dev_alloc(wsb);
+                // threshold 0: W = I, so the data stays as it is between runs
+                const double t = time_best_ms(
+                    [&] { run_llr_batch((float2*)img, g, blocks, 0.f, 1, ws, wsb, s); }, s, 3);
+                if (t < 0.98 * tbest) {
+                    tbest = t;
+                    best = v;
+                }
+            }
+        } catch (...) {
+            dev_free(ws);
+            dev_free(img);
+            std::lock_guard<std::mutex> lock(g_nsplit_mu);
+            g_nsplit_tuned.erase(key);
+            throw;
+        }
+        dev_free(ws);
+        dev_free(img);
+        {
+            std::lock_guard<std::mutex> lock(g_nsplit_mu);
+            g_nsplit_tuned[key] = best;
+        }
+        if (nsplit) *nsplit = best;
+        if (ms) *ms = tbest;
+    });
+}
+
+int hasty_graph_begin(void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(stream != nullptr, "the NULL stream cannot be captured; use a created stream");
+        HASTY_HIP_CHECK(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
+    });
+}
+
+int hasty_graph_end(void* stream, hasty_graph* graph)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(stream != nullptr && graph != nullptr, "null argument");
+        *graph = nullptr;
+        hipGraph_t g = nullptr;
+        HASTY_HIP_CHECK(hipStreamEndCapture(as_stream(stream), &g));
+        HASTY_REQUIRE(g != nullptr, "the capture produced no graph");
+        size_t nodes = 0;
+        hipGraphExec_t e = nullptr;
+        hipError_t err = hipGraphGetNodes(g, nullptr, &nodes);
+        if (err == hipSuccess) err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        if (err != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            HASTY_HIP_CHECK(err);
+        }
+        *graph = new hasty_graph_s{g, e, (int64_t)nodes};
+    });
+}
+
+int hasty_graph_launch(hasty_graph graph, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(graph != nullptr, "null graph");
+        HASTY_REQUIRE(stream != nullptr, "graphs launch on a created stream, not the NULL stream");
+        HASTY_HIP_CHECK(hipGraphLaunch(graph->exec, as_stream(stream)));
+    });
+}
+
+int64_t hasty_graph_nodes(hasty_graph graph) { return graph ? graph->nodes : -1; }
+
+int hasty_graph_destroy(hasty_graph graph)
+{
+    return guarded([&] {
+        if (!graph) return;
+        hipError_t e1 = graph->exec ? hipGraphExecDestroy(graph->exec) : hipSuccess;
+        hipError_t e2 = graph->graph ? hipGraphDestroy(graph->graph) : hipSuccess;
+        delete graph;
+        HASTY_HIP_CHECK(e1);
+        HASTY_HIP_CHECK(e2);
     });
 }
 

@@ -33,8 +33,10 @@ __device__ __forceinline__ double block_reduce(double v, double* sh)
     return r;
 }
 
-// partial[p][blk] = sum Re(conj(a) * b) over this block's slice of problem p
-__global__ void __launch_bounds__(kThreads) dot_re_kernel(const float2* a, const float2* b, int64_t n, double* partial)
+// partial[p][blk] = sum Re(conj(a) * (b + shift a)) over this block's slice of problem p
+// (b + shift a rounded to float, as cg_update_xr_kernel forms it)
+__global__ void __launch_bounds__(kThreads) dot_re_kernel(const float2* a, const float2* b, int64_t n, double* partial,
+                                                          float shift)
 {
     __shared__ double sh[kThreads];
     const int p = blockIdx.y;
@@ -42,7 +44,12 @@ __global__ void __launch_bounds__(kThreads) dot_re_kernel(const float2* a, const
     const float2* bp = b + (int64_t)p * n;
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        const float2 u = ap[i], v = bp[i];
+        const float2 u = ap[i];
+        float2 v = bp[i];
+        if (shift != 0.f) {
+            v.x += shift * u.x;
+            v.y += shift * u.y;
+        }
         s += (double)u.x * (double)v.x + (double)u.y * (double)v.y;
     }
     s = block_reduce(s, sh);
@@ -110,9 +117,10 @@ __global__ void __launch_bounds__(kThreads) cg_alpha_kernel(const double* partia
     }
 }
 
-// x += alpha p; r -= alpha Ap; partial |r|^2
+// x += alpha p; r -= alpha (Ap + shift p); partial |r|^2
 __global__ void __launch_bounds__(kThreads) cg_update_xr_kernel(float2* x, float2* r, const float2* pv, const float2* Ap,
-                                                                int64_t n, const double* state, double* partial)
+                                                                int64_t n, const double* state, double* partial,
+                                                                float shift)
 {
     __shared__ double sh[kThreads];
     const int p = blockIdx.y;
@@ -124,7 +132,12 @@ __global__ void __launch_bounds__(kThreads) cg_update_xr_kernel(float2* x, float
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
         float2 rv = r[off + i];
         if (active) {
-            const float2 pp = pv[off + i], ap = Ap[off + i];
+            const float2 pp = pv[off + i];
+            float2 ap = Ap[off + i];
+            if (shift != 0.f) {
+                ap.x += shift * pp.x;
+                ap.y += shift * pp.y;
+            }
             float2 xv = x[off + i];
             xv.x += alpha * pp.x;
             xv.y += alpha * pp.y;
@@ -215,22 +228,24 @@ void cg_init(int64_t P, int64_t n, const float2* b, const float2* Ax, float2* r,
 void cg_dot_re(int64_t P, int64_t n, const float2* a, const float2* b, double* partial, int* nblk, hipStream_t s)
 {
     const int nb = blocks_for(n);
-    hipLaunchKernelGGL(dot_re_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, a, b, n, partial);
+    hipLaunchKernelGGL(dot_re_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, a, b, n, partial, 0.f);
     check_launch("dot_re_kernel");
     *nblk = nb;
 }
 
 void cg_iterate(int64_t P, int64_t n, float2* x, float2* r, float2* p, const float2* Ap, double* state, double* ws,
-                double tol, hipStream_t s)
+                double tol, hipStream_t s, double shift)
 {
     HASTY_REQUIRE(P >= 1 && P < 65536 && n >= 1, "invalid CG problem sizes");
+    HASTY_REQUIRE(shift >= 0.0, "the l2 regularisation weight must be >= 0");
     const int nb = blocks_for(n);
-    hipLaunchKernelGGL(dot_re_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, (const float2*)p, Ap, n, ws);
+    const float sh = (float)shift;
+    hipLaunchKernelGGL(dot_re_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, (const float2*)p, Ap, n, ws, sh);
     check_launch("dot_re_kernel");
     hipLaunchKernelGGL(cg_alpha_kernel, dim3((unsigned)P), dim3(kThreads), 0, s, (const double*)ws, nb, state, tol);
     check_launch("cg_alpha_kernel");
     hipLaunchKernelGGL(cg_update_xr_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, x, r, (const float2*)p, Ap, n,
-                       (const double*)state, ws);
+                       (const double*)state, ws, sh);
     check_launch("cg_update_xr_kernel");
     hipLaunchKernelGGL(cg_beta_kernel, dim3((unsigned)P), dim3(kThreads), 0, s, (const double*)ws, nb, state);
     check_launch("cg_beta_kernel");
@@ -247,7 +262,7 @@ void cg_init_precond(int64_t P, int64_t n, const float2* r, const float2* z, flo
 {
     HASTY_REQUIRE(P >= 1 && P < 65536 && n >= 1, "invalid CG problem sizes");
     const int nb = blocks_for(n);
-    hipLaunchKernelGGL(dot_re_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, r, z, n, ws);
+    hipLaunchKernelGGL(dot_re_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, r, z, n, ws, 0.f);
     check_launch("dot_re_kernel");
     hipLaunchKernelGGL(cg_rz_finalize, dim3((unsigned)P), dim3(kThreads), 0, s, (const double*)ws, nb, state);
     check_launch("cg_rz_finalize");
@@ -260,12 +275,12 @@ void cg_iterate_xr(int64_t P, int64_t n, float2* x, float2* r, const float2* p, 
 {
     HASTY_REQUIRE(P >= 1 && P < 65536 && n >= 1, "invalid CG problem sizes");
     const int nb = blocks_for(n);
-    hipLaunchKernelGGL(dot_re_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, p, Ap, n, ws);
+    hipLaunchKernelGGL(dot_re_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, p, Ap, n, ws, 0.f);
     check_launch("dot_re_kernel");
     hipLaunchKernelGGL(cg_alpha_kernel, dim3((unsigned)P), dim3(kThreads), 0, s, (const double*)ws, nb, state, tol);
     check_launch("cg_alpha_kernel");
     hipLaunchKernelGGL(cg_update_xr_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, x, r, p, Ap, n,
-                       (const double*)state, ws);
+                       (const double*)state, ws, 0.f);
     check_launch("cg_update_xr_kernel");
 }
 
@@ -274,7 +289,7 @@ void cg_iterate_pz(int64_t P, int64_t n, const float2* r, const float2* z, float
 {
     HASTY_REQUIRE(P >= 1 && P < 65536 && n >= 1, "invalid CG problem sizes");
     const int nb = blocks_for(n);
-    hipLaunchKernelGGL(dot_re_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, r, z, n, ws);
+    hipLaunchKernelGGL(dot_re_kernel, dim3(nb, (unsigned)P), dim3(kThreads), 0, s, r, z, n, ws, 0.f);
     check_launch("dot_re_kernel");
     hipLaunchKernelGGL(cg_beta_kernel, dim3((unsigned)P), dim3(kThreads), 0, s, (const double*)ws, nb, state);
     check_launch("cg_beta_kernel");

@@ -107,8 +107,18 @@ void timing_enable(bool on);                                        // also clea
 int timing_entries();
 bool timing_read(int index, std::string* name, double* total_ms, int64_t* count);
 
+// True while `s` is being captured into a graph (its launches are recorded, not run).
+inline bool stream_capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return s != nullptr && hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
+}
+
 struct KernelTimer {
-    KernelTimer(const char* n, hipStream_t st) : name(n), s(st), ev(timing_on() ? timing_begin(st) : nullptr) {}
+    KernelTimer(const char* n, hipStream_t st)
+        : name(n), s(st), ev(timing_on() && !stream_capturing(st) ? timing_begin(st) : nullptr)
+    {
+    }
     ~KernelTimer()
     {
         if (ev) timing_end(name, ev, s);

@@ -498,22 +498,23 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(SpreadArgs<R> a)
     a.cout[j] = cplx<R>{sr, si};
 }
 
-template <typename R, int NS, int DIM> void launch_spread_ns(const SpreadArgs<R>& a, int64_t nsub, size_t smem, hipStream_t s)
+template <typename R, int NS, int DIM>
+void launch_spread_ns(const SpreadArgs<R>& a, int64_t nsub, size_t smem, hipStream_t s, int variant)
 {
     if constexpr (DIM == 3) {
-        if (g_spread_variant == 1) {
+        if (variant == 1) {
             allow_lds(spread_rows_kernel<R, NS, 1, true>, smem);
             KernelTimer t("nufft_spread", s);
             hipLaunchKernelGGL((spread_rows_kernel<R, NS, 1, true>), dim3((unsigned)nsub), dim3(64), smem, s, a);
             return;
         }
-        if (g_spread_variant == 2) {
+        if (variant == 2) {
             allow_lds(spread_rows_kernel<R, NS, 4, false>, smem);
             KernelTimer t("nufft_spread", s);
             hipLaunchKernelGGL((spread_rows_kernel<R, NS, 4, false>), dim3((unsigned)nsub), dim3(256), smem, s, a);
             return;
         }
-        if (g_spread_variant == 3) {
+        if (variant == 3) {
             allow_lds(spread_rows_kernel<R, NS, 4, true>, smem);
             KernelTimer t("nufft_spread", s);
             hipLaunchKernelGGL((spread_rows_kernel<R, NS, 4, true>), dim3((unsigned)nsub), dim3(256), smem, s, a);
@@ -531,12 +532,12 @@ template <typename R, int NS, int DIM> void launch_interp_ns(const SpreadArgs<R>
 }
 
 template <typename R, int DIM>
-void dispatch_ns(bool spread, int ns, const SpreadArgs<R>& a, int64_t n, size_t smem, hipStream_t s)
+void dispatch_ns(bool spread, int ns, const SpreadArgs<R>& a, int64_t n, size_t smem, hipStream_t s, int variant)
 {
 #define HASTY_NS_CASE(K)                                              \
     case K:                                                           \
         if (spread)                                                   \
-            launch_spread_ns<R, K, DIM>(a, n, smem, s);               \
+            launch_spread_ns<R, K, DIM>(a, n, smem, s, variant);      \
         else                                                          \
             launch_interp_ns<R, K, DIM>(a, n, s);                     \
         break;
@@ -562,15 +563,19 @@ void dispatch_ns(bool spread, int ns, const SpreadArgs<R>& a, int64_t n, size_t 
     check_launch(spread ? "spread_kernel" : "interp_kernel");
 }
 
+// the plan's 3-D spreader (nufft_autotune) or else the library option
+int spread_variant(const NufftPlan* p) { return p->spread_variant >= 0 ? p->spread_variant : g_spread_variant; }
+
 template <typename R> void dispatch_dim(bool spread, const NufftPlan* p, const SpreadArgs<R>& a, int64_t n, size_t smem,
                                         hipStream_t s)
 {
+    const int v = spread_variant(p);
     if (p->dim == 3)
-        dispatch_ns<R, 3>(spread, p->ns, a, n, smem, s);
+        dispatch_ns<R, 3>(spread, p->ns, a, n, smem, s, v);
     else if (p->dim == 2)
-        dispatch_ns<R, 2>(spread, p->ns, a, n, smem, s);
+        dispatch_ns<R, 2>(spread, p->ns, a, n, smem, s, v);
     else
-        dispatch_ns<R, 1>(spread, p->ns, a, n, smem, s);
+        dispatch_ns<R, 1>(spread, p->ns, a, n, smem, s, v);
 }
 
 // Gauss-Legendre nodes/weights on [-1, 1] by Newton's method on P_n.
@@ -801,7 +806,7 @@ template <typename R> void setpts_typed(NufftPlan* p, hipStream_t s)
 }
 
 // padded LDS tiles: complex64 always, complex128 for the 3-D window-row spreader
-bool pad_tile(const NufftPlan* p) { return !p->fp64 || (p->dim == 3 && g_spread_variant != 0); }
+bool pad_tile(const NufftPlan* p) { return !p->fp64 || (p->dim == 3 && spread_variant(p) != 0); }
 
 template <typename R> SpreadArgs<R> spread_args(const NufftPlan* p)
 {

@@ -38,6 +38,7 @@ struct NufftPlan {
     int64_t cap_sub = 0;
     int64_t nsub = 0;
     int device = 0;
+    int spread_variant = -1;  // 3-D spreader of this plan (-1: the library option)
 };
 
 // nmodes in FINUFFT order (fastest-varying first), as the reference passes
@@ -54,6 +55,11 @@ void nufft_destroy(NufftPlan* p);
 // flushing the tile while the first wave accumulates without atomics
 void nufft_set_spread_variant(int v);
 int nufft_get_spread_variant();
+
+// Time the 3-D type-1 spreaders on the plan's points (c -> f, f overwritten) and keep
+// the fastest for this plan; returns it and its execute time in *ms. Other plans: no-op
+// returning the current spreader.
+int nufft_autotune(NufftPlan* p, void* c, void* f, hipStream_t s, double* ms);
 
 // ES kernel parameters for upsampling factor 2.
 void es_params(double tol, int* ns, double* beta);

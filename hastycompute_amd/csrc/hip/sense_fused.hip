@@ -1548,12 +1548,15 @@ bool sense_fused_supported(const int64_t* n3)
            2 * n3[0] <= 512;
 }
 
-void sense_fused_permute_kernel(const float* natural, float* permuted, const int64_t* n3, hipStream_t s)
+bool sense_fused_register_layout(const int64_t* n3) { return use_r16((int)n3[1]); }
+
+void sense_fused_permute_kernel(const float* natural, float* permuted, const int64_t* n3, bool r16, hipStream_t s)
 {
     const int L0 = (int)(2 * n3[0]), L = (int)(2 * n3[1]);
     const int64_t n = (int64_t)L0 * L * L;
+    HASTY_REQUIRE(!r16 || L == 256, "the register-kernel layout needs N1 = N2 = 128");
     hipLaunchKernelGGL(permute_kernel_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, natural, permuted, L0,
-                       L, use_r16((int)n3[1]) ? 1 : 0);
+                       L, r16 ? 1 : 0);
     check_launch("permute_kernel_kernel");
 }
 
@@ -1563,9 +1566,9 @@ size_t sense_fused_workspace_bytes(const int64_t* n3, int pchunk, int ccount)
     return sizeof(float2) * (size_t)pchunk * ccount * n3[0] * L * L;
 }
 
-// kernels: permuted layout (sense_fused_permute_kernel); kidx: host int [P].
+// kernels: permuted layout (sense_fused_permute_kernel with r16); kidx: host int [P].
 void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const int* coils_dev, int ncoil,
-                       const float* kernels, const int* kidx, const float2* sub, int P, const int64_t* n3,
+                       const float* kernels, bool r16, const int* kidx, const float2* sub, int P, const int64_t* n3,
                        void* ws, size_t ws_bytes, int pchunk, int cchunk, hipStream_t s, int coil_pstride,
                        const int* pcoils_dev)
 {
@@ -1612,7 +1615,8 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
     }
     if (g_xy_variant == 3)
         inv = NB == 1 ? inv_xy_d_kernel<1> : NB == 2 ? inv_xy_d_kernel<2> : NB == 3 ? inv_xy_d_kernel<3> : inv_xy_d_kernel<4>;
-    if (use_r16(N)) {
+    HASTY_REQUIRE(!r16 || N == 128, "the register-kernel layout needs N1 = N2 = 128");
+    if (r16) {
         fwd = fwd_xy_r16_kernel;
         inv = inv_xy_r16_kernel;
         nxy = kQT;

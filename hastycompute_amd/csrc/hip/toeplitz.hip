@@ -275,15 +275,20 @@ void toeplitz_apply_ref(const float2* in, float2* out, const float2* mul, const 
     (void)nvox;
 }
 
-int sense_kernel_layout(const int64_t* n3, bool allow_fused) { return allow_fused && sense_fused_supported(n3) ? 1 : 0; }
+int sense_kernel_layout(const int64_t* n3, bool allow_fused)
+{
+    if (!allow_fused || !sense_fused_supported(n3)) return 0;
+    return sense_fused_register_layout(n3) ? 2 : 1;
+}
 
 void sense_prepare_kernels(const float* natural, float* prepared, int64_t nk, const int64_t* n3, int layout,
                            hipStream_t s)
 {
     const int64_t kvox = 8 * n3[0] * n3[1] * n3[2];
-    if (layout == 1) {
+    if (layout == 1 || layout == 2) {
         HASTY_REQUIRE(sense_fused_supported(n3), "fused kernel layout not available for these sizes");
-        for (int64_t k = 0; k < nk; ++k) sense_fused_permute_kernel(natural + k * kvox, prepared + k * kvox, n3, s);
+        for (int64_t k = 0; k < nk; ++k)
+            sense_fused_permute_kernel(natural + k * kvox, prepared + k * kvox, n3, layout == 2, s);
     } else {
         HASTY_REQUIRE(layout == 0, "unknown kernel layout");
         if (prepared != natural)
@@ -312,13 +317,14 @@ void sense_normal_toeplitz(const float2* x, float2* out, const float2* smaps, co
     if (pcoils_host)
         for (int p = 0; p < P; ++p)
             HASTY_REQUIRE(pcoils_host[p] >= 1 && pcoils_host[p] <= ncoil, "per-problem coil count out of range");
-    if (layout == 1) {
+    if (layout == 1 || layout == 2) {
+        HASTY_REQUIRE(sense_fused_supported(n3), "fused kernel layout not available for these sizes");
         HASTY_REQUIRE(!pcoils_host || pcoils_dev, "the fused layout needs the device copy of the coil counts");
         const int cb = std::min(coil_batch, ncoil);
         const size_t per = sense_fused_workspace_bytes(n3, 1, cb);
         const int pchunk = (int)std::max<size_t>(1, std::min<size_t>(64, ws_bytes / per));
-        sense_fused_apply(x, out, smaps, coils_dev, ncoil, kernels, kidx, sub, P, n3, workspace, ws_bytes, pchunk, cb,
-                          s, coil_pstride, pcoils_host ? pcoils_dev : nullptr);
+        sense_fused_apply(x, out, smaps, coils_dev, ncoil, kernels, layout == 2, kidx, sub, P, n3, workspace, ws_bytes,
+                          pchunk, cb, s, coil_pstride, pcoils_host ? pcoils_dev : nullptr);
         return;
     }
     HASTY_REQUIRE(layout == 0, "unknown kernel layout");

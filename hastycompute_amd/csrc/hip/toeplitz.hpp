@@ -31,7 +31,10 @@ void toeplitz_apply_ref(const float2* in, float2* out, const float2* mul, const 
 // coils: device int array of ncoil coil indices. Workspace from
 // sense_normal_workspace_bytes.
 // Kernel layouts: 0 natural [kz][ky][kx] (generic five-pass path), 1 the fused path's
-// permuted [kz][s][ky][m] order (kx = P m + s, P = 2 N2 / 32).
+// coset order [kz][s][ky][m] (kx = P m + s, P = 2 N2 / 32) for the LDS and coset plane
+// kernels, 2 the fused path's register-kernel order at N1 = N2 = 128. A prepared kernel
+// keeps its layout, and the apply picks the plane kernels from it, so changing the
+// sense_xy_variant option later cannot mismatch kernels prepared before.
 int sense_kernel_layout(const int64_t* n3, bool allow_fused);
 void sense_prepare_kernels(const float* natural, float* prepared, int64_t nk, const int64_t* n3, int layout,
                            hipStream_t s);
@@ -65,10 +68,12 @@ void sense_fused_set_xy_variant(int v);
 void sense_fused_set_zconv_batch(int v);
 int sense_fused_get_xy_variant();
 int sense_fused_get_zconv_batch();
-void sense_fused_permute_kernel(const float* natural, float* permuted, const int64_t* n3, hipStream_t s);
+// true when the current sense_xy_variant gives the register kernels at this size
+bool sense_fused_register_layout(const int64_t* n3);
+void sense_fused_permute_kernel(const float* natural, float* permuted, const int64_t* n3, bool r16, hipStream_t s);
 size_t sense_fused_workspace_bytes(const int64_t* n3, int pchunk, int ccount);
 void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const int* coils_dev, int ncoil,
-                       const float* kernels, const int* kidx, const float2* sub, int P, const int64_t* n3, void* ws,
+                       const float* kernels, bool r16, const int* kidx, const float2* sub, int P, const int64_t* n3, void* ws,
                        size_t ws_bytes, int pchunk, int cchunk, hipStream_t s, int coil_pstride = 0,
                        const int* pcoils_dev = nullptr);
 

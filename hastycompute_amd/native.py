@@ -167,6 +167,7 @@ class SenseNormalToeplitz:
         self.problem_batch = max(1, problem_batch)
         ws = C.lib().hasty_sense_normal_workspace_size(C.i64arr(self.n3), self.coil_batch, self.problem_batch)
         self.workspace = torch.empty(ws, dtype=torch.uint8, device=smaps.device)
+        self._ncoil_dev = {}
 
     def _check_image(self, t: torch.Tensor, name: str, P: int):
         _cuda(t, name)
@@ -199,7 +200,10 @@ class SenseNormalToeplitz:
             return out
         nc = [self.ncoil] * P if problem_ncoil is None else [int(c) for c in problem_ncoil]
         _require(len(nc) == P and all(1 <= c <= self.ncoil for c in nc), "problem_ncoil must be in 1..ncoil")
-        nc_dev = torch.tensor(nc, dtype=torch.int32, device=x.device)
+        key = tuple(nc)
+        nc_dev = self._ncoil_dev.get(key)
+        if nc_dev is None:  # kept, so repeated calls (and graph captures) allocate nothing
+            nc_dev = self._ncoil_dev[key] = torch.tensor(nc, dtype=torch.int32, device=x.device)
         C.call("hasty_sense_normal_toeplitz_ex", x.data_ptr(), out.data_ptr(), self.smaps.data_ptr(),
                int(self.smaps.shape[0]), self.coils.data_ptr(), self.ncoil, int(coil_problem_stride), C.i32arr(nc),
                nc_dev.data_ptr(), self.kernels.data_ptr(), self.kernels.shape[0], self.layout, C.i32arr(kidx),
@@ -221,7 +225,13 @@ class DeviceCG:
         C.call("hasty_cg_init", self.P, self.n, b.data_ptr(), Ax.data_ptr(), r.data_ptr(), p.data_ptr(),
                self.state.data_ptr(), self.ws.data_ptr(), C.stream_ptr())
 
-    def iterate(self, x, r, p, Ap, tol: float = 0.0):
+    def iterate(self, x, r, p, Ap, tol: float = 0.0, l2: float = 0.0):
+        """One CG step given Ap = A p; with l2 > 0 the step is for A + l2 I
+        (hasty_cg_iterate_l2)."""
+        if l2:
+            C.call("hasty_cg_iterate_l2", self.P, self.n, x.data_ptr(), r.data_ptr(), p.data_ptr(), Ap.data_ptr(),
+                   self.state.data_ptr(), self.ws.data_ptr(), float(tol), float(l2), C.stream_ptr())
+            return
         C.call("hasty_cg_iterate", self.P, self.n, x.data_ptr(), r.data_ptr(), p.data_ptr(), Ap.data_ptr(),
                self.state.data_ptr(), self.ws.data_ptr(), float(tol), C.stream_ptr())
 
@@ -239,11 +249,58 @@ class DeviceCG:
                self.state.data_ptr(), self.ws.data_ptr(), C.stream_ptr())
 
 
+class Graph:
+    """A recorded sequence of library launches (hasty_graph_*): `record()` runs under a
+    capture of `stream` (a created stream; it becomes the current stream meanwhile), and
+    `replay()` launches the recorded graph on it. What `record` calls must not allocate
+    or synchronise: run it once eagerly on the same buffers first."""
+
+    def __init__(self, record, stream: Optional[torch.cuda.Stream] = None):
+        self.stream = stream if stream is not None else torch.cuda.Stream()
+        sp = self.stream.cuda_stream
+        self._h = None
+        with torch.cuda.stream(self.stream):
+            C.call("hasty_graph_begin", sp)
+            h = ctypes.c_void_p()
+            try:
+                record()
+            except BaseException:
+                if C.lib().hasty_graph_end(sp, ctypes.byref(h)) == 0 and h.value:
+                    C.lib().hasty_graph_destroy(h)
+                raise
+            C.call("hasty_graph_end", sp, ctypes.byref(h))
+        self._h = h
+
+    @property
+    def nodes(self) -> int:
+        return int(C.lib().hasty_graph_nodes(self._h))
+
+    def replay(self):
+        C.call("hasty_graph_launch", self._h, self.stream.cuda_stream)
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            C.call("hasty_graph_destroy", self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def cg_solve(apply_A, b: torch.Tensor, x0: torch.Tensor, iters: int, tol: float = 0.0, check_pd: bool = True,
-             apply_P=None):
+             apply_P=None, graph: bool = False, l2: float = 0.0):
     """ConjugateGradient::run (lib/op/opalgs.cpp:50-96) for a batch [P, ...] with the
     device CG; apply_A(v, out) writes A v into out; apply_P(v, out), if given, writes the
-    preconditioned P v (the _P branch). Returns (x, state)."""
+    preconditioned P v (the _P branch). With graph, the first iteration runs eagerly and
+    the later ones replay a HIP graph of one iteration (apply_A + the CG step), so
+    apply_A must not allocate. l2 > 0 solves with A + l2 I (l2_reg_linop,
+    python/python/hastypy/base/opalg.py:626-627) without a pass for l2 p. Returns
+    (x, state)."""
+    _require(l2 >= 0.0, "l2 must be >= 0")
+    _require(not (l2 and apply_P is not None), "l2 covers the unpreconditioned loop")
     P = b.shape[0]
     n = b[0].numel()
     x = x0.clone()
@@ -253,13 +310,42 @@ def cg_solve(apply_A, b: torch.Tensor, x0: torch.Tensor, iters: int, tol: float 
     cg = DeviceCG(P, n, b.device)
     if iters < 1:
         return x, cg.state
-    apply_A(x, Ap)
-    cg.init(b, Ap, r, p)
-    if apply_P is None:
+    if graph:
+        _require(apply_P is None, "graph replay covers the unpreconditioned loop")
+        main = torch.cuda.current_stream()
+        side = torch.cuda.Stream()
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            apply_A(x, Ap)
+            if l2:
+                Ap.add_(x, alpha=l2)
+            cg.init(b, Ap, r, p)
+            apply_A(p, Ap)
+            cg.iterate(x, r, p, Ap, tol, l2)
+
+            def one_iteration():
+                apply_A(p, Ap)
+                cg.iterate(x, r, p, Ap, tol, l2)
+
+            if iters > 1:
+                g = Graph(one_iteration, side)
+                for _ in range(iters - 1):
+                    g.replay()
+        main.wait_stream(side)
+        if iters > 1:
+            side.synchronize()
+            g.close()
+    elif apply_P is None:
+        apply_A(x, Ap)
+        if l2:
+            Ap.add_(x, alpha=l2)
+        cg.init(b, Ap, r, p)
         for _ in range(iters):
             apply_A(p, Ap)
-            cg.iterate(x, r, p, Ap, tol)
+            cg.iterate(x, r, p, Ap, tol, l2)
     else:
+        apply_A(x, Ap)
+        cg.init(b, Ap, r, p)
         z = torch.empty_like(b)
         apply_P(r, z)
         cg.init_precond(r, z, p)

@@ -89,16 +89,12 @@ class _Group:
         self.reqs: List[ReconRequest] = []
         self.ncoil_host: List[int] = []
         self.iters_done: List[int] = []
-        self.layout = 1 if self._fused_ok() else 0
+        self.layout = None  # the kernels' layout, fixed by the first prepared kernel
         self.coils_dev = torch.arange(cmax, dtype=torch.int32, device=device)
         ws = C.lib().hasty_sense_normal_workspace_size(C.i64arr(self.n3), self.coil_batch, problem_batch)
         self.ws = torch.empty(ws, dtype=torch.uint8, device=device)
         self.cap = 0
         self._alloc(capacity)
-
-    def _fused_ok(self):
-        n = self.n
-        return n % 32 == 0 and 32 <= n <= 128
 
     def _alloc(self, cap: int):
         n, dev = self.n, self.device
@@ -132,7 +128,9 @@ class _Group:
         layout = ctypes.c_int()
         C.call("hasty_sense_normal_prepare", kernel_nat.data_ptr(), self.kern[i].data_ptr(), 1, C.i64arr(self.n3),
                ctypes.byref(layout), C.stream_ptr())
-        if layout.value != self.layout:
+        if self.layout is None:
+            self.layout = layout.value
+        elif layout.value != self.layout:
             raise RuntimeError("kernel layout changed within a size group")
         self.b[i].copy_(b.reshape(self.n, self.n, self.n))
         self.x[i].zero_()

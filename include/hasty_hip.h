@@ -88,12 +88,15 @@ int hasty_toeplitz_apply(const void* in, void* out, const void* mul, const void*
  * [nk][2N0][2N1][2N2] as written by hasty_sense_normal_prepare with `layout`;
  * kernel_index: HOST int32 [P], kernel of each problem. coil_batch coils share one
  * oversampled-grid workspace; problem_batch problems go through each launch.
- * Layout 1 (N1 == N2 in {32, 64, 96, 128}) runs the fused three-kernel path; layout 0
- * the generic five-pass path.
+ * Layouts 1 and 2 (N1 == N2 in {32, 64, 96, 128}) run the fused three-kernel path
+ * (2: the register plane kernels at N1 = N2 = 128, 1: the LDS or coset plane kernels);
+ * layout 0 the generic five-pass path. The apply follows the kernels' layout, whatever
+ * the sense_xy_variant option is at that time.
  * ------------------------------------------------------------------------------- */
 size_t hasty_sense_normal_workspace_size(const int64_t* nmodes3, int coil_batch, int problem_batch);
 /* Convert nk kernels from hasty_toeplitz_build's natural layout into the operator's
- * layout (*layout receives 0 or 1; prepared may equal natural only for layout 0). */
+ * layout (*layout receives 0, 1 or 2, chosen from the sense_fused and sense_xy_variant
+ * options at this call; prepared may equal natural only for layout 0). */
 int hasty_sense_normal_prepare(const float* natural, float* prepared, int64_t nk, const int64_t* nmodes3,
                                int* layout, void* stream);
 int hasty_sense_normal_toeplitz(const void* x, void* out, const void* smaps, const int32_t* coils, int ncoil,
@@ -132,6 +135,32 @@ int hasty_set_option(const char* name, int64_t value);
 /* Current value of an option of hasty_set_option (0, or 1 for an unknown name). */
 int hasty_get_option(const char* name, int64_t* value);
 
+/* Autotuner (no counterpart in the reference, whose launch shapes are fixed inside
+ * cuFINUFFT, cuFFT and cuSOLVER; it replaces the hand-set options above for a given
+ * shape). Candidates are timed with HIP events on `stream` in a fixed coordinate-wise
+ * order starting from the defaults; a candidate must be 2 % faster to replace the
+ * incumbent. Results are cached per shape for the process. None of these may run
+ * while `stream` is being captured.
+ * hasty_autotune_sense_normal: the batched Toeplitz SENSE normal operator at image size
+ *   nmodes3 with ncoil coils and nproblems problems (at most 8 timed), on synthetic
+ *   data: plane kernels, problems per launch, problems per z-convolution launch and
+ *   coils per workspace. Sets sense_xy_variant and sense_zconv_batch to the winners
+ *   (prepare kernels after this call) and writes choice[4] = {sense_xy_variant,
+ *   sense_zconv_batch, problem_batch, coil_batch} (size the workspace with the last two)
+ *   and the time per problem-apply.
+ * hasty_nufft_autotune: the 3-D type-1 spreaders (nufft_spread_variant 0..3) of `plan`
+ *   on its own points, executing c -> f (f is overwritten); the plan keeps the fastest.
+ *   Other plans keep their spreader. *spread_variant receives the choice, *ms the
+ *   execute time.
+ * hasty_autotune_llr: the Gram row split of hasty_llr_normal_blocks for this image and
+ *   block grid (threshold 0 on synthetic data); later calls with the same number of
+ *   blocks per pass, rows and frames use it (size workspaces after this call). */
+int hasty_autotune_sense_normal(const int64_t* nmodes3, int ncoil, int nproblems, int64_t* choice,
+                                double* ms_per_problem, void* stream);
+int hasty_nufft_autotune(hasty_nufft_plan plan, void* c, void* f, int64_t* spread_variant, double* ms, void* stream);
+int hasty_autotune_llr(const int64_t* dims5, const int64_t* block_strides, const int64_t* block_shape,
+                       int64_t* nsplit, double* ms, void* stream);
+
 /* Kernel timing (no counterpart in the reference; a measurement hook for benchmarks).
  * While enabled, each instrumented launch is bracketed by two HIP events on its own
  * stream, and the durations are summed per kernel name. hasty_timing_enable clears the
@@ -141,6 +170,30 @@ int hasty_get_option(const char* name, int64_t* value);
 int hasty_timing_enable(int on);
 int hasty_timing_entries(void);
 int hasty_timing_read(int index, char* name, int name_capacity, double* total_ms, int64_t* count);
+
+/* ---------------------------------------------------------------------------------
+ * HIP graphs of library launch sequences. The reference drives each frame's operator
+ * from a host worker thread per device context (ContextThreadPool,
+ * lib/threading/thread_pool.ixx:110-259, fanned out by batch_sense.cpp:349-450 and
+ * BatchedConjugateGradient, lib/op/opalgs.ixx:84-136); here a whole batched iteration
+ * (normal operator over every problem + the device CG step) is recorded once and
+ * replayed as one graph launch.
+ * hasty_graph_begin starts capturing `stream` (a created stream, not NULL; thread-local
+ * capture mode); every library call made on that stream by this thread until
+ * hasty_graph_end is recorded instead of run. The recorded calls must not allocate or
+ * synchronise: call each entry point once on the same shapes before capturing (that
+ * creates its cached tables), and keep every buffer alive and unmoved while the graph
+ * exists; host arrays (kernel_index, coil counts) are read at capture time.
+ * Recorded launches are not timed (hasty_timing_*). hasty_graph_launch replays the
+ * sequence on `stream`; hasty_graph_nodes returns its number of nodes (-1 for NULL).
+ * ------------------------------------------------------------------------------- */
+typedef struct hasty_graph_s* hasty_graph;
+
+int hasty_graph_begin(void* stream);
+int hasty_graph_end(void* stream, hasty_graph* graph);
+int hasty_graph_launch(hasty_graph graph, void* stream);
+int64_t hasty_graph_nodes(hasty_graph graph);
+int hasty_graph_destroy(hasty_graph graph);
 
 /* ---------------------------------------------------------------------------------
  * Device-side batched conjugate gradient, the loop body of ConjugateGradient::run
@@ -158,6 +211,13 @@ int hasty_cg_init(int64_t nproblems, int64_t n, const void* b, const void* Ax, v
                   void* workspace, void* stream);
 int hasty_cg_iterate(int64_t nproblems, int64_t n, void* x, void* r, void* p, const void* Ap, double* state,
                      void* workspace, double tol, void* stream);
+/* The iteration of ConjugateGradient::run on l2_reg_linop(A, lambda) = A + lambda I
+ * (python/python/hastypy/base/opalg.py:626-627, the reference's Tikhonov-regularised
+ * normal operator): the caller passes Ap = A p and lambda p is added inside the CG
+ * kernels, so the regularisation costs no pass over the data. Initialise with
+ * hasty_cg_init(b, A x0 + lambda x0) (for x0 = 0, A x0 alone). lambda >= 0. */
+int hasty_cg_iterate_l2(int64_t nproblems, int64_t n, void* x, void* r, void* p, const void* Ap, double* state,
+                        void* workspace, double tol, double lambda, void* stream);
 /* The preconditioned loop of ConjugateGradient::run (the _P branch, opalgs.cpp:56-90),
  * z = P r computed by the caller between the calls:
  *   hasty_cg_init(b, A x0) -> r; z = P r; hasty_cg_init_precond(r, z) -> p = z, rzold = <r, z>;

@@ -1,0 +1,92 @@
+"""The launch-shape autotuner (hasty_autotune_sense_normal, hasty_nufft_autotune,
+hasty_autotune_llr): choices inside their ranges, cached per shape, and results with the
+tuned shapes equal to the default shapes' within floating-point reordering."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def N():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from hastycompute_amd import native
+
+    return native
+
+
+@pytest.fixture()
+def restore_options():
+    from hastycompute_amd import _capi as C
+
+    names = ("sense_xy_variant", "sense_zconv_batch", "nufft_spread_variant")
+    saved = {k: C.get_option(k) for k in names}
+    yield
+    for k, v in saved.items():
+        C.set_option(k, v)
+
+
+@pytest.mark.parametrize("shape,C", [((32, 64, 64), 4), ((128, 32, 32), 3)])
+def test_autotune_sense_normal(N, restore_options, shape, C):
+    from hastycompute_amd import _capi as Cap
+    from hastycompute_amd import autotune
+
+    rng = np.random.default_rng(1)
+    P = 6
+    co = torch.as_tensor(rng.uniform(-np.pi, np.pi, (3, 3000)).astype(np.float32)).cuda()
+    _, k = N.toeplitz_build(co, list(shape), diag=False)
+    S = torch.randn((C,) + shape, dtype=torch.complex64, device="cuda")
+    x = torch.randn((P,) + shape, dtype=torch.complex64, device="cuda")
+    ref = N.SenseNormalToeplitz(S, k, coil_batch=C, problem_batch=1).apply(x)
+    t = autotune.sense_normal(shape, C, P)
+    print(f"autotune {shape} x {C} coils: {t}", flush=True)
+    assert t["sense_xy_variant"] in (0, 1, 2, 3) and 1 <= t["sense_zconv_batch"] <= 16
+    assert 1 <= t["problem_batch"] <= min(P, 8) and 1 <= t["coil_batch"] <= C
+    assert t["ms_per_problem"] > 0
+    assert Cap.get_option("sense_xy_variant") == t["sense_xy_variant"]
+    assert autotune.sense_normal(shape, C, P) == t  # cached
+    op = N.SenseNormalToeplitz(S, k, coil_batch=t["coil_batch"], problem_batch=t["problem_batch"])
+    out = op.apply(x)
+    torch.cuda.synchronize()
+    e = O.rel_l2(out.cpu().numpy(), ref.cpu().numpy())
+    assert e < 1e-5, e
+
+
+def test_autotune_nufft_type1(N, restore_options):
+    from hastycompute_amd import autotune
+
+    n = 32
+    co = torch.as_tensor(O.kooshball(300, 64).astype(np.float32)).cuda()
+    M = co.shape[1]
+    c = torch.randn((1, M), dtype=torch.complex64, device="cuda")
+    plan = N.NufftPlan(co, [n, n, n], 1, 1, 1, 1e-6)
+    f0 = torch.empty((1, n, n, n), dtype=torch.complex64, device="cuda")
+    plan.execute(c, f0)
+    f = torch.empty_like(f0)
+    t = autotune.nufft(plan, c, f)
+    print(f"autotune nufft: {t}", flush=True)
+    assert t["nufft_spread_variant"] in (0, 1, 2, 3) and t["execute_ms"] > 0
+    plan.execute(c, f)
+    torch.cuda.synchronize()
+    assert O.rel_l2(f.cpu().numpy(), f0.cpu().numpy()) < 1e-5
+    plan.close()
+
+
+def test_autotune_llr_row_split(N):
+    from hastycompute_amd import autotune
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    dims = (20, 1, 48, 48, 48)
+    x = torch.randn(dims, dtype=torch.complex64, device="cuda", generator=g)
+    before = N.llr_normal_blocks(x.clone(), (16, 16, 16), (16, 16, 16), 1, 2.0, True)
+    t = autotune.llr(dims, (16, 16, 16), (16, 16, 16))
+    print(f"autotune llr: {t}", flush=True)
+    assert 1 <= t["llr_row_split"] <= 8
+    after = N.llr_normal_blocks(x.clone(), (16, 16, 16), (16, 16, 16), 1, 2.0, True)
+    torch.cuda.synchronize()
+    assert O.rel_l2(after.cpu().numpy(), before.cpu().numpy()) < 1e-4

@@ -230,14 +230,17 @@ def test_sense_normal_cfg1_batched_problems(N):
         assert e < 1e-5
 
 
-@pytest.mark.parametrize("traj", ["uniform", "kooshball"])
-def test_device_cg_vs_oracle_cg(N, traj):
+@pytest.mark.parametrize("traj,l2", [("uniform", 0.0), ("kooshball", 0.0), ("kooshball", 0.1)])
+def test_device_cg_vs_oracle_cg(N, traj, l2):
     # Uniform-density points give a well-conditioned system, where FP32 and FP64 CG
-    # iterates agree to ~1e-7, so the image is held to 1e-5. A kooshball samples only
-    # the ball |k| < pi of the cube, so its normal operator is near-singular (corner
-    # frequencies unsampled) and 10 CG steps amplify FP32 rounding: an all-FP32 CG lands
-    # 5-18% away from the FP64 CG. There the device CG must reach the FP64 CG's energy
-    # phi(x) = x^H A x / 2 - Re b^H x and image error within 20%.
+    # iterates agree to ~1e-7, so the image is held to 1e-5. On a kooshball the later CG
+    # iterates are ill-conditioned in their inputs: relative perturbations of the
+    # operator's outputs at the size of the device operator's own error (~1e-7) move the
+    # FP64 CG's 10th iterate by 2-6 %. There the device CG is held to that envelope: the
+    # largest distance (and phantom error) the oracle's FP64 CG reaches over three seeds
+    # of Gaussian output noise at 3x the device operator's measured error. With the
+    # reference's l2_reg_linop (python/python/hastypy/base/opalg.py:626-627) at
+    # 0.1 lambda_max the system is well posed and the image is held to 1e-5 again.
     rng = np.random.default_rng(5)
     n, C, P, iters = 16, 4, 2, 10
     S = O.coil_maps(C, n, seed=4)
@@ -258,26 +261,48 @@ def test_device_cg_vs_oracle_cg(N, traj):
     else:
         b = np.concatenate([Arefs[p](xt[p:p + 1]) for p in range(P)])
     x0 = np.zeros_like(b)
+    lam = 0.0
+    if l2:
+        v = rnd(np.random.default_rng(1), (1, n, n, n))
+        for _ in range(40):
+            w = Arefs[0](v)
+            lmax = np.linalg.norm(w)
+            v = w / lmax
+        lam = l2 * lmax
 
     def A(v, out):
         op.apply(v, out=out, kernel_index=[0, 1])
 
-    x, state = N.cg_solve(A, g(b), g(x0), iters)
+    x, state = N.cg_solve(A, g(b), g(x0), iters, l2=lam)
     xs = h(x)
+    Ab = h(op.apply(g(b), kernel_index=[0, 1]))
     for p in range(P):
-        Aref = Arefs[p]
+        Aref = (lambda v, p=p: Arefs[p](v) + lam * v)
         xr = O.conjugate_gradient(Aref, b[p:p + 1], x0[p:p + 1], iters)
         e = O.rel_l2(xs[p:p + 1], xr)
-        phi = lambda v: 0.5 * np.vdot(v, Aref(v)).real - np.vdot(b[p:p + 1], v).real
-        print(f"CG {traj} problem {p}: image vs FP64 CG {e:.2e}, energy {phi(xs[p:p + 1]):.6e} vs "
-              f"{phi(xr):.6e}", flush=True)
-        if traj == "uniform":
+        print(f"CG {traj} l2={l2} problem {p}: image vs FP64 CG {e:.2e}", flush=True)
+        if traj == "uniform" or l2:
             assert e < 1e-5
         else:
-            assert phi(xs[p:p + 1]) <= 0.8 * phi(xr)
+            e_op = O.rel_l2(Ab[p:p + 1], Arefs[p](b[p:p + 1]))
+            env_d, env_ph = 0.0, 0.0
+            for seed in range(3):
+                r = np.random.default_rng(seed)
+
+                def Anoisy(v, r=r, Aref=Aref):
+                    y = Aref(v)
+                    return y + 3 * e_op * np.linalg.norm(y) / math.sqrt(y.size) * (
+                        r.standard_normal(y.shape) + 1j * r.standard_normal(y.shape)) / math.sqrt(2)
+
+                xn = O.conjugate_gradient(Anoisy, b[p:p + 1], x0[p:p + 1], iters)
+                env_d = max(env_d, O.rel_l2(xn, xr))
+                env_ph = max(env_ph, O.rel_l2(xn, xt[p:p + 1]))
             eg, er = O.rel_l2(xs[p:p + 1], xt[p:p + 1]), O.rel_l2(xr, xt[p:p + 1])
-            print(f"   image error vs phantom: device {eg:.4f}, FP64 CG {er:.4f}", flush=True)
-            assert eg <= 1.2 * er
+            print(f"   device operator error {e_op:.2e}; FP64 CG under 3x that output noise: up to {env_d:.2e} "
+                  f"away, phantom error up to {env_ph:.4f}; device phantom error {eg:.4f}, FP64 CG {er:.4f}",
+                  flush=True)
+            assert e <= max(1.25 * env_d, 1e-5)
+            assert eg <= 1.02 * max(env_ph, er)
     st = state.cpu().numpy()
     assert np.all(st[:, 6] == iters) and np.all(st[:, 4] == 0)
 

@@ -4,10 +4,20 @@ oracle of the build's path (FP32 data, FP64 Toeplitz-kernel build, FP64 CG scala
 of an all-FP32 variant (FP32 PSF NUFFT for the kernel, FP32 CG scalars on the host).
 
 Bounds: NUFFT type-1/type-2 1e-5; normal operator 1e-5 (the FP64-built kernel must also
-beat the FP32-built one); CG image after 15 iterations: reported, and the FP64-scalar
-path must not be worse than the all-FP32 variant (the kooshball leaves the cube's
-corners unsampled, so CG amplifies any FP32 rounding there; the error of both variants
-is printed for the record).
+beat the FP32-built one).
+
+CG image after 15 iterations (ConjugateGradient::run, lib/op/opalgs.cpp:50-96):
+- On the raw kooshball system the 15th iterate is ill-conditioned in its inputs: the
+  oracle's own FP64 CG moves by ~3.5e-5 when b is scaled by (1 + 1e-15), and any
+  perturbation of 1e-10 or more (complex64 operator outputs, complex64 vectors, FP32
+  dot products, FP64 with 1e-7 noise) lands ~3.3e-2 away, the same for every kind. The
+  device CG is therefore held to the envelope of the oracle's FP64 CG under such
+  perturbations (complex64-rounded operator inputs/outputs, and Gaussian output noise at
+  3x the device operator's measured error, three seeds): at most 1.25x the largest
+  distance, and a phantom error within 2 % of the largest.
+- On the reference's Tikhonov-regularised operator l2_reg_linop(A, lambda) = A + lambda I
+  (python/python/hastypy/base/opalg.py:626-627) with lambda = 0.1 lambda_max the system
+  is well posed and the device CG image is held to 1e-5 of the FP64 CG.
 """
 import math
 
@@ -81,36 +91,94 @@ def test_precision_study(setup):
     rows += [("normal op", en64, en32)]
     print(f"precision: normal op done ({en64:.2e}, {en32:.2e})", flush=True)
 
-    # CG 15 iterations on b = A^H (A x_true + noise)
+    print("\nprecision vs FP64 oracle (relative L2): step | this path | all-FP32 variant", flush=True)
+    for name, a, b_ in rows:
+        print(f"  {name:18s} {a:.3e}  {'' if b_ is None else f'{b_:.3e}'}", flush=True)
+    assert e1 < 1e-5 and e2 < 1e-5
+    assert en64 < 1e-5 and en64 <= en32
+
+
+@pytest.fixture(scope="module")
+def cg_setup(setup):
+    """b = A^H (A x_true + noise) (sigma = 1e-3 per sample) with exact NUFFTs, the
+    oracle operator, and the device operator with the FP64-built kernel."""
+    N = setup["native"]
+    n, C, co32, cod, S, x_true = (setup[k] for k in ("n", "C", "co32", "cod", "S", "x_true"))
+    dev = torch.device("cuda:0")
+    cog = torch.as_tensor(co32).to(dev)
+    nvox = n ** 3
+    rng = np.random.default_rng(4)
     noise = 1e-3 * (rng.standard_normal((C, cod.shape[1])) + 1j * rng.standard_normal((C, cod.shape[1])))
     y = np.stack([O.nufft_type2(cod, (x_true[0] * S[ci]), -1, 1e-12) / math.sqrt(nvox) for ci in range(C)]) + noise
     b = sum(np.conj(S[ci])[None] * O.nufft_type1(cod, y[ci], [n] * 3, 1, 1e-12)[None] / math.sqrt(nvox)
             for ci in range(C))
-    ref_cg = O.conjugate_gradient(lambda v: O.sense_normal_toeplitz(diag, v, S, list(range(C)),
-                                                                    workers=O.default_threads()),
-                                  b, np.zeros_like(b), 15)
+    diag = O.toeplitz_diagonal(cod, [1, n, n, n])
+    Aref = lambda v: O.sense_normal_toeplitz(diag, v, S, list(range(C)), workers=O.default_threads())
+    _, kern64 = N.toeplitz_build(cog, [n] * 3, tol=1e-8, diag=False)
+    op = N.SenseNormalToeplitz(torch.as_tensor(S).to(torch.complex64).to(dev), kern64[None].contiguous())
     bg = torch.as_tensor(b).to(torch.complex64).to(dev)
-    x64, _ = N.cg_solve(lambda v, o: op64.apply(v, out=o), bg, torch.zeros_like(bg), 15, check_pd=False)
-    # all-FP32 variant: FP32 kernel, FP32 scalars read back every iteration
+    return dict(N=N, b=b, bg=bg, Aref=Aref, op=op, x_true=x_true, cog=cog, S=S)
+
+
+def test_precision_cg_raw_kooshball(cg_setup):
+    N, b, bg, Aref, op, x_true = (cg_setup[k] for k in ("N", "b", "bg", "Aref", "op", "x_true"))
+    c64 = lambda a: a.astype(np.complex64).astype(np.complex128)
+    ref = O.conjugate_gradient(Aref, b, np.zeros_like(b), 15)
+    ulp = O.conjugate_gradient(Aref, b * (1 + 1e-15), np.zeros_like(b), 15) / (1 + 1e-15)
+    rounded = O.conjugate_gradient(lambda v: c64(Aref(c64(v))), b, np.zeros_like(b), 15)
+    e_op = O.rel_l2(op.apply(bg).cpu().numpy(), Aref(b))
+    noisy = []
+    for seed in range(3):
+        r = np.random.default_rng(seed)
+
+        def Anoisy(v, r=r):
+            y = Aref(v)
+            return y + 3 * e_op * np.linalg.norm(y) / math.sqrt(y.size) * (
+                r.standard_normal(y.shape) + 1j * r.standard_normal(y.shape)) / math.sqrt(2)
+
+        noisy.append(O.conjugate_gradient(Anoisy, b, np.zeros_like(b), 15))
+    x, st = N.cg_solve(lambda v, o: op.apply(v, out=o), bg, torch.zeros_like(bg), 15)
+    xd = x.cpu().numpy()
+    # all-FP32 variant: FP32 scalars read back every iteration
     x32 = torch.zeros_like(bg)
-    r = bg - op32.apply(x32)
+    r = bg - op.apply(x32)
     p = r.clone()
     rz = torch.vdot(r.flatten(), r.flatten()).real
     for _ in range(15):
-        Ap = op32.apply(p)
+        Ap = op.apply(p)
         alpha = rz / torch.vdot(p.flatten(), Ap.flatten()).real
         x32 += alpha * p
         r -= alpha * Ap
         rz_new = torch.vdot(r.flatten(), r.flatten()).real
         p = r + (rz_new / rz) * p
         rz = rz_new
-    ec64 = rel(x64.cpu().numpy(), ref_cg)
-    ec32 = rel(x32.cpu().numpy(), ref_cg)
-    rows += [("CG image (15 it)", ec64, ec32)]
+    e_dev, e_32 = O.rel_l2(xd, ref), O.rel_l2(x32.cpu().numpy(), ref)
+    e_ulp, e_rnd = O.rel_l2(ulp, ref), O.rel_l2(rounded, ref)
+    e_noise = [O.rel_l2(xn, ref) for xn in noisy]
+    env_d = max([e_rnd] + e_noise)
+    env_ph = max(O.rel_l2(xn, x_true) for xn in [ref, rounded] + noisy)
+    ph_dev, ph_ref = O.rel_l2(xd, x_true), O.rel_l2(ref, x_true)
+    print(f"\nCG 15 it, raw kooshball, vs FP64 CG: device {e_dev:.3e}, all-FP32 {e_32:.3e}; oracle FP64 CG with "
+          f"b*(1+1e-15) {e_ulp:.3e}, with complex64-rounded operator {e_rnd:.3e}, with output noise at 3x the "
+          f"device operator error ({e_op:.2e}) {', '.join(f'{v:.3e}' for v in e_noise)}; phantom error device "
+          f"{ph_dev:.4f}, FP64 CG {ph_ref:.4f}, envelope {env_ph:.4f}", flush=True)
+    assert e_dev <= 1.25 * env_d
+    assert ph_dev <= 1.02 * env_ph
 
-    print("\nprecision vs FP64 oracle (relative L2): step | this path | all-FP32 variant", flush=True)
-    for name, a, b_ in rows:
-        print(f"  {name:18s} {a:.3e}  {'' if b_ is None else f'{b_:.3e}'}", flush=True)
-    assert e1 < 1e-5 and e2 < 1e-5
-    assert en64 < 1e-5 and en64 <= en32
-    assert ec64 <= ec32 * 1.05
+
+def test_precision_cg_l2_regularised(cg_setup):
+    N, b, bg, Aref, op = (cg_setup[k] for k in ("N", "b", "bg", "Aref", "op"))
+    # lambda_max by power iteration on the device operator; the same lambda on both sides
+    v = bg.clone()
+    for _ in range(30):
+        w = op.apply(v)
+        lmax = float(torch.linalg.vector_norm(w))
+        v = w / lmax
+    lam = 0.1 * lmax
+    ref = O.conjugate_gradient(lambda u: Aref(u) + lam * u, b, np.zeros_like(b), 15)
+    x, st = N.cg_solve(lambda u, o: op.apply(u, out=o), bg, torch.zeros_like(bg), 15, l2=lam)
+    xg, _ = N.cg_solve(lambda u, o: op.apply(u, out=o), bg, torch.zeros_like(bg), 15, l2=lam, graph=True)
+    e = O.rel_l2(x.cpu().numpy(), ref)
+    print(f"\nCG 15 it on A + {lam:.4g} I (0.1 lambda_max): device vs FP64 CG {e:.3e}", flush=True)
+    assert e < 1e-5
+    assert torch.equal(x, xg)
