@@ -64,6 +64,7 @@ def parse():
     ap.add_argument("--cg-iters", type=int, default=15)
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--plain", type=int, default=3, help="problems solved by the plain per-frame path (0: skip)")
+    ap.add_argument("--plain-tol", type=float, default=1e-6, help="NUFFT tolerance of the plain per-frame path")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU baseline: repeat whole applies until this many seconds have passed")
@@ -407,14 +408,16 @@ def percentile(v, q):
 
 
 def plain_recons(args, torch, spokes, offsets, smaps, b_all, x_batched, toeplitz_apply, lam=0.0):
-    """The plain per-frame drop-in: for each sampled problem, the explicit NUFFT
-    SenseNormal (HastySense.SenseNormal, default NUFFT options) inside the host CG of
+    """The plain per-frame drop-in: for each sampled problem, the reference's explicit
+    SENSE normal operator (the coil loop of sense_normal_apply, lib/mri/sense.ixx:533-577,
+    over CUDANufftNormal, lib/fft/nufft.cpp:862-880, through HastyNufft.NufftNormal with
+    type-2 / type-1 options at args.plain_tol) plus lam I, inside the host CG of
     ConjugateGradient::run (opalgs.cpp:50-96), one problem at a time. Also returns, per
-    problem, the relative difference of one apply (explicit vs Toeplitz operator) and
-    the relative residuals |T x - b| / |b| of both CG results under the Toeplitz
-    operator T."""
+    problem, the relative difference of one apply (explicit vs Toeplitz operator), the
+    relative residuals |T x - b| / |b| of both CG results under the Toeplitz operator T,
+    and the images."""
     from hastycompute_amd import synthetic
-    from hastycompute_amd.ffi import hasty_sense
+    from hastycompute_amd.ffi import hasty_nufft
 
     N = args.n
     coils = list(range(args.coils))
@@ -425,13 +428,18 @@ def plain_recons(args, torch, spokes, offsets, smaps, b_all, x_batched, toeplitz
         co = synthetic.kooshball_torch(int(spokes[f]), args.readout, int(offsets[f]), smaps.device)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        op = hasty_sense.SenseNormal(co, [1, N, N, N])
+        op = hasty_nufft.NufftNormal(co, [1, N, N, N], hasty_nufft.NufftOptions(2, False, args.plain_tol),
+                                     hasty_nufft.NufftOptions(1, True, args.plain_tol))
         im = torch.empty((1, N, N, N), dtype=torch.complex64, device=smaps.device)
+        res = torch.empty_like(im)
         ks = torch.empty((1, co.shape[1]), dtype=torch.complex64, device=smaps.device)
 
         def A(v):  # l2_reg_linop(SenseNormal, lam) (python/python/hastypy/base/opalg.py:626-627)
-            out = torch.empty_like(v)
-            op.apply(v, out, smaps, coils, im, ks)
+            out = torch.zeros_like(v)
+            for c in coils:
+                torch.mul(v, smaps[c], out=im)
+                op.apply(im, res, ks)
+                out.addcmul_(res, smaps[c].conj())
             if lam:
                 out.add_(v, alpha=lam)
             return out

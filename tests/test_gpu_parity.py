@@ -230,7 +230,7 @@ def test_sense_normal_cfg1_batched_problems(N):
         assert e < 1e-5
 
 
-@pytest.mark.parametrize("traj,l2", [("uniform", 0.0), ("kooshball", 0.0), ("kooshball", 0.1)])
+@pytest.mark.parametrize("traj,l2", [("uniform", 0.0), ("kooshball", 0.0), ("kooshball", 0.3)])
 def test_device_cg_vs_oracle_cg(N, traj, l2):
     # Uniform-density points give a well-conditioned system, where FP32 and FP64 CG
     # iterates agree to ~1e-7, so the image is held to 1e-5. On a kooshball the later CG
@@ -240,7 +240,9 @@ def test_device_cg_vs_oracle_cg(N, traj, l2):
     # largest distance (and phantom error) the oracle's FP64 CG reaches over three seeds
     # of Gaussian output noise at 3x the device operator's measured error. With the
     # reference's l2_reg_linop (python/python/hastypy/base/opalg.py:626-627) at
-    # 0.1 lambda_max the system is well posed and the image is held to 1e-5 again.
+    # 0.3 lambda_max the system is well posed (at this size 0.1 lambda_max still leaves
+    # the FP64 CG 5e-5 sensitive to 1e-7 output noise; 0.3: 5e-7) and the image is held
+    # to 1e-5 again.
     rng = np.random.default_rng(5)
     n, C, P, iters = 16, 4, 2, 10
     S = O.coil_maps(C, n, seed=4)
