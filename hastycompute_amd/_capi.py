@@ -77,6 +77,7 @@ _SIGS = {
     "hasty_llr_blocks_workspace_size": ([P_i64, P_i64, c_i64], c_sz),
     "hasty_svt_workspace_size": ([c_i64, c_i64], c_sz),
     "hasty_svt_matrix": ([c_vp, c_i64, c_i64, c_dbl, c_int, c_vp, c_sz, c_vp], c_int),
+    "hasty_svt_matrix_noalloc": ([c_vp, c_i64, c_i64, c_dbl, c_int, c_vp], c_int),
 }
 
 ERROR_NAMES = {1: "invalid argument", 2: "HIP runtime error", 3: "not positive definite", 4: "unsupported",

@@ -147,6 +147,36 @@ void check_dims5(const int64_t* dims5, const int64_t* shape)
     llr_check(dims5[0]);
 }
 
+// Scratch of hasty_svt_matrix_noalloc: the corner of its one block (first 256 B), then
+// the Gram partial sums, W and the recorded Jacobi rotations of a block of up to 128
+// columns (3.13 MB at 128 columns and 8 row splits). One copy per GPU, alive until the
+// process ends; host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device (the symbol itself is a host-side stand-in).
+constexpr size_t kSvtScratchBytes = size_t(4) << 20;
+__device__ __align__(256) unsigned char scratch[kSvtScratchBytes];
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;  // device ordinal -> device address of `scratch`
+
+// Address of `scratch` on the current device (looked up once per device).
+void* scratch_address()
+{
+    const int dev = current_device();
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) return it->second;
+    void* p = nullptr;
+    HASTY_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    HASTY_REQUIRE(p != nullptr, "no device address for the static scratch");
+    g_scratch_ptr[dev] = p;
+    return p;
+}
+
+__global__ void zero_corner_kernel(int3* c)
+{
+    if (threadIdx.x == 0) *c = make_int3(0, 0, 0);
+}
+
 }  // namespace
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
@@ -612,6 +642,41 @@ int hasty_svt_matrix(void* A, int64_t R, int64_t F, double thresh, int soft, voi
         g.rstride = F;
         std::vector<int32_t> c = {0, 0, 0};
         run_llr_batch((float2*)A, g, c, (float)thresh, soft, workspace, workspace_bytes, as_stream(stream));
+    });
+}
+
+int hasty_svt_matrix_noalloc(void* A, int64_t R, int64_t F, double thresh, int soft, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(A, "null argument");
+        HASTY_REQUIRE(R >= 1, "matrix must have rows");
+        HASTY_REQUIRE(thresh >= 0.0, "threshold must be >= 0");
+        llr_check(F);
+        // the matrix as a one-block image, as in hasty_svt_matrix
+        LlrGeom g;
+        g.F = F;
+        g.E = 1;
+        g.X = 1;
+        g.Y = 1;
+        g.Z = R;
+        g.bx = 1;
+        g.by = 1;
+        g.bz = (int)R;
+        g.fstride = 1;
+        g.rstride = F;
+        const int nsplit = choose_nsplit(1, R, F);
+        const size_t need = 256 + llr_workspace_bytes(1, F, nsplit);
+        HASTY_REQUIRE(need <= kSvtScratchBytes, "hasty_svt_matrix_noalloc: the block needs " + std::to_string(need) +
+                                                    " bytes of scratch, more than the static " +
+                                                    std::to_string(kSvtScratchBytes));
+        hipStream_t s = as_stream(stream);
+        unsigned char* ws = static_cast<unsigned char*>(scratch_address());
+        int3* corner = reinterpret_cast<int3*>(ws);
+        hipLaunchKernelGGL(zero_corner_kernel, dim3(1), dim3(64), 0, s, corner);
+        check_launch("zero_corner_kernel");
+        const int rank = soft ? 0 : (int)(thresh + 0.5);
+        llr_blocks((float2*)A, g, corner, 1, (float)thresh, soft, rank, ws + 256, kSvtScratchBytes - 256, nsplit,
+                   nullptr, s);
     });
 }
 

@@ -300,6 +300,18 @@ size_t hasty_svt_workspace_size(int64_t R, int64_t F);
 int hasty_svt_matrix(void* A, int64_t R, int64_t F, double thresh, int soft, void* workspace, size_t workspace_bytes,
                      void* stream);
 
+/* hasty_svt_matrix without a workspace argument and without allocating: no
+ * counterpart in the reference; for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * result as hasty_svt_matrix (same kernels). Its scratch (Gram partial sums, W,
+ * recorded rotations) is a static 4 MiB device array per GPU, enough for any R and
+ * F <= 128 (3.13 MB at F = 128); larger needs are rejected with an error. Nothing in
+ * it needs to start at zero (every part is written before it is read), and no host
+ * copy or synchronisation is made. All calls on one GPU share that array: calls that
+ * overlap on one GPU (other streams, other threads) race and must be serialised by the
+ * caller. */
+int hasty_svt_matrix_noalloc(void* A, int64_t R, int64_t F, double thresh, int soft, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
