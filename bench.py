@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--generic", action="store_true", help="use the generic five-pass path")
     ap.add_argument("--xy-variant", type=int, default=None,
                     help="plane kernels of the fused path (library option sense_xy_variant)")
+    ap.add_argument("--zconv-variant", type=int, default=None,
+                    help="z-convolution kernel at N0 = 128 (library option sense_zconv_variant, 0..3)")
     ap.add_argument("--zconv-batch", type=int, default=None,
                     help="problems per z-convolution launch at N0 = 128 (library option sense_zconv_batch, 1..16)")
     ap.add_argument("--cg-iters", type=int, default=15)
@@ -475,13 +477,50 @@ def plain_recons(args, torch, spokes, offsets, smaps, b_all, x_batched, toeplitz
 
 
 def copy_calibration(torch):
+    """The streaming ceiling: hasty_stream_probe over 1 GiB streams for the read:write
+    mixes of the hot kernels (1:0 like the inverse plane kernel, 1:1 like the
+    z-convolution, 1:4 like the forward plane kernel, and others), plain and
+    non-temporal, best of 5 event-timed runs; then a 1 GiB torch copy, the known-size
+    kernel the PMC byte counters are calibrated against."""
+    import ctypes
+
+    from hastycompute_amd import _capi as C
+
+    GiB = 1 << 30
+    bufs = [torch.full((GiB // 4,), 1.0, dtype=torch.float32, device="cuda") for _ in range(5)]
+    ptr = [b.data_ptr() for b in bufs]
+    stream = torch.cuda.current_stream()
+    rows = []
+    for nr, nw in ((1, 0), (2, 0), (0, 1), (1, 1), (2, 1), (1, 2), (1, 4), (4, 1)):
+        for nt in (0, 1):
+            src = (ctypes.c_void_p * 4)(*(ptr[:nr] + [None] * (4 - nr)))
+            dst_list = ptr[nr:nr + nw] if nw else [ptr[4]]
+            dst = (ctypes.c_void_p * 4)(*(dst_list + [None] * (4 - len(dst_list))))
+
+            def run():
+                C.call("hasty_stream_probe", nr, nw, nt, src, dst, GiB, C.stream_ptr())
+
+            run()
+            best = float("inf")
+            for _ in range(5):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                run()
+                e1.record(stream)
+                e1.synchronize()
+                best = min(best, e0.elapsed_time(e1))
+            gbs = (nr + nw) * GiB / (best / 1e3) / 1e9
+            rows.append({"read_streams": nr, "write_streams": nw, "nontemporal": bool(nt), "ms": best,
+                         "GB_s": gbs, "frac_of_8TBs": gbs / HBM_PEAK_GBS})
     n = 1 << 27  # 2^27 complex64 = 1 GiB
     a = torch.ones(n, dtype=torch.complex64, device="cuda")
     b = torch.empty_like(a)
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8}), flush=True)
+    print(json.dumps({"stream_ceiling_GiB_per_stream": 1, "mixes": rows, "copy_calibration_bytes_read": n * 8,
+                      "copy_calibration_bytes_written": n * 8}), flush=True)
 
 
 def cg_section(args, torch, native, op, x, K, kidx, smaps, spokes, offsets):
@@ -589,8 +628,12 @@ def main():
     if args.zconv_batch is not None:
         from hastycompute_amd import _capi
         _capi.call("hasty_set_option", b"sense_zconv_batch", args.zconv_batch)
+    if args.zconv_variant is not None:
+        from hastycompute_amd import _capi
+        _capi.call("hasty_set_option", b"sense_zconv_variant", args.zconv_variant)
     tune = None
-    manual = any(v is not None for v in (args.xy_variant, args.zconv_batch, args.coil_batch, args.problem_batch))
+    manual = any(v is not None for v in (args.xy_variant, args.zconv_batch, args.zconv_variant, args.coil_batch,
+                                         args.problem_batch))
     if not (args.no_autotune or args.generic or manual):
         from hastycompute_amd import autotune
 

@@ -51,6 +51,7 @@ _SIGS = {
     "hasty_timing_enable": ([c_int], c_int),
     "hasty_timing_entries": ([], c_int),
     "hasty_timing_read": ([c_int, ctypes.c_char_p, c_int, P_dbl, P_i64], c_int),
+    "hasty_stream_probe": ([c_int, c_int, c_int, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), c_i64, c_vp], c_int),
     "hasty_graph_begin": ([c_vp], c_int),
     "hasty_graph_end": ([c_vp, ctypes.POINTER(c_vp)], c_int),
     "hasty_graph_launch": ([c_vp, c_vp], c_int),

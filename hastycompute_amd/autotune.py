@@ -15,16 +15,18 @@ from . import _capi as C
 
 
 def sense_normal(nmodes3: Sequence[int], ncoil: int, nproblems: int) -> dict:
-    """Plane kernels, problems per launch, problems per z-convolution launch and coils per
-    workspace of the batched Toeplitz SENSE normal operator. Sets the library options to
-    the winners, so build SenseNormalToeplitz objects (which prepare their kernels) after
-    this call, with the returned coil_batch and problem_batch."""
-    choice = C.i64arr([0, 0, 0, 0])
+    """Plane kernels, problems per launch, the z-convolution kernel and problems per
+    z-convolution launch, and coils per workspace of the batched Toeplitz SENSE normal
+    operator. Sets the library options to the winners, so build SenseNormalToeplitz
+    objects (which prepare their kernels) after this call, with the returned coil_batch
+    and problem_batch."""
+    choice = C.i64arr([0, 0, 0, 0, 0])
     ms = ctypes.c_double()
     C.call("hasty_autotune_sense_normal", C.i64arr(nmodes3), int(ncoil), int(nproblems), choice, ctypes.byref(ms),
            C.stream_ptr())
     return {"sense_xy_variant": int(choice[0]), "sense_zconv_batch": int(choice[1]),
-            "problem_batch": int(choice[2]), "coil_batch": int(choice[3]), "ms_per_problem": ms.value}
+            "problem_batch": int(choice[2]), "coil_batch": int(choice[3]), "sense_zconv_variant": int(choice[4]),
+            "ms_per_problem": ms.value}
 
 
 def nufft(plan, c, f) -> dict:

@@ -373,6 +373,10 @@ int hasty_set_option(const char* name, int64_t value)
         llr_set_max_sweeps((int)value);
         return 0;
     }
+    if (name && std::strcmp(name, "sense_zconv_variant") == 0 && value >= 0 && value <= 3) {
+        sense_fused_set_zconv_variant((int)value);
+        return 0;
+    }
     set_last_error(std::string("unknown option ") + (name ? name : "(null)"));
     return 1;
 }
@@ -388,6 +392,7 @@ int hasty_get_option(const char* name, int64_t* value)
     else if (name && std::strcmp(name, "nufft_spread_variant") == 0) *value = nufft_get_spread_variant();
     else if (name && std::strcmp(name, "sense_zconv_batch") == 0) *value = sense_fused_get_zconv_batch();
     else if (name && std::strcmp(name, "llr_max_sweeps") == 0) *value = llr_get_max_sweeps();
+    else if (name && std::strcmp(name, "sense_zconv_variant") == 0) *value = sense_fused_get_zconv_variant();
     else {
         set_last_error(std::string("unknown option ") + (name ? name : "(null)"));
         return 1;
@@ -692,6 +697,7 @@ int hasty_autotune_sense_normal(const int64_t* nmodes3, int ncoil, int nproblems
         choice[1] = c.zconv_batch;
         choice[2] = c.problem_batch;
         choice[3] = c.coil_batch;
+        choice[4] = c.zconv_variant;
         if (ms_per_problem) *ms_per_problem = c.ms_per_problem;
     });
 }
@@ -771,6 +777,12 @@ int hasty_autotune_llr(const int64_t* dims5, const int64_t* block_strides, const
         if (nsplit) *nsplit = best;
         if (ms) *ms = tbest;
     });
+}
+
+int hasty_stream_probe(int nread, int nwrite, int nontemporal, const void* const* src, void* const* dst,
+                       int64_t bytes, void* stream)
+{
+    return guarded([&] { stream_probe(nread, nwrite, nontemporal != 0, src, dst, bytes, as_stream(stream)); });
 }
 
 int hasty_graph_begin(void* stream)

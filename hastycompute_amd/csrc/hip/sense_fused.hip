@@ -1402,13 +1402,16 @@ __global__ void __launch_bounds__(kQT) inv_xy_r16_kernel(FusedArgs a)
 }
 
 // z-axis pass for L0 = 256 (N0 = 128) as a 16 x 16 four-step FFT with the data in
-// registers: each of 256 threads owns one (column, group) pair of a 16-column bundle,
-// does 16-point DFTs in registers and meets the others through one LDS transpose per
-// direction; loads and stores go straight between HBM and registers.
-constexpr int kR16C = 16;   // columns per bundle
-constexpr int kR16T = 256;  // threads
-constexpr int kR16S1 = 17;  // LDS stride of the inner index (padded)
-constexpr int kR16S2 = 280; // LDS stride of the outer index (padded)
+// registers: each thread owns one (column, group) pair of a COLS-column bundle, does
+// 16-point DFTs in registers and meets the others through one LDS transpose per
+// direction; loads and stores go straight between HBM and registers. DB: two transpose
+// buffers (forward and inverse), so a coil needs two workgroup barriers instead of four.
+template <int COLS> struct ZConvShape {
+    static constexpr int T = 16 * COLS;                  // threads
+    static constexpr int S1 = COLS + 1;                  // LDS stride of the inner index (padded)
+    static constexpr int S2 = COLS == 16 ? 280 : 152;    // LDS stride of the outer index (padded)
+    static constexpr size_t lds_bytes(bool db) { return sizeof(float2) * (db ? 2 : 1) * 16 * S2; }
+};
 
 // Toeplitz kernels of the problems of one launch (problem pl0 + blockIdx.y uses k[blockIdx.y])
 constexpr int kZMaxProblems = 16;
@@ -1416,15 +1419,20 @@ struct ZKernels {
     const float* k[kZMaxProblems];
 };
 
-__global__ void __launch_bounds__(kR16T) z_conv_r16_kernel(FusedArgs a, int LL, int pl0, ZKernels zk)
+template <int COLS, bool DB>
+__global__ void __launch_bounds__(ZConvShape<COLS>::T) z_conv_r16_kernel(FusedArgs a, int LL, int pl0, ZKernels zk)
 {
+    using Sh = ZConvShape<COLS>;
+    constexpr int S1 = Sh::S1, S2 = Sh::S2;
     const int pl = pl0 + (int)blockIdx.y;
     const float* __restrict__ K = zk.k[blockIdx.y];
-    __shared__ float2 lds[16 * kR16S2];
+    extern __shared__ float2 lds_all[];  // ZConvShape<COLS>::lds_bytes(DB)
+    float2* ldsf = lds_all;
+    float2* ldsi = lds_all + (DB ? 16 * S2 : 0);
     const int t = threadIdx.x;
-    const int c = t & 15;
-    const int g = t >> 4;
-    const int pos0 = blockIdx.x * kR16C;
+    const int c = t % COLS;
+    const int g = t / COLS;
+    const int pos0 = blockIdx.x * COLS;
     float2 tw[16];  // w256^{g m}, forward sign
 #pragma unroll
     for (int m = 0; m < 16; ++m) tw[m] = a.twL0[(g * m) & 255];
@@ -1463,11 +1471,11 @@ __global__ void __launch_bounds__(kR16T) z_conv_r16_kernel(FusedArgs a, int LL, 
 #pragma unroll
         for (int k1 = 1; k1 < 16; ++k1) v[k1] = cmul(v[k1], tw[k1]);
 #pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) lds[k1 * kR16S2 + g * kR16S1 + c] = v[k1];
+        for (int k1 = 0; k1 < 16; ++k1) ldsf[k1 * S2 + g * S1 + c] = v[k1];
         lds_barrier();
         // forward, step B (k1 = g): DFT over n2 -> X[g + 16 k2]; times the kernel
 #pragma unroll
-        for (int n2 = 0; n2 < 16; ++n2) v[n2] = lds[g * kR16S2 + n2 * kR16S1 + c];
+        for (int n2 = 0; n2 < 16; ++n2) v[n2] = ldsf[g * S2 + n2 * S1 + c];
         dft16<-1>(v);
 #pragma unroll
         for (int k2 = 0; k2 < 16; ++k2) v[k2] = cscale(v[k2], kv[k2]);
@@ -1475,18 +1483,30 @@ __global__ void __launch_bounds__(kR16T) z_conv_r16_kernel(FusedArgs a, int LL, 
         dft16<1>(v);
 #pragma unroll
         for (int n1 = 1; n1 < 16; ++n1) v[n1] = cmulc(tw[n1], v[n1]);
-        lds_barrier();  // all step-B reads are done before the transpose is overwritten
+        // one buffer: all step-B reads are done before the transpose is overwritten.
+        // Two buffers: the next coil's forward writes come after this coil's second
+        // barrier, which every thread passes only after its step-B reads, and the next
+        // inverse writes after the next first barrier, past this coil's step-A' reads.
+        if (!DB) lds_barrier();
 #pragma unroll
-        for (int n1 = 0; n1 < 16; ++n1) lds[n1 * kR16S2 + g * kR16S1 + c] = v[n1];
+        for (int n1 = 0; n1 < 16; ++n1) ldsi[n1 * S2 + g * S1 + c] = v[n1];
         lds_barrier();
         // inverse, step A' (n1 = g): IDFT over k1 -> out[g + 16 n2], keep rows < 128
 #pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) v[k1] = lds[g * kR16S2 + k1 * kR16S1 + c];
+        for (int k1 = 0; k1 < 16; ++k1) v[k1] = ldsi[g * S2 + k1 * S1 + c];
         dft16<1>(v);
 #pragma unroll
         for (int n2 = 0; n2 < 8; ++n2) bstore<true>(rc, v[n2], voff, n2 * zstep);
-        lds_barrier();
+        if (!DB) lds_barrier();
     }
+}
+
+template <int COLS, bool DB> void launch_zconv(const FusedArgs& a, int LL, int q0, int nq, const ZKernels& zk, hipStream_t s)
+{
+    const size_t smem = ZConvShape<COLS>::lds_bytes(DB);
+    allow_lds(z_conv_r16_kernel<COLS, DB>, smem);
+    hipLaunchKernelGGL((z_conv_r16_kernel<COLS, DB>), dim3((unsigned)(LL / COLS), (unsigned)nq),
+                       dim3(ZConvShape<COLS>::T), smem, s, a, LL, q0, zk);
 }
 
 uint32_t radix_code(int L, int* npass)
@@ -1533,6 +1553,11 @@ namespace {
 // variant, so it is set before kernels are prepared.
 int g_xy_variant = 2;
 int g_zconv_batch = 16;  // problems per z-convolution launch (N0 = 128), 1..16
+// z-convolution at N0 = 128: 0 16 columns x 256 threads, one LDS buffer (four barriers
+// per coil); 1 8 columns x 128 threads; 2 (default) 16 columns with two buffers (two
+// barriers per coil); 3 8 columns with two buffers. At 128^3 x 16 coils, 4 problems per
+// launch: 1.85, 3.58, 1.66 and 3.59 ms (8 columns leave 64-byte row segments per wave)
+int g_zconv_variant = 2;
 
 bool use_r16(int N) { return g_xy_variant == 2 && N == 128; }
 }  // namespace
@@ -1541,6 +1566,8 @@ void sense_fused_set_xy_variant(int v) { g_xy_variant = v; }
 void sense_fused_set_zconv_batch(int v) { g_zconv_batch = v; }
 int sense_fused_get_xy_variant() { return g_xy_variant; }
 int sense_fused_get_zconv_batch() { return g_zconv_batch; }
+void sense_fused_set_zconv_variant(int v) { g_zconv_variant = v; }
+int sense_fused_get_zconv_variant() { return g_zconv_variant; }
 
 bool sense_fused_supported(const int64_t* n3)
 {
@@ -1645,8 +1672,12 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
                     for (int j = 0; j < nq; ++j)
                         zk.k[j] = kernels + (int64_t)kidx[p0 + q0 + j] * L0 * (int64_t)L * L;
                     KernelTimer t("sense_z_conv", s);
-                    hipLaunchKernelGGL(z_conv_r16_kernel, dim3((unsigned)(L * L / kR16C), (unsigned)nq), dim3(kR16T), 0,
-                                       s, a, L * L, q0, zk);
+                    switch (g_zconv_variant) {
+                    case 1: launch_zconv<8, false>(a, L * L, q0, nq, zk, s); break;
+                    case 2: launch_zconv<16, true>(a, L * L, q0, nq, zk, s); break;
+                    case 3: launch_zconv<8, true>(a, L * L, q0, nq, zk, s); break;
+                    default: launch_zconv<16, false>(a, L * L, q0, nq, zk, s); break;
+                    }
                     check_launch("z_conv_kernel");
                 }
             }

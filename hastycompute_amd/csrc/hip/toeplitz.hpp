@@ -68,6 +68,9 @@ void sense_fused_set_xy_variant(int v);
 void sense_fused_set_zconv_batch(int v);
 int sense_fused_get_xy_variant();
 int sense_fused_get_zconv_batch();
+// z-convolution kernel at N0 = 128 (0..3, see sense_fused.hip)
+void sense_fused_set_zconv_variant(int v);
+int sense_fused_get_zconv_variant();
 // true when the current sense_xy_variant gives the register kernels at this size
 bool sense_fused_register_layout(const int64_t* n3);
 void sense_fused_permute_kernel(const float* natural, float* permuted, const int64_t* n3, bool r16, hipStream_t s);

@@ -104,6 +104,7 @@ SenseChoice autotune_sense_normal(const int64_t* n3, int ncoil, int nproblems, b
         if (fused) {
             sense_fused_set_xy_variant(it->second.xy_variant);
             sense_fused_set_zconv_batch(it->second.zconv_batch);
+            sense_fused_set_zconv_variant(it->second.zconv_variant);
         }
         return it->second;
     }
@@ -145,6 +146,7 @@ SenseChoice autotune_sense_normal(const int64_t* n3, int ncoil, int nproblems, b
         best.problem_batch = fused ? std::min(4, Pt) : 1;
         best.xy_variant = sense_fused_get_xy_variant();
         best.zconv_batch = sense_fused_get_zconv_batch();
+        best.zconv_variant = sense_fused_get_zconv_variant();
         double tbest = 1e300;
         if (fused) {
             std::vector<int> variants = n3[1] == 128 ? std::vector<int>{2, 1, 3, 0} : std::vector<int>{1, 3, 0};
@@ -170,6 +172,18 @@ SenseChoice autotune_sense_normal(const int64_t* n3, int ncoil, int nproblems, b
                 }
             }
             if (2 * n3[0] == 256) {
+                // the z-convolution kernel: the two 16-column ones (8 columns halve the
+                // row segments a wave reads and run at half speed)
+                for (int zv : {2, 0}) {
+                    if (zv == best.zconv_variant) continue;
+                    sense_fused_set_zconv_variant(zv);
+                    const double t = run_with(best.coil_batch, best.problem_batch);
+                    if (t < kGain * tbest) {
+                        tbest = t;
+                        best.zconv_variant = zv;
+                    }
+                }
+                sense_fused_set_zconv_variant(best.zconv_variant);
                 // problems per z-convolution launch: values at or above the problem batch
                 // all mean one launch per chunk, so each effective value is timed once
                 std::vector<int> seen = {std::min(best.zconv_batch, best.problem_batch)};

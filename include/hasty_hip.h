@@ -125,6 +125,9 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
  *     coset elsewhere, 3 coset with an LDS-DMA inverse), set before preparing kernels;
  *   "sense_zconv_batch" 1..16 (default 16): problems per z-convolution launch of the fused
  *     normal operator at N0 = 128.
+ *   "sense_zconv_variant" 0..3 (default 2): z-convolution kernel at N0 = 128 (0: 16
+ *     columns per workgroup, one LDS transpose buffer; 1: 8 columns; 2: 16 columns, two
+ *     buffers; 3: 8 columns, two buffers).
  *   "llr_max_sweeps" 1..15 (default 15): cap on the Jacobi sweeps of the LLR eigensolver.
  *   "nufft_spread_variant" 0..3 (default 3): 3-D type-1 spreader; 0 one lane per
  *     point with LDS atomics, 1 window rows with one wave per tile, 2 window rows with four
@@ -144,10 +147,11 @@ int hasty_get_option(const char* name, int64_t* value);
  * hasty_autotune_sense_normal: the batched Toeplitz SENSE normal operator at image size
  *   nmodes3 with ncoil coils and nproblems problems (at most 8 timed), on synthetic
  *   data: plane kernels, problems per launch, problems per z-convolution launch and
- *   coils per workspace. Sets sense_xy_variant and sense_zconv_batch to the winners
- *   (prepare kernels after this call) and writes choice[4] = {sense_xy_variant,
- *   sense_zconv_batch, problem_batch, coil_batch} (size the workspace with the last two)
- *   and the time per problem-apply.
+ *   coils per workspace, and at N0 = 128 the z-convolution kernel. Sets
+ *   sense_xy_variant, sense_zconv_variant and sense_zconv_batch to the winners (prepare
+ *   kernels after this call) and writes choice[5] = {sense_xy_variant,
+ *   sense_zconv_batch, problem_batch, coil_batch, sense_zconv_variant} (size the
+ *   workspace with problem_batch and coil_batch) and the time per problem-apply.
  * hasty_nufft_autotune: the 3-D type-1 spreaders (nufft_spread_variant 0..3) of `plan`
  *   on its own points, executing c -> f (f is overwritten); the plan keeps the fastest.
  *   Other plans keep their spreader. *spread_variant receives the choice, *ms the
@@ -170,6 +174,15 @@ int hasty_autotune_llr(const int64_t* dims5, const int64_t* block_strides, const
 int hasty_timing_enable(int on);
 int hasty_timing_entries(void);
 int hasty_timing_read(int index, char* name, int name_capacity, double* total_ms, int64_t* count);
+
+/* Streaming-bandwidth probe (no counterpart in the reference; a measurement hook for
+ * the roofline's ceiling): reads nread device arrays src[0..nread) and writes nwrite
+ * arrays dst[0..nwrite) of `bytes` each (a multiple of 16), 16 bytes per lane, with
+ * non-temporal loads and stores if `nontemporal`. Mixes (nread:nwrite): 1:0, 2:0, 0:1,
+ * 0:2, 1:1, 2:1, 1:2, 1:4, 4:1. A read-only mix needs dst[0], a 16-byte sink that is
+ * not written for finite data. Timed as "stream_probe" by hasty_timing_*. */
+int hasty_stream_probe(int nread, int nwrite, int nontemporal, const void* const* src, void* const* dst,
+                       int64_t bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------
  * HIP graphs of library launch sequences. The reference drives each frame's operator

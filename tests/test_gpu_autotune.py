@@ -23,7 +23,7 @@ def N():
 def restore_options():
     from hastycompute_amd import _capi as C
 
-    names = ("sense_xy_variant", "sense_zconv_batch", "nufft_spread_variant")
+    names = ("sense_xy_variant", "sense_zconv_batch", "sense_zconv_variant", "nufft_spread_variant")
     saved = {k: C.get_option(k) for k in names}
     yield
     for k, v in saved.items():

@@ -128,3 +128,33 @@ def test_prepared_layout_survives_option_change(N):
     finally:
         C.set_option("sense_xy_variant", prev)
     assert O.rel_l2(a, b) < 1e-5 and O.rel_l2(c, b) < 1e-5
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_zconv_variants_vs_oracle(N, variant):
+    # every z-convolution kernel at N0 = 128 (2N0 = 256): 16 or 8 columns per workgroup,
+    # one or two LDS transpose buffers; three problems with their own kernels
+    from hastycompute_amd import _capi as C
+
+    rng = np.random.default_rng(40 + variant)
+    shape = (128, 32, 32)
+    S = maps(rng, 3, shape)
+    cos = [rng.uniform(-np.pi, np.pi, (3, 3000)) for _ in range(2)]
+    kern, diags = [], []
+    for co in cos:
+        _, k = N.toeplitz_build(g(co, torch.float32), list(shape), diag=False)
+        kern.append(k)
+        diags.append(O.toeplitz_diagonal(co.astype(np.float32).astype(np.float64), [1, *shape]))
+    x = rnd(rng, (3,) + shape)
+    kidx = [1, 0, 1]
+    prev = C.get_option("sense_zconv_variant")
+    try:
+        C.set_option("sense_zconv_variant", variant)
+        op = N.SenseNormalToeplitz(g(S), torch.stack(kern), coil_batch=3, problem_batch=3)
+        out = h(op.apply(g(x), kernel_index=kidx))
+    finally:
+        C.set_option("sense_zconv_variant", prev)
+    for p in range(3):
+        e = O.rel_l2(out[p:p + 1], O.sense_normal_toeplitz(diags[kidx[p]], x[p:p + 1], S, [0, 1, 2]))
+        print(f"z-conv variant {variant} problem {p}: {e:.2e}", flush=True)
+        assert e < 1e-5
