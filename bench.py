@@ -138,15 +138,23 @@ def nufft_main(args, torch, native):
                      "kernels_ms": {k: v[0] / v[1] for k, v in kt.items()}}
     p1.close()
     p2.close()
-    # CPU: the FP64 oracle's ES-kernel type 2 on a bounded sample of the points
-    from oracle import oracle as O
+    cpu = None
+    if not args.no_cpu:
+        # CPU: the oracle's ES-kernel NUFFT (C spreading/interpolation with OpenMP +
+        # scipy.fft, FP64) at the same tolerance on the whole workload, one pair
+        from oracle import oracle as O
 
-    Ms = 8192
-    co_s = co[:, :Ms].double().cpu().numpy()
-    f_s = f[0].cpu().numpy().astype(np.complex128)
-    t = time.perf_counter()
-    O.nufft_type2(co_s, f_s, -1, 1e-6)
-    cdt = time.perf_counter() - t
+        co_h = co.double().cpu().numpy()
+        f_h = f[0].cpu().numpy().astype(np.complex128)
+        c_h = c[0].cpu().numpy().astype(np.complex128)
+        t = time.perf_counter()
+        O.nufft_type2(co_h, f_h, -1, 1e-6)
+        O.nufft_type1(co_h, c_h, [n, n, n], 1, 1e-6)
+        cdt = time.perf_counter() - t
+        cpu = {"value": 1.0 / cdt, "unit": "pairs/s", "cores": O.default_threads(), "kind": "port",
+               "sample": f"one type-2 + type-1 pair at {n}^3, M={M}, tol 1e-6: the FP64 oracle ES-kernel NUFFT "
+                         f"(C spread/interp with OpenMP + scipy.fft, {O.default_threads()} threads)",
+               "seconds": cdt}
     tr_s = 1e3 / sum(r["ms"] for r in res.values())
     dom = max(res, key=lambda k: res[k]["ms"])
     line = {"metric": "NUFFT type-2 + type-1 pairs/s @128^3, M=147200 (kooshball 1150x128), tol 1e-6",
@@ -157,12 +165,9 @@ def nufft_main(args, torch, native):
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": res[dom]["achieved_GBs"], "peak": 8000.0,
                          "unit": "GB/s", "frac": res[dom]["achieved_GBs"] / 8000.0, "traffic": None,
                          "alg_bytes_per_transform": B},
-            "transforms": res,
-            "cpu_baseline": {"value": 1.0 / (cdt * M / Ms), "unit": "type-2 transforms/s", "cores": O.default_threads(),
-                             "kind": "port",
-                             "sample": f"FP64 oracle ES-kernel type 2 (scipy.fft) with {Ms} of the {M} points, "
-                                       f"scaled to M by the point count (the FFT part is not scaled)",
-                             "seconds": cdt}}
+            "transforms": res}
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 
 
@@ -279,6 +284,24 @@ def serve_main(args, torch):
             **{k: st[k] for k in ("latency_p50_s", "latency_p99_s", "latency_mean_s", "makespan_s", "mean_batch",
                                   "max_active", "peak_hbm_bytes", "ticks")}}
     print(json.dumps(line), flush=True)
+
+
+def stored_traffic(key: str, units_per_launch: float):
+    """HBM bytes per launch from the calibrated FETCH_SIZE / WRITE_SIZE figures in
+    profiles/traffic.json (bytes per work unit x this run's units per launch), or None
+    when there is no figure or the kernel's source has changed since it was measured."""
+    from hastycompute_amd import profiling
+
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, "no counter figure"
+    ent = json.load(open(TRAFFIC_FILE)).get(key)
+    if not ent or "hbm_bytes_per_unit" not in ent:
+        return None, f"no counter figure for {key}"
+    if ent.get("source_sha256") != profiling.source_sha256(ent["kernel_symbol"]):
+        return None, f"counter figure for {key} was measured on another version of the kernel source"
+    return ent["hbm_bytes_per_unit"] * units_per_launch, (
+        f"calibrated FETCH_SIZE + WRITE_SIZE, {ent['hbm_bytes_per_unit']:.4g} B per problem x "
+        f"{units_per_launch:g} problems per launch")
 
 
 def normal_op_bytes(N: int, C: int) -> int:
@@ -683,13 +706,10 @@ def main():
     if kernels:
         dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
         d = kernels[dom]
-        traffic = None
-        if os.path.exists(TRAFFIC_FILE):
-            tr = json.load(open(TRAFFIC_FILE))
-            key = f"{dom}@{args.n}x{args.coils}x{args.problem_batch}"
-            traffic = tr.get(key, {}).get("hbm_bytes_per_launch")
+        traffic, traffic_note = stored_traffic(f"{dom}@{args.n}x{args.coils}", P * args.steps / d["launches"])
         roofline.update({"kernel": dom, "achieved": d["achieved_GBs"], "frac": (d["achieved_GBs"] or 0) / HBM_PEAK_GBS,
-                         "traffic": traffic, "alg_bytes_per_launch": d["alg_bytes_per_launch"],
+                         "traffic": traffic, "traffic_note": traffic_note,
+                         "alg_bytes_per_launch": d["alg_bytes_per_launch"],
                          "avg_launch_ms": d["avg_ms"], "timing": "HIP events on the launch stream over the timed region"})
     else:
         roofline.update({"kernel": "sense_normal (whole pipeline)", "achieved": op_achieved,
