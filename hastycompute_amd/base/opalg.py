@@ -14,7 +14,10 @@ def _wrap(x):
     if isinstance(x, Vector):
         return x
     if isinstance(x, (int, float, complex)):
-        return Vector(torch.tensor(x))
+        # a zero-dim double tensor: it keeps the scalar's precision, and type promotion
+        # still gives the result the dtype of the data it multiplies (float32 / complex64
+        # data stay single precision)
+        return Vector(torch.tensor(x, dtype=torch.complex128 if isinstance(x, complex) else torch.float64))
     return Vector(x)
 
 

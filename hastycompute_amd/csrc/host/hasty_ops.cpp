@@ -946,6 +946,18 @@ std::vector<at::Tensor> SenseNOp::apply_backward(const std::vector<at::Tensor>& 
 
 std::shared_ptr<SenseNOp> SenseNOp::adjoint() const { return std::shared_ptr<SenseNOp>(new SenseNOp(_h)); }
 
+// opalgs.cpp:29-44
+at::Tensor PowerIteration::run(const Op& A, at::Tensor& v, int iters)
+{
+    at::Tensor lambda = at::linalg_vector_norm(v);
+    for (int i = 0; i < iters; ++i) {
+        v = A(v);
+        lambda = at::linalg_vector_norm(v);
+        v.div_(lambda);
+    }
+    return lambda;
+}
+
 ConjugateGradient::ConjugateGradient(Op A, at::Tensor b, at::optional<Op> P)
     : _A(std::move(A)), _b(std::move(b)), _P(std::move(P))
 {

@@ -404,6 +404,15 @@ private:
     std::shared_ptr<Holder> _h;
 };
 
+// lib/op/opalgs.cpp:29-44 on tensors: per iteration v = A v, lambda = |v|, v /= lambda;
+// returns lambda (|v| if iters < 1). v is updated in place.
+class PowerIteration {
+public:
+    using Op = std::function<at::Tensor(const at::Tensor&)>;
+    PowerIteration() = default;
+    at::Tensor run(const Op& A, at::Tensor& v, int iters = 30);
+};
+
 // lib/op/opalgs.cpp:50-96 on tensors: A is a callable, P an optional preconditioner.
 class ConjugateGradient {
 public:

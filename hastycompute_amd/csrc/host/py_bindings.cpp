@@ -227,6 +227,7 @@ public:
     {
         catcher([&] { _bs->apply(in, out, coils.has_value() ? *coils : all_coils(_bs->nouter_batches(), _bs->ncoils())); });
     }
+    mri::BatchedSenseNormal& impl() { return *_bs; }
 
 private:
     std::unique_ptr<mri::BatchedSenseNormal> _bs;
@@ -396,6 +397,76 @@ TORCH_LIBRARY(HastySVT, m)
     });
 }
 
+// The C++ iterative algorithms over the operators above (no TorchScript binding in the
+// reference; they let Python drive op::PowerIteration / ConjugateGradient /
+// BatchedConjugateGradient, lib/op/opalgs.cpp:29-96, opalgs.ixx:84-136).
+class PowerIteration : public torch::CustomClassHolder {
+public:
+    at::Tensor run(const c10::intrusive_ptr<SenseNOp>& A, at::Tensor v, int64_t iters)
+    {
+        at::Tensor lambda;
+        catcher([&] {
+            at::Tensor w = v;
+            lambda = hasty::op::PowerIteration().run([&](const at::Tensor& t) { return A->apply(t); }, w, (int)iters);
+            v.copy_(w);
+        });
+        return lambda;
+    }
+};
+
+class ConjugateGradient : public torch::CustomClassHolder {
+public:
+    ConjugateGradient(const c10::intrusive_ptr<SenseNOp>& A, const at::Tensor& b, double l2) : _A(A), _b(b), _l2(l2) {}
+    void run(at::Tensor x, int64_t iter, double tol)
+    {
+        catcher([&] {
+            auto A = _A;
+            const double l2 = _l2;
+            hasty::op::ConjugateGradient cg([A, l2](const at::Tensor& t) { return l2 != 0.0 ? A->apply(t).add_(t, l2) : A->apply(t); },
+                                     _b);
+            cg.run(x, (int)iter, tol);
+        });
+    }
+
+private:
+    c10::intrusive_ptr<SenseNOp> _A;
+    at::Tensor _b;
+    double _l2;
+};
+
+class BatchedConjugateGradient : public torch::CustomClassHolder {
+public:
+    // A: a BatchedSenseNormal without k-space data (its apply is then the normal
+    // operator); b, x: [F, E, N...] complex64 on the GPU, solved as F * E problems.
+    BatchedConjugateGradient(const c10::intrusive_ptr<BatchedSenseNormal>& A, const at::Tensor& b) : _A(A), _b(b) {}
+    at::Tensor run(at::Tensor x, int64_t iter, double tol)
+    {
+        at::Tensor state;
+        catcher([&] {
+            if (!x.is_cuda() || !_b.is_cuda() || x.dim() < 3 || x.sizes() != _b.sizes())
+                throw std::runtime_error("BatchedConjugateGradient takes x and b of one [F, E, N...] shape on the GPU");
+            const auto shape = x.sizes().vec();
+            std::vector<int64_t> flat = {shape[0] * shape[1]};
+            flat.insert(flat.end(), shape.begin() + 2, shape.end());
+            auto A = _A;
+            auto coils = all_coils((int)shape[0], A->impl().ncoils());
+            hasty::op::BatchedConjugateGradient cg(
+                [A, shape, coils](const at::Tensor& v, at::Tensor& out) {
+                    at::Tensor o = out.view(shape);
+                    A->impl().apply(v.view(shape), o, coils);
+                },
+                _b.reshape(flat));
+            at::Tensor xf = x.view(flat);
+            state = cg.run(xf, (int)iter, tol);
+        });
+        return state;
+    }
+
+private:
+    c10::intrusive_ptr<BatchedSenseNormal> _A;
+    at::Tensor _b;
+};
+
 TORCH_LIBRARY(HastyOp, m)
 {
     m.class_<SenseNOp>("SenseNOp")
@@ -407,6 +478,14 @@ TORCH_LIBRARY(HastyOp, m)
         .def("apply_backward", &SenseNOp::apply_backward)
         .def("apply_list", &SenseNOp::apply_list)
         .def("adjoint", &SenseNOp::adjoint);
+    // registered after SenseNOp, whose type their signatures use
+    m.class_<PowerIteration>("PowerIteration").def(torch::init<>()).def("run", &PowerIteration::run);
+    m.class_<ConjugateGradient>("ConjugateGradient")
+        .def(torch::init<const c10::intrusive_ptr<SenseNOp>&, const at::Tensor&, double>())
+        .def("run", &ConjugateGradient::run);
+    m.class_<BatchedConjugateGradient>("BatchedConjugateGradient")
+        .def(torch::init<const c10::intrusive_ptr<BatchedSenseNormal>&, const at::Tensor&>())
+        .def("run", &BatchedConjugateGradient::run);
     m.def("doc", []() -> std::string {
         return "SenseNOp(coords, nmodes, smaps, coils, forward_opts?, backward_opts?): apply(x) = A^H A x, "
                "apply_forward(x) = A x [coils, M], apply_backward(y) = A^H y [1, N...], apply_list maps apply over a "

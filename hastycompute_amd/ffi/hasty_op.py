@@ -44,3 +44,39 @@ class SenseNOp:
 
     def adjoint(self) -> "SenseNOp":
         return SenseNOp._wrap(self._op.adjoint())
+
+
+class PowerIteration:
+    """op::PowerIteration (lib/op/opalgs.cpp:29-44): run(A, v, iters) replaces v by the
+    normalised A^iters v and returns the last |A v| (the largest eigenvalue estimate)."""
+
+    def __init__(self):
+        self._alg = _op.PowerIteration()
+
+    def run(self, A: SenseNOp, v: torch.Tensor, iters: int = 30) -> torch.Tensor:
+        return self._alg.run(A._op, v, iters)
+
+
+class ConjugateGradient:
+    """op::ConjugateGradient (opalgs.cpp:50-96) on A = SenseNOp (+ l2 I, the reference's
+    l2_reg_linop): run(x, iter, tol) updates x in place; raises "A was not positive
+    definite" as the reference does."""
+
+    def __init__(self, A: SenseNOp, b: torch.Tensor, l2: float = 0.0):
+        self._alg = _op.ConjugateGradient(A._op, b, float(l2))
+
+    def run(self, x: torch.Tensor, iter: int = 30, tol: float = 0.0) -> None:
+        self._alg.run(x, iter, tol)
+
+
+class BatchedConjugateGradient:
+    """op::BatchedConjugateGradient (opalgs.ixx:84-136) over every (frame, encoding) of a
+    hasty_sense.BatchedSenseNormal built without k-space data: the F * E problems of x, b
+    [F, E, N...] (GPU) go through the device CG together. run returns the per-problem
+    state [F * E, 8] (rzold, alpha, beta, done, not positive definite, pAp, iterations)."""
+
+    def __init__(self, A, b: torch.Tensor):
+        self._alg = _op.BatchedConjugateGradient(getattr(A, "_senseop", A), b)
+
+    def run(self, x: torch.Tensor, iter: int = 30, tol: float = 0.0) -> torch.Tensor:
+        return self._alg.run(x, iter, tol)

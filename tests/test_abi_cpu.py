@@ -65,7 +65,8 @@ def test_torchscript_modules_registered(ffi):
                          "HastyBatchedSense": ["BatchedSense", "BatchedSenseAdjoint", "BatchedSenseNormal"],
                          "HastySVT": ["Random3DBlocksSVT", "Normal3DBlocksSVT"],
                          "HastyInterface": ["FunctionLambda"],
-                         "HastyOp": ["SenseNOp"]}.items():
+                         "HastyOp": ["SenseNOp", "PowerIteration", "ConjugateGradient",
+                                     "BatchedConjugateGradient"]}.items():
         ns = getattr(torch.classes, mod)
         for c in classes:
             assert getattr(ns, c) is not None

@@ -266,3 +266,73 @@ def test_ffi_random3d_blocks_svt(F):
     zero = (t.abs().amax(dim=(0, 1)) == 0).numpy()
     assert 64 <= zero.sum() <= 3 * 64
     assert t.abs().amax() > 0
+
+
+def test_ffi_cpp_power_iteration_and_cg(F):
+    """op::PowerIteration (opalgs.cpp:29-44) and op::ConjugateGradient (opalgs.cpp:50-96),
+    plain and on l2_reg_linop(A, lam), driven through TorchScript over op::SenseNOp, against
+    the oracle's power iteration and CG on the oracle SENSE normal operator; the CG's
+    not-positive-definite error is the reference's."""
+    from hastycompute_amd.ffi import hasty_nufft, hasty_op
+
+    rng = np.random.default_rng(21)
+    img = [12, 12, 12]
+    M = 6000
+    co, co_used = coords32(rng, 3, M)
+    S = rnd(rng, [3] + img) * 0.4
+    opts = hasty_nufft.NufftOptions(2, False, 1e-8), hasty_nufft.NufftOptions(1, True, 1e-8)
+    op = hasty_op.SenseNOp(g(co, torch.float32), [1] + img, g(S), [0, 1, 2], *opts)
+    Aref = lambda v: O.sense_normal_apply(co_used, v, S, [0, 1, 2], exact=True)
+    v0 = rnd(rng, [1] + img)
+    v = g(v0)
+    lam = float(hasty_op.PowerIteration().run(op, v, 25))
+    lam_ref = O.power_iteration(Aref, v0, 25)
+    e_v = O.rel_l2(h(v), _power_vec(Aref, v0, 25))
+    print(f"C++ PowerIteration: lambda {lam:.6e} vs oracle {lam_ref:.6e}, vector {e_v:.2e}", flush=True)
+    assert abs(lam - lam_ref) / lam_ref < 1e-5 and e_v < 1e-4
+    b = rnd(rng, [1] + img)
+    for l2 in (0.0, 0.2 * lam_ref):
+        x = torch.zeros((1, *img), dtype=torch.complex64, device="cuda")
+        hasty_op.ConjugateGradient(op, g(b), l2).run(x, 8, 0.0)
+        ref = O.conjugate_gradient(lambda w: Aref(w) + l2 * w, b, np.zeros_like(b), 8)
+        e = O.rel_l2(h(x), ref)
+        print(f"C++ ConjugateGradient l2={l2:.3g}: {e:.2e}", flush=True)
+        assert e < 1e-4
+    neg = hasty_op.ConjugateGradient(op, g(b), -2.0 * lam_ref)
+    with pytest.raises(RuntimeError, match="A was not positive definite"):
+        neg.run(torch.zeros((1, *img), dtype=torch.complex64, device="cuda"), 3, 0.0)
+
+
+def _power_vec(A, v0, iters):
+    v = np.array(v0, dtype=np.complex128)
+    for _ in range(iters):
+        v = A(v)
+        v = v / np.linalg.norm(v)
+    return v
+
+
+def test_ffi_cpp_batched_cg(F):
+    """op::BatchedConjugateGradient (opalgs.ixx:84-136) over a BatchedSenseNormal of 2
+    frames x 2 encodings: each problem equals the oracle CG on its frame's Toeplitz SENSE
+    normal operator (uniform points, well conditioned), and equals itself solved alone."""
+    from hastycompute_amd.ffi import hasty_op
+
+    rng = np.random.default_rng(22)
+    n, C, nf, E = 16, 3, 2, 2
+    cos = [rng.uniform(-np.pi, np.pi, (3, 3 * n ** 3)) for _ in range(nf)]
+    S = rnd(rng, (C, n, n, n)) * 0.3 + 1.0
+    S /= np.sqrt(np.sum(np.abs(S) ** 2, axis=0, keepdims=True))
+    bs = F.sense.BatchedSenseNormal([g(c, torch.float32) for c in cos], g(S))
+    b = rnd(rng, (nf, E, n, n, n))
+    x = torch.zeros((nf, E, n, n, n), dtype=torch.complex64, device="cuda")
+    st = hasty_op.BatchedConjugateGradient(bs, g(b)).run(x, 6, 0.0)
+    xs = h(x)
+    assert tuple(st.shape) == (nf * E, 8) and bool((st[:, 6] == 6).all())
+    for f in range(nf):
+        diag = O.toeplitz_diagonal(cos[f].astype(np.float32).astype(np.float64), [1, n, n, n])
+        A = lambda v: O.sense_normal_toeplitz(diag, v, S, list(range(C)))
+        for e in range(E):
+            ref = O.conjugate_gradient(A, b[f, e][None], np.zeros((1, n, n, n)), 6)
+            err = O.rel_l2(xs[f, e][None], ref)
+            print(f"C++ BatchedConjugateGradient frame {f} encoding {e}: {err:.2e}", flush=True)
+            assert err < 1e-5
