@@ -84,6 +84,8 @@ def parse():
     ap.add_argument("--llr-max-sweeps", type=int, default=None, help="cap on the LLR Jacobi sweeps (1..15)")
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--rate", type=float, default=2.0)
+    ap.add_argument("--llr-fraction", type=float, default=0.25,
+                    help="serve mode: share of multi-frame requests with the LLR prox")
     ap.add_argument("--seed", type=int, default=2024)
     return ap.parse_args()
 
@@ -267,8 +269,9 @@ def serve_main(args, torch):
     samples, 15 CG iterations) through the continuous-batching server, in real time."""
     from hastycompute_amd import serving
 
-    reqs = serving.poisson_requests(args.requests, args.rate, args.seed, iters=args.cg_iters)
-    srv = serving.ReconServer()
+    reqs = serving.poisson_requests(args.requests, args.rate, args.seed, iters=args.cg_iters,
+                                    llr_fraction=args.llr_fraction)
+    srv = serving.ReconServer(autotune=not args.no_autotune)
 
     def progress(msg):  # one line per finished request on stderr (the JSON line stays alone on stdout)
         if " done " in msg:
@@ -279,10 +282,13 @@ def serve_main(args, torch):
             "value": st["recons_per_s"], "unit": "recons/s", "n_gpus": 1, "higher_is_better": True,
             "dtype": "c64 (fp32), FP64 CG scalars", "data": "synthetic (seeded phantoms, coil maps, kooshball)",
             "config": {"workload": f"{args.requests} requests, Poisson mean {args.rate}/s, N in {{96,128,160}}, "
-                                   f"8-32 coils, 1k-10k spokes x 2N, {args.cg_iters} CG iterations",
-                       "seed": args.seed},
+                                   f"8-32 coils, 1k-10k spokes x 2N, {args.cg_iters} CG iterations, "
+                                   f"{args.llr_fraction:.0%} multi-frame (4-8 frames) with the LLR prox",
+                       "seed": args.seed, "llr_requests": sum(r.llr_thresh is not None for r in reqs),
+                       "autotune": st["autotune"]},
+            "latency_note": "from arrival to result; the k-space is produced by a scanner thread before arrival",
             **{k: st[k] for k in ("latency_p50_s", "latency_p99_s", "latency_mean_s", "makespan_s", "mean_batch",
-                                  "max_active", "peak_hbm_bytes", "ticks")}}
+                                  "max_active", "peak_hbm_bytes", "ticks", "late_scanner_data")}}
     print(json.dumps(line), flush=True)
 
 
