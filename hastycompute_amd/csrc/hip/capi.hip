@@ -377,6 +377,10 @@ int hasty_set_option(const char* name, int64_t value)
         sense_fused_set_zconv_variant((int)value);
         return 0;
     }
+    if (name && std::strcmp(name, "nufft_locality") == 0 && value >= 0 && value <= 4) {
+        nufft_set_locality((int)value);
+        return 0;
+    }
     set_last_error(std::string("unknown option ") + (name ? name : "(null)"));
     return 1;
 }
@@ -393,6 +397,7 @@ int hasty_get_option(const char* name, int64_t* value)
     else if (name && std::strcmp(name, "sense_zconv_batch") == 0) *value = sense_fused_get_zconv_batch();
     else if (name && std::strcmp(name, "llr_max_sweeps") == 0) *value = llr_get_max_sweeps();
     else if (name && std::strcmp(name, "sense_zconv_variant") == 0) *value = sense_fused_get_zconv_variant();
+    else if (name && std::strcmp(name, "nufft_locality") == 0) *value = nufft_get_locality();
     else {
         set_last_error(std::string("unknown option ") + (name ? name : "(null)"));
         return 1;

@@ -32,6 +32,9 @@ namespace {
 // (20^3 cells, one workgroup per CU) are costlier to clear and flush, so they take more
 inline int chunk_cap(bool fp64) { return fp64 ? 1024 : 256; }
 int g_spread_variant = 3;  // 3-D spreader, values as nufft_set_spread_variant
+// point ordering of new plans: bit 0 super-tile bin order, bit 1 XCD-aware block order;
+// 4 (default): 3 for type-1 plans (spreading), 0 for type-2 plans (interpolation)
+int g_locality = 4;
 constexpr int kThreads = 256;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
@@ -71,11 +74,71 @@ template <typename R> __device__ __forceinline__ R es_eval(R z, R beta, R c)
     return t > (R)0 ? exp(beta * (sqrt(t) - (R)1)) : (R)0;
 }
 
+// Bin numbering: bins are grouped into super-tiles of kSuper bins per active dimension
+// (sup[d] super-tiles, g[d] = kSuper or 1 bins each), numbered super-tile by super-tile,
+// so the points of neighbouring tiles sit next to each other in the sorted order and the
+// fine-grid cells they share stay in L2 between them.
+constexpr int kSuper = 4;
+struct BinOrder {
+    int g[3];
+    int64_t sup[3];
+};
+__host__ __device__ __forceinline__ int64_t bin_index(const BinOrder& o, const int64_t* b)
+{
+    int64_t s = 0, l = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        s = s * o.sup[d] + b[d] / o.g[d];
+        l = l * o.g[d] + b[d] % o.g[d];
+    }
+    return s * (o.g[0] * o.g[1] * o.g[2]) + l;
+}
+__host__ __device__ __forceinline__ void bin_coords(const BinOrder& o, int64_t idx, int64_t* b)
+{
+    const int64_t G = o.g[0] * o.g[1] * o.g[2];
+    int64_t s = idx / G, l = idx - (idx / G) * G;
+#pragma unroll
+    for (int d = 2; d >= 0; --d) {
+        b[d] = (s % o.sup[d]) * o.g[d] + l % o.g[d];
+        s /= o.sup[d];
+        l /= o.g[d];
+    }
+}
+BinOrder bin_order(const int64_t* nbins, int super)
+{
+    BinOrder o;
+    for (int d = 0; d < 3; ++d) {
+        o.g[d] = nbins[d] > 1 ? super : 1;
+        o.sup[d] = ceil_div(nbins[d], o.g[d]);
+    }
+    return o;
+}
+// Number of bin indices (bins of partial super-tiles included; they stay empty).
+int64_t bin_range(const int64_t* nbins, int super)
+{
+    const BinOrder o = bin_order(nbins, super);
+    return o.sup[0] * o.sup[1] * o.sup[2] * o.g[0] * o.g[1] * o.g[2];
+}
+
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup i to XCD i mod 8), and each
+// XCD has its own L2. For a grid padded to a multiple of 8, this maps workgroup i to the
+// block the XCD's own contiguous eighth of the work, so neighbouring blocks (neighbouring
+// tiles) share an L2. A bijection on [0, gridDim.x).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_block(int xcd)
+{
+    if (!xcd) return blockIdx.x;
+    const int64_t per = gridDim.x / kXcds;
+    return (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+inline int64_t xcd_grid(int64_t nblocks, int xcd) { return xcd ? ceil_div(nblocks, kXcds) * kXcds : nblocks; }
+
 template <typename R> struct PtsArgs {
     const R* pts[3];
     int64_t nf[3];
     int bs[3];
     int64_t nbins[3];
+    BinOrder ord;
     int64_t M;
 };
 
@@ -93,7 +156,7 @@ template <typename R> __device__ __forceinline__ int64_t point_bin(const PtsArgs
             b[d] = 0;
         }
     }
-    return (b[0] * a.nbins[1] + b[1]) * a.nbins[2] + b[2];
+    return bin_index(a.ord, b);
 }
 
 template <typename R> __global__ void bin_count_kernel(PtsArgs<R> a, int* counts, int* rank)
@@ -181,6 +244,8 @@ template <typename R> struct SpreadArgs {
     int rp;  // LDS tile pitches (complex elements) between rows (last axis) and planes
     int pp;
     int maxsub;  // points per subproblem at most
+    int64_t nsub;  // subproblems (with xcd the grid is padded to a multiple of the XCD count)
+    int xcd;       // XCD-aware block order (xcd_block)
 };
 
 // LDS tile pitches for tile extents T and kernel width ns. With e complex elements
@@ -208,19 +273,14 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(SpreadArgs<R> a)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<R>* tile = reinterpret_cast<cplx<R>*>(smem_raw);
     const PtsArgs<R>& pa = a.pa;
-    const int sp = blockIdx.x;
+    const int64_t sp = xcd_block(a.xcd);
+    if (sp >= a.nsub) return;
     const int bin = a.subbin[sp];
     const int start = a.binstart[bin] + a.suboff[sp];
     const int end = min(a.binstart[bin + 1], start + a.maxsub);
 
     int64_t bb[3];
-    {
-        int64_t r = bin;
-        bb[2] = r % pa.nbins[2];
-        r /= pa.nbins[2];
-        bb[1] = r % pa.nbins[1];
-        bb[0] = r / pa.nbins[1];
-    }
+    bin_coords(pa.ord, bin, bb);
     int T[3];
     int64_t o[3];
 #pragma unroll
@@ -331,7 +391,8 @@ __global__ void __launch_bounds__(64 * NW) spread_rows_kernel(SpreadArgs<R> a)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<R>* tile = reinterpret_cast<cplx<R>*>(smem_raw);
     const PtsArgs<R>& pa = a.pa;
-    const int sp = blockIdx.x;
+    const int64_t sp = xcd_block(a.xcd);
+    if (sp >= a.nsub) return;
     const int bin = a.subbin[sp];
     const int start = a.binstart[bin] + a.suboff[sp];
     const int end = min(a.binstart[bin + 1], start + a.maxsub);
@@ -340,13 +401,7 @@ __global__ void __launch_bounds__(64 * NW) spread_rows_kernel(SpreadArgs<R> a)
     const int wave = tid >> 6;
 
     int64_t bb[3];
-    {
-        int64_t r = bin;
-        bb[2] = r % pa.nbins[2];
-        r /= pa.nbins[2];
-        bb[1] = r % pa.nbins[1];
-        bb[0] = r / pa.nbins[1];
-    }
+    bin_coords(pa.ord, bin, bb);
     int T[3];
     int64_t o[3];
 #pragma unroll
@@ -431,7 +486,7 @@ template <typename R, int NS, int DIM>
 __global__ void __launch_bounds__(kThreads) interp_kernel(SpreadArgs<R> a)
 {
     const PtsArgs<R>& pa = a.pa;
-    const int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t p = xcd_block(a.xcd) * kThreads + threadIdx.x;
     if (p >= pa.M) return;
     const int j = a.sortidx[p];
     R w[2][NS];
@@ -505,30 +560,34 @@ void launch_spread_ns(const SpreadArgs<R>& a, int64_t nsub, size_t smem, hipStre
         if (variant == 1) {
             allow_lds(spread_rows_kernel<R, NS, 1, true>, smem);
             KernelTimer t("nufft_spread", s);
-            hipLaunchKernelGGL((spread_rows_kernel<R, NS, 1, true>), dim3((unsigned)nsub), dim3(64), smem, s, a);
+            hipLaunchKernelGGL((spread_rows_kernel<R, NS, 1, true>), dim3((unsigned)xcd_grid(nsub, a.xcd)), dim3(64), smem, s,
+                               a);
             return;
         }
         if (variant == 2) {
             allow_lds(spread_rows_kernel<R, NS, 4, false>, smem);
             KernelTimer t("nufft_spread", s);
-            hipLaunchKernelGGL((spread_rows_kernel<R, NS, 4, false>), dim3((unsigned)nsub), dim3(256), smem, s, a);
+            hipLaunchKernelGGL((spread_rows_kernel<R, NS, 4, false>), dim3((unsigned)xcd_grid(nsub, a.xcd)), dim3(256), smem,
+                               s, a);
             return;
         }
         if (variant == 3) {
             allow_lds(spread_rows_kernel<R, NS, 4, true>, smem);
             KernelTimer t("nufft_spread", s);
-            hipLaunchKernelGGL((spread_rows_kernel<R, NS, 4, true>), dim3((unsigned)nsub), dim3(256), smem, s, a);
+            hipLaunchKernelGGL((spread_rows_kernel<R, NS, 4, true>), dim3((unsigned)xcd_grid(nsub, a.xcd)), dim3(256), smem,
+                               s, a);
             return;
         }
     }
     allow_lds(spread_kernel<R, NS, DIM>, smem);
     KernelTimer t("nufft_spread", s);
-    hipLaunchKernelGGL((spread_kernel<R, NS, DIM>), dim3((unsigned)nsub), dim3(kThreads), smem, s, a);
+    hipLaunchKernelGGL((spread_kernel<R, NS, DIM>), dim3((unsigned)xcd_grid(nsub, a.xcd)), dim3(kThreads), smem, s, a);
 }
 template <typename R, int NS, int DIM> void launch_interp_ns(const SpreadArgs<R>& a, int64_t M, hipStream_t s)
 {
     KernelTimer t("nufft_interp", s);
-    hipLaunchKernelGGL((interp_kernel<R, NS, DIM>), dim3((unsigned)ceil_div(M, kThreads)), dim3(kThreads), 0, s, a);
+    hipLaunchKernelGGL((interp_kernel<R, NS, DIM>), dim3((unsigned)xcd_grid(ceil_div(M, kThreads), a.xcd)), dim3(kThreads), 0,
+                       s, a);
 }
 
 template <typename R, int DIM>
@@ -647,6 +706,7 @@ template <typename R> PtsArgs<R> pts_args(const NufftPlan* p)
         a.bs[d] = p->bs[d];
         a.nbins[d] = p->nbins[d];
     }
+    a.ord = bin_order(p->nbins, (p->locality & 1) ? kSuper : 1);
     a.M = p->M;
     return a;
 }
@@ -655,6 +715,8 @@ template <typename R> PtsArgs<R> pts_args(const NufftPlan* p)
 
 void nufft_set_spread_variant(int v) { g_spread_variant = v; }
 int nufft_get_spread_variant() { return g_spread_variant; }
+void nufft_set_locality(int v) { g_locality = v; }
+int nufft_get_locality() { return g_locality; }
 
 void es_params(double tol, int* ns, double* beta)
 {
@@ -730,7 +792,8 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int iflag, i
             else
                 twiddles<float>((int)p->nf[d]);
         }
-        const int64_t nb = p->nbins[0] * p->nbins[1] * p->nbins[2];
+        p->locality = g_locality <= 3 ? g_locality : (type == 1 ? 3 : 0);
+        const int64_t nb = bin_range(p->nbins, (p->locality & 1) ? kSuper : 1);
         p->counts = (int*)dev_alloc(sizeof(int) * nb);
         p->binstart = (int*)dev_alloc(sizeof(int) * (nb + 1));
         p->nsubbin = (int*)dev_alloc(sizeof(int) * (nb + 1));
@@ -761,7 +824,7 @@ namespace {
 
 template <typename R> void setpts_typed(NufftPlan* p, hipStream_t s)
 {
-    const int64_t nb = p->nbins[0] * p->nbins[1] * p->nbins[2];
+    const int64_t nb = bin_range(p->nbins, (p->locality & 1) ? kSuper : 1);
     PtsArgs<R> a = pts_args<R>(p);
     launch_fill(p->counts, nb, 0, s);
     if (p->M > 0) {
@@ -825,6 +888,8 @@ template <typename R> SpreadArgs<R> spread_args(const NufftPlan* p)
     for (int d = 0; d < 3; ++d) T[d] = (d >= 3 - p->dim) ? p->bs[d] + p->ns + 1 : 1;
     tile_pitches(T, p->ns, (int)sizeof(cplx<R>), pad_tile(p), a.rp, a.pp);
     a.maxsub = chunk_cap(p->fp64 != 0);
+    a.nsub = p->nsub;
+    a.xcd = (p->locality & 2) ? 1 : 0;
     return a;
 }
 
@@ -979,7 +1044,7 @@ void nufft_setpts(NufftPlan* p, int64_t M, const void* x, const void* y, const v
         p->pts[2 - d] = xyz[d];
     }
     p->M = M;
-    const int64_t nb = p->nbins[0] * p->nbins[1] * p->nbins[2];
+    const int64_t nb = bin_range(p->nbins, (p->locality & 1) ? kSuper : 1);
     const int64_t need = std::max<int64_t>(M, nb);
     if (need > p->cap_M) {
         dev_free(p->rank);

@@ -39,6 +39,7 @@ struct NufftPlan {
     int64_t nsub = 0;
     int device = 0;
     int spread_variant = -1;  // 3-D spreader of this plan (-1: the library option)
+    int locality = 3;         // bit 0 super-tile bin order, bit 1 XCD-aware blocks (fixed at makeplan)
 };
 
 // nmodes in FINUFFT order (fastest-varying first), as the reference passes
@@ -55,6 +56,11 @@ void nufft_destroy(NufftPlan* p);
 // flushing the tile while the first wave accumulates without atomics
 void nufft_set_spread_variant(int v);
 int nufft_get_spread_variant();
+// Point ordering of new plans: bit 0 groups bins into 4^3 super-tiles, bit 1 maps the
+// spreading and interpolation workgroups to XCDs in contiguous runs; 4 (default): 3 for
+// type-1 plans, 0 for type-2 plans (measured best for spreading and for interpolation).
+void nufft_set_locality(int v);
+int nufft_get_locality();
 
 // Time the 3-D type-1 spreaders on the plan's points (c -> f, f overwritten) and keep
 // the fastest for this plan; returns it and its execute time in *ms. Other plans: no-op

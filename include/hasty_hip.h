@@ -125,6 +125,10 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
  *     coset elsewhere, 3 coset with an LDS-DMA inverse), set before preparing kernels;
  *   "sense_zconv_batch" 1..16 (default 16): problems per z-convolution launch of the fused
  *     normal operator at N0 = 128.
+ *   "nufft_locality" 0..4 (default 4): point ordering of plans made after the call: bit 0
+ *     numbers the bins by 4^3 super-tiles, bit 1 hands the spreading and interpolation
+ *     workgroups to the XCDs in contiguous runs; 4 picks 3 for type-1 and 0 for type-2
+ *     plans.
  *   "sense_zconv_variant" 0..3 (default 2): z-convolution kernel at N0 = 128 (0: 16
  *     columns per workgroup, one LDS transpose buffer; 1: 8 columns; 2: 16 columns, two
  *     buffers; 3: 8 columns, two buffers).

@@ -151,6 +151,66 @@ def test_nufft_type1_spread_variants(N, variant, traj, tol, bound):
     assert e < bound
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("fp64,tol,bound", [(True, 1e-10, 1e-8), (False, 1e-8, 1e-5)])
+def test_nufft_type1_spread_variants_fp64_and_wide(N, variant, fp64, tol, bound):
+    # The spreaders with complex128 data (ns = 11: 121 window rows, two per lane in the
+    # window-row spreaders) and complex64 with a kernel wider than 8 (tol 1e-8: ns = 9),
+    # on a kooshball with every spoke's k = 0 sample at the origin, against the direct sum.
+    from hastycompute_amd import _capi
+
+    n = 20
+    co = O.kooshball(250, 20)
+    rng = np.random.default_rng(12)
+    y = rnd(rng, co.shape[1])
+    rt, ct = (torch.float64, torch.complex128) if fp64 else (torch.float32, torch.complex64)
+    co_used = co if fp64 else co.astype(np.float32).astype(np.float64)
+    prev = _capi.get_option("nufft_spread_variant")
+    _capi.set_option("nufft_spread_variant", variant)
+    try:
+        plan = N.NufftPlan(g(co, rt), [n, n, n], 1, +1, 1, tol)
+        ns = plan.info()[1]
+        img = torch.empty((1, n, n, n), dtype=ct, device="cuda")
+        plan.execute(g(y[None], ct), img)
+        img = h(img)[0]
+        plan.close()
+    finally:
+        _capi.set_option("nufft_spread_variant", prev)
+    e = O.rel_l2(img, O.nudft_type1(co_used, y, (n, n, n), +1))
+    print(f"spread variant {variant} {'fp64' if fp64 else 'fp32'} tol {tol} (ns {ns}): {e:.2e}", flush=True)
+    assert ns >= 9 and e < bound
+
+
+@pytest.mark.parametrize("locality", [0, 1, 2, 3])
+@pytest.mark.parametrize("traj", ["kooshball", "pileup"])
+def test_nufft_point_orders(N, locality, traj):
+    # Every point ordering (bins by super-tile or in plain order, workgroups XCD-aware or
+    # not): type 2 and type 1 against the direct sums.
+    from hastycompute_amd import _capi
+
+    rng = np.random.default_rng(13)
+    n = 28
+    if traj == "kooshball":
+        co = O.kooshball(400, 28)
+    else:
+        p0 = rng.uniform(-np.pi, np.pi, (3, 1))
+        co = np.concatenate([np.repeat(p0, 1500, axis=1), rng.uniform(-np.pi, np.pi, (3, 3000))], axis=1)
+    co32 = co.astype(np.float32).astype(np.float64)
+    f = rnd(rng, (n, n, n))
+    y = rnd(rng, co.shape[1])
+    prev = _capi.get_option("nufft_locality")
+    _capi.set_option("nufft_locality", locality)
+    try:
+        out2 = h(N.nufft(g(co, torch.float32), g(f[None]), [n, n, n], 2, -1, 1e-6))[0]
+        out1 = h(N.nufft(g(co, torch.float32), g(y[None]), [n, n, n], 1, +1, 1e-6))[0]
+    finally:
+        _capi.set_option("nufft_locality", prev)
+    e2 = O.rel_l2(out2, O.nudft_type2(co32, f, -1))
+    e1 = O.rel_l2(out1, O.nudft_type1(co32, y, (n, n, n), +1))
+    print(f"locality {locality} {traj}: type 2 {e2:.2e}, type 1 {e1:.2e}", flush=True)
+    assert e2 < 1e-5 and e1 < 1e-5
+
+
 @pytest.mark.parametrize("nm", [[1, 8, 6, 10], [2, 6, 6, 6], [1, 12, 9], [3, 20]])
 def test_toeplitz_diag_and_apply_modes(N, nm):
     rng = np.random.default_rng(sum(nm))
