@@ -1,5 +1,6 @@
 // extern "C" entry points of libhasty_hip.so (declared in include/hasty_hip.h) and the
 // host-side drivers behind them (LLR block grid, workspace layout).
+#include <atomic>
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -21,7 +22,7 @@ namespace hasty_hip {
 
 namespace {
 thread_local std::string g_last_error;
-int g_opt_sense_fused = 1;
+std::atomic<int> g_opt_sense_fused{1};
 
 template <typename Fn> int guarded(Fn&& fn)
 {

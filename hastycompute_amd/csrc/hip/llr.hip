@@ -13,6 +13,7 @@
 //             the threshold are rebuilt from them, then W = V diag(g) V^H;
 //   3. apply: MFMA A <- A W in place, 16-row slabs staged through LDS.
 // Blocks are read and written directly in the image; no gather/scatter copies.
+#include <atomic>
 #include <algorithm>
 #include <vector>
 
@@ -27,7 +28,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr int kThreads = 256;
 constexpr int kMaxTilesPerWave = 9;  // F <= 128 -> 8x8 tiles -> 36 upper tiles / 4 waves
 constexpr int kMaxSweeps = 15;       // Jacobi sweeps per block (4-6 reach FP32 rounding)
-int g_max_sweeps = kMaxSweeps;       // cap in force (option "llr_max_sweeps")
+std::atomic<int> g_max_sweeps{kMaxSweeps};  // cap in force (option "llr_max_sweeps")
 
 __device__ __forceinline__ int64_t row_addr(const LlrGeom& g, int3 c, int64_t r)
 {

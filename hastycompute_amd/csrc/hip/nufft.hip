@@ -15,6 +15,7 @@
 // banks: points at one position (every spoke of a radial trajectory crosses k = 0)
 // then update different words at any instant instead of serialising on one. Interpolation
 // walks points in tile order so the w^3 gathers hit L2.
+#include <atomic>
 #include <algorithm>
 #include <type_traits>
 #include <cmath>
@@ -31,10 +32,10 @@ namespace {
 // (the k-space centre of a radial trajectory) spreads over many CUs; complex128 tiles
 // (20^3 cells, one workgroup per CU) are costlier to clear and flush, so they take more
 inline int chunk_cap(bool fp64) { return fp64 ? 1024 : 256; }
-int g_spread_variant = 3;  // 3-D spreader, values as nufft_set_spread_variant
+std::atomic<int> g_spread_variant{3};  // 3-D spreader, values as nufft_set_spread_variant
 // point ordering of new plans: bit 0 super-tile bin order, bit 1 XCD-aware block order;
 // 4 (default): 3 for type-1 plans (spreading), 0 for type-2 plans (interpolation)
-int g_locality = 4;
+std::atomic<int> g_locality{4};
 constexpr int kThreads = 256;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
@@ -623,7 +624,7 @@ void dispatch_ns(bool spread, int ns, const SpreadArgs<R>& a, int64_t n, size_t 
 }
 
 // the plan's 3-D spreader (nufft_autotune) or else the library option
-int spread_variant(const NufftPlan* p) { return p->spread_variant >= 0 ? p->spread_variant : g_spread_variant; }
+int spread_variant(const NufftPlan* p) { return p->spread_variant >= 0 ? p->spread_variant : g_spread_variant.load(); }
 
 template <typename R> void dispatch_dim(bool spread, const NufftPlan* p, const SpreadArgs<R>& a, int64_t n, size_t smem,
                                         hipStream_t s)
@@ -792,7 +793,8 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int iflag, i
             else
                 twiddles<float>((int)p->nf[d]);
         }
-        p->locality = g_locality <= 3 ? g_locality : (type == 1 ? 3 : 0);
+        const int loc = g_locality.load();
+        p->locality = loc <= 3 ? loc : (type == 1 ? 3 : 0);
         const int64_t nb = bin_range(p->nbins, (p->locality & 1) ? kSuper : 1);
         p->counts = (int*)dev_alloc(sizeof(int) * nb);
         p->binstart = (int*)dev_alloc(sizeof(int) * (nb + 1));

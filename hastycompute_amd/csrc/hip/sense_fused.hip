@@ -22,6 +22,7 @@
 //              conj(S_c) multiply and the coil sum, all accumulated in registers; the
 //              output plane is written once.
 // The kernel K is stored in the same permuted (s, ky, m) order as T.
+#include <atomic>
 #include <vector>
 
 #include "common.hpp"
@@ -1551,13 +1552,13 @@ namespace {
 // register kernels where N1 = N2 = 128 and the coset kernels elsewhere, 3 coset kernels
 // with the inverse fed by direct-to-LDS loads. The kernel permutation follows the
 // variant, so it is set before kernels are prepared.
-int g_xy_variant = 2;
-int g_zconv_batch = 16;  // problems per z-convolution launch (N0 = 128), 1..16
+std::atomic<int> g_xy_variant{2};
+std::atomic<int> g_zconv_batch{16};  // problems per z-convolution launch (N0 = 128), 1..16
 // z-convolution at N0 = 128: 0 16 columns x 256 threads, one LDS buffer (four barriers
 // per coil); 1 8 columns x 128 threads; 2 (default) 16 columns with two buffers (two
 // barriers per coil); 3 8 columns with two buffers. At 128^3 x 16 coils, 4 problems per
 // launch: 1.85, 3.58, 1.66 and 3.59 ms (8 columns leave 64-byte row segments per wave)
-int g_zconv_variant = 2;
+std::atomic<int> g_zconv_variant{2};
 
 bool use_r16(int N) { return g_xy_variant == 2 && N == 128; }
 }  // namespace
@@ -1666,8 +1667,9 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
             }
             if (L0 == 256) {
                 // one launch per up to g_zconv_batch problems of the chunk
-                for (int q0 = 0; q0 < np; q0 += g_zconv_batch) {
-                    const int nq = std::min(g_zconv_batch, np - q0);
+                const int zbatch = g_zconv_batch.load();
+                for (int q0 = 0; q0 < np; q0 += zbatch) {
+                    const int nq = std::min(zbatch, np - q0);
                     ZKernels zk{};
                     for (int j = 0; j < nq; ++j)
                         zk.k[j] = kernels + (int64_t)kidx[p0 + q0 + j] * L0 * (int64_t)L * L;

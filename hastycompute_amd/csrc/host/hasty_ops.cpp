@@ -846,6 +846,7 @@ void Random3DBlocksSVT::apply(at::Tensor in, int64_t nblocks, std::array<int64_t
             corners.push_back((int32_t)dist(rng));
         }
     }
+    _last = corners;
     on_gpu(in, [&](at::Tensor& g) {
         std::vector<int64_t> dims = g.sizes().vec();
         std::vector<int64_t> shp(block_shape.begin(), block_shape.end());

@@ -352,6 +352,13 @@ class Random3DBlocksSVT : public BlocksSVTBase {
 public:
     using BlocksSVTBase::BlocksSVTBase;
     void apply(at::Tensor in, int64_t nblocks, std::array<int64_t, 3> block_shape, double thresh, bool soft);
+    // corners [nblocks][3] (x, y, z) of the last apply's blocks, in the order they were
+    // thresholded (no counterpart in the reference: the draws are not reproducible, so
+    // this is how a caller can replay them)
+    const std::vector<int32_t>& last_corners() const { return _last; }
+
+private:
+    std::vector<int32_t> _last;
 };
 
 // svt.cpp:354-449 (the TorchScript target of HastySVT.Normal3DBlocksSVT).

@@ -244,6 +244,12 @@ public:
     {
         catcher([&] { _op->apply(in, nblocks, block_shape, thresh, soft); });
     }
+    at::Tensor last_corners() const
+    {
+        const auto& c = _op->last_corners();
+        std::vector<int64_t> v(c.begin(), c.end());
+        return torch::tensor(v, torch::kInt64).reshape({(int64_t)c.size() / 3, 3});
+    }
 
 private:
     std::unique_ptr<svt::Random3DBlocksSVT> _op;
@@ -389,7 +395,10 @@ TORCH_LIBRARY(HastyBatchedSense, m)
 
 TORCH_LIBRARY(HastySVT, m)
 {
-    m.class_<Random3DBlocksSVT>("Random3DBlocksSVT").def(torch::init<const Streams&>()).def("apply", &Random3DBlocksSVT::apply);
+    m.class_<Random3DBlocksSVT>("Random3DBlocksSVT")
+        .def(torch::init<const Streams&>())
+        .def("apply", &Random3DBlocksSVT::apply)
+        .def("last_corners", &Random3DBlocksSVT::last_corners);
     m.class_<Normal3DBlocksSVT>("Normal3DBlocksSVT").def(torch::init<const Streams&>()).def("apply", &Normal3DBlocksSVT::apply);
     m.def("doc", []() -> std::string {
         return "Random3DBlocksSVT(streams?).apply(in [F,E,X,Y,Z], nblocks, block_shape, thresh, soft); "

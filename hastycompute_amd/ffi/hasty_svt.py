@@ -20,6 +20,11 @@ class Random3DBlocksSVT:
     def apply(self, input: torch.Tensor, nblocks: int, block_shape: list[int], thresh: float, soft: bool):
         self._svtop.apply(input, nblocks, block_shape, thresh, soft)
 
+    def last_corners(self) -> torch.Tensor:
+        """[nblocks, 3] corners (x, y, z) of the last apply, in the order they were
+        thresholded (an addition: the reference's draws cannot be replayed)."""
+        return self._svtop.last_corners()
+
 
 class Normal3DBlocksSVT:
     def __init__(self, streams: list[torch.Stream] | None = None):

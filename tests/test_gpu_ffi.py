@@ -266,6 +266,21 @@ def test_ffi_random3d_blocks_svt(F):
     zero = (t.abs().amax(dim=(0, 1)) == 0).numpy()
     assert 64 <= zero.sum() <= 3 * 64
     assert t.abs().amax() > 0
+    # seeded comparison: the blocks the op drew (overlapping ones included), replayed in
+    # the same order through the oracle's svt_soft on the Casorati matrices
+    x = (rnd(rng, (6, 2, 14, 14, 14)) + 2.0).astype(np.complex64)
+    t = torch.as_tensor(x.copy())
+    op = F.svt.Random3DBlocksSVT(None)
+    op.apply(t, 12, [5, 5, 5], 3.0, True)
+    corners = op.last_corners().numpy()
+    assert corners.shape == (12, 3) and (corners >= 0).all() and (corners <= 14 - 5 - 1).all()
+    ref = x.astype(np.complex128)
+    for c in corners:
+        blk = O.extract_block(ref, c, [5, 5, 5])
+        O.insert_block(ref, c, [5, 5, 5], O.svt_soft(blk, 3.0))
+    e = O.rel_l2(t.numpy(), ref)
+    print(f"Random3DBlocksSVT replayed through the oracle: {e:.2e}", flush=True)
+    assert e < 1e-4 and O.rel_l2(ref, x) > 1e-3
 
 
 def test_ffi_cpp_power_iteration_and_cg(F):

@@ -386,14 +386,18 @@ def _lowrank(rng, F, E, X, rank, noise):
     return ((t @ sp).reshape(F, E, X, X, X) + noise * rnd(rng, (F, E, X, X, X))) / math.sqrt(rank)
 
 
-@pytest.mark.parametrize("F,E,X,blk,it,soft", [(20, 1, 32, 8, 3, True), (12, 2, 24, 8, 2, True),
-                                              (20, 1, 32, 8, 2, False), (100, 1, 32, 16, 2, True)])
-def test_llr_normal_blocks_vs_oracle(N, F, E, X, blk, it, soft):
+@pytest.mark.parametrize("F,E,X,blk,it,soft,rel", [(20, 1, 32, 8, 3, True, 0.05), (12, 2, 24, 8, 2, True, 0.05),
+                                                  (20, 1, 32, 8, 2, False, 0.0), (100, 1, 32, 16, 2, True, 0.05),
+                                                  (100, 1, 48, 16, 2, True, 0.01)])
+def test_llr_normal_blocks_vs_oracle(N, F, E, X, blk, it, soft, rel):
+    # rel = 0.01: the LLR-regularised 4D-flow configuration's threshold, 0.01 sigma_max,
+    # on its 16^3 blocks of 100 frames (4096 x 100 Casorati matrices), where most small
+    # singular values survive the threshold and the kept eigenvectors are many
     rng = np.random.default_rng(F + X)
     x = _lowrank(rng, F, E, X, 5, 0.05).astype(np.complex64)
     A = O.extract_block(x.astype(np.complex128), (0, 0, 0), (blk,) * 3)
     smax = np.linalg.svd(A, compute_uv=False)[0]
-    thresh = 0.05 * smax if soft else 3
+    thresh = rel * smax if soft else 3
     ref = O.normal_3d_blocks_svt(x.astype(np.complex128), (blk,) * 3, (blk,) * 3, it, thresh, soft)
     xg = g(x)
     N.llr_normal_blocks(xg, (blk,) * 3, (blk,) * 3, it, thresh, soft)
