@@ -860,8 +860,9 @@ def main():
         "vs_baseline": None,
         "dtype": "c64 (fp32)",
         "data": "synthetic (seeded kooshball trajectories, Gaussian x linear-phase coil maps, random images)",
-        "config": {"workload": "4D-flow batched normal op: 5 enc x 20 frames = 100 problems, 128^3, 16 coils, "
-                               "grid 256^3, 800-1500 spokes x 128 per frame",
+        "config": {"workload": f"4D-flow batched normal op: {args.encodings} enc x {args.frames} frames = {P} problems, "
+                               f"{args.n}^3, {args.coils} coils, grid {2 * args.n}^3, "
+                               f"{args.spokes_min}-{args.spokes_max} spokes x {args.readout} per frame",
                    "problems": P, "n": args.n, "coils": args.coils, "coil_batch": args.coil_batch,
                    "problem_batch": args.problem_batch, "path": "fused" if op.layout != 0 else "generic",
                    "kernel_layout": op.layout, "autotune": tune,

@@ -1,4 +1,5 @@
-// Fused Toeplitz SENSE normal operator for N1 = N2 = 32*NB (NB = 1..4), any N0.
+// Fused Toeplitz SENSE normal operator for N1 = N2 = 32*NB (NB = 1..5), any N0 whose
+// z line 2 N0 is 2,3,5-smooth and fits the z-convolution's LDS.
 //
 // For every problem p and coil c the operator is
 //   out[p] += conj(S_c) * crop( IFFT3( K[p] * FFT3( pad(S_c * x[p]) ) ) ),
@@ -758,13 +759,51 @@ template <int S> __device__ __forceinline__ void dft12(float2* v)
     }
 }
 
-// Natural-order DFT of length R in {4, 8, 12, 16} in registers.
-template <int R, int S> __device__ __forceinline__ void dftn(float2* v)
+// 20-point DFT in registers, in place: n = 5 n1 + n2, k = k1 + 4 k2,
+//   X[k1 + 4 k2] = sum_{n2} w20^{n2 k1} w5^{n2 k2} sum_{n1} v[5 n1 + n2] w4^{n1 k1};
+// the 4-point DFTs over n1 write back into the slots they read (n2 + 5 k1), the
+// 5-point DFTs over n2 then read and write slots 5 k1 + k2, so X[k] ends in slot
+// dft_slot<20>(k) = 5 (k mod 4) + k / 4. With HALF the inputs n >= 10 (n1 >= 2) are
+// zero and are not read.
+template <int S, bool HALF = false> __device__ __forceinline__ void dft20_dr(float2* v)
 {
+    // w20^m = exp(S 2 pi i m / 20), m = 0..9 (n2 k1 <= 12; m >= 10 as -w20^{m-10})
+    const float C[10] = {1.f, 0.95105651629515357f, 0.80901699437494742f, 0.58778525229247313f,
+                         0.30901699437494742f, 0.f, -0.30901699437494742f, -0.58778525229247313f,
+                         -0.80901699437494742f, -0.95105651629515357f};
+    const float Sn[10] = {0.f, 0.30901699437494742f, 0.58778525229247313f, 0.80901699437494742f,
+                          0.95105651629515357f, 1.f, 0.95105651629515357f, 0.80901699437494742f,
+                          0.58778525229247313f, 0.30901699437494742f};
+#pragma unroll
+    for (int n2 = 0; n2 < 5; ++n2) {
+        float2 q[4] = {v[n2], v[5 + n2], HALF ? float2{0.f, 0.f} : v[10 + n2], HALF ? float2{0.f, 0.f} : v[15 + n2]};
+        dft4_first<S, HALF>(q);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            const int mm = n2 * k1;
+            const float sg = mm >= 10 ? -1.f : 1.f;
+            const int m10 = mm >= 10 ? mm - 10 : mm;
+            v[n2 + 5 * k1] = mm == 0 ? q[k1] : cmul(q[k1], float2{sg * C[m10], sg * (float)S * Sn[m10]});
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) dft5<S>(v + 5 * k1);
+}
+
+// Register DFTs of length R: natural order for R in {4, 8, 12, 16}; for R = 20 the
+// in-place form above, with output k in slot dft_slot<R>(k). With HALF (R = 20 only)
+// the upper half of the inputs is zero and is not read.
+template <int R> __host__ __device__ constexpr int dft_slot(int k) { return R == 20 ? 5 * (k % 4) + k / 4 : k; }
+
+template <int R, int S, bool HALF = false> __device__ __forceinline__ void dftn(float2* v)
+{
+    static_assert(R == 4 || R == 8 || R == 12 || R == 16 || R == 20, "unsupported register DFT length");
+    static_assert(!HALF || R == 20, "HALF is implemented for R = 20");
     if constexpr (R == 4) dft4<S>(v);
     else if constexpr (R == 8) dft8<S>(v);
     else if constexpr (R == 12) dft12<S>(v);
-    else dft16<S>(v);
+    else if constexpr (R == 16) dft16<S>(v);
+    else dft20_dr<S, HALF>(v);
 }
 
 // Plane kernels with the L-point y transforms as register four-step FFTs,
@@ -777,15 +816,26 @@ template <int R, int S> __device__ __forceinline__ void dftn(float2* v)
 // the plane (x S products, or the coil sum and conj(S)) plus two cosets of transforms.
 // Compared with the Stockham passes in LDS this moves each element through LDS about
 // three times instead of about ten.
+// At N = 160 (L = 320 = 16 x 20) one coset block is 80 KB, so one coset goes through a
+// step instead of two, and the y transforms over k1 (20 values) take threads r and r + 16.
 constexpr int kXYW = 512;
+
+// cosets per step of the coset kernels: two while both blocks fit in LDS (N <= 128)
+__host__ __device__ constexpr int cosets_per_step(int NB) { return NB <= 4 ? 2 : 1; }
+
+// LDS bytes of fwd_xy_w_kernel / inv_xy_w_kernel
+constexpr size_t coset_kernel_lds(int NB)
+{
+    return sizeof(float2) * ((size_t)cosets_per_step(NB) * 64 * NB * kW + 64 * NB + 32);
+}
 
 template <int NB> __global__ void __launch_bounds__(kXYW) fwd_xy_w_kernel(FusedArgs a)
 {
-    constexpr int N = 32 * NB, L = 64 * NB, P = 2 * NB, R2 = 4 * NB;
+    constexpr int N = 32 * NB, L = 64 * NB, P = 2 * NB, R2 = 4 * NB, CS = cosets_per_step(NB);
     constexpr int CW = L * kW;  // one coset block: L rows x 32 columns
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    float2* buf = reinterpret_cast<float2*>(smem_raw);  // [2][CW]
-    float2* twL = buf + 2 * CW;
+    float2* buf = reinterpret_cast<float2*>(smem_raw);  // [CS][CW]
+    float2* twL = buf + CS * CW;
     float2* tw32 = twL + L;
     const int t = threadIdx.x;
     const int b = t & 31;   // x step: lane of the half-wave (x index b + 32 j)
@@ -815,11 +865,11 @@ template <int NB> __global__ void __launch_bounds__(kXYW) fwd_xy_w_kernel(FusedA
                 v[k][j] = cmul(xp[idx], sp[idx]);
             }
         float2* Tp = a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L;
-        for (int s0 = 0; s0 < P; s0 += 2) {
+        for (int s0 = 0; s0 < P; s0 += CS) {
             // x step: for coset s the 32 frequencies P m + s of each row, written as rows
             // y < N of the coset block (column m)
 #pragma unroll
-            for (int cc = 0; cc < 2; ++cc) {
+            for (int cc = 0; cc < CS; ++cc) {
                 const int s = s0 + cc;
                 const float2 w1 = twL[(s * b) % L];
                 const float2 w32 = twL[(s * 32) % L];
@@ -839,30 +889,32 @@ template <int NB> __global__ void __launch_bounds__(kXYW) fwd_xy_w_kernel(FusedA
             // 16 k1 + r of its own column: the same rows (mod 16), touched by no other
             // thread, so the transform is in place without a barrier.
 #pragma unroll 1
-            for (int cs = 0; cs < 2; ++cs) {
-                float2 u[16];
+            for (int cs = 0; cs < CS; ++cs) {
+                float2 u[R2];
 #pragma unroll
                 for (int y2 = 0; y2 < R2; ++y2)
                     u[y2] = y2 < R2 / 2 ? buf[cs * CW + (r + 16 * y2) * kW + m] : float2{0.f, 0.f};
-                dftn<R2, -1>(u);
+                dftn<R2, -1, R2 == 20>(u);
 #pragma unroll
-                for (int k1 = 1; k1 < R2; ++k1) u[k1] = cmul(u[k1], twL[(r * k1) % L]);
-#pragma unroll
-                for (int k1 = 0; k1 < R2; ++k1) buf[cs * CW + (k1 * 16 + r) * kW + m] = u[k1];
+                for (int k1 = 0; k1 < R2; ++k1) {
+                    const float2 f = u[dft_slot<R2>(k1)];
+                    buf[cs * CW + (k1 * 16 + r) * kW + m] = k1 == 0 ? f : cmul(f, twL[(r * k1) % L]);
+                }
             }
             lds_barrier();
-            // y step 2 (thread: column m, k1 = r < R2): DFT over y1 -> k2,
+            // y step 2 (thread: column m, k1 = r, r + 16, ... < R2): DFT over y1 -> k2,
             // ky = k1 + R2 k2, straight to HBM
-            if (r < R2) {
 #pragma unroll 1
-                for (int cs = 0; cs < 2; ++cs) {
+            for (int k1 = r; k1 < R2; k1 += 16) {
+#pragma unroll 1
+                for (int cs = 0; cs < CS; ++cs) {
                     float2 u[16];
 #pragma unroll
-                    for (int y1 = 0; y1 < 16; ++y1) u[y1] = buf[cs * CW + (r * 16 + y1) * kW + m];
+                    for (int y1 = 0; y1 < 16; ++y1) u[y1] = buf[cs * CW + (k1 * 16 + y1) * kW + m];
                     dft16<-1>(u);
                     float2* dst = Tp + (int64_t)(s0 + cs) * CW + m;
 #pragma unroll
-                    for (int k2 = 0; k2 < 16; ++k2) dst[(r + R2 * k2) * kW] = u[k2];
+                    for (int k2 = 0; k2 < 16; ++k2) dst[(k1 + R2 * k2) * kW] = u[k2];
                 }
             }
             lds_barrier();
@@ -870,29 +922,50 @@ template <int NB> __global__ void __launch_bounds__(kXYW) fwd_xy_w_kernel(FusedA
     }
 }
 
+// Row halves of the inverse coset kernel: at N = 160 a thread would hold 50 pixels of
+// coil sum and 50 of coil image (200 VGPRs) besides the transforms, so two workgroups
+// share a z-plane, each producing the rows y = y1 + 16 y2 with y2 in its half. Both run
+// the whole step 1 (the same T reads); they are workgroups i and i + 8 of the launch,
+// which dispatch sends to the same XCD at nearly the same time, so the second one's
+// reads are L2 hits.
+__host__ __device__ constexpr int inv_row_splits(int NB) { return NB == 5 ? 2 : 1; }
+
+// grid.x of the inverse coset kernel for N0 z-planes
+inline unsigned inv_w_grid_x(int NB, int N0)
+{
+    return inv_row_splits(NB) == 1 ? (unsigned)N0 : (unsigned)(16 * ((N0 + 7) / 8));
+}
+
 template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedArgs a)
 {
-    constexpr int N = 32 * NB, L = 64 * NB, P = 2 * NB, R2 = 4 * NB;
+    constexpr int N = 32 * NB, L = 64 * NB, P = 2 * NB, R2 = 4 * NB, CS = cosets_per_step(NB);
+    constexpr int HS = inv_row_splits(NB), KR = 2 * NB / HS;  // row halves; y2 values per thread
     constexpr int CW = L * kW;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    float2* buf = reinterpret_cast<float2*>(smem_raw);  // [2][16 y1][R2 k1][32 m]
-    float2* twL = buf + 2 * CW;
+    float2* buf = reinterpret_cast<float2*>(smem_raw);  // [CS][16 y1][R2 k1][32 m]
+    float2* twL = buf + CS * CW;
     float2* tw32 = twL + L;
     const int t = threadIdx.x;
     const int b = t & 31;   // step 2: lane (x index b + 32 j after the inverse DFT)
-    const int yl = t >> 5;  // step 2: pixel rows yl + 16 k, i.e. y1 = yl, y2 = k
+    const int yl = t >> 5;  // step 2: pixel rows yl + 16 (KR h + k), i.e. y1 = yl, y2 = KR h + k
     const int m = t & 31;   // step 1: column
-    const int r = t >> 5;   // step 1: k1 (< R2)
-    const int z = blockIdx.x;
+    const int r = t >> 5;   // step 1: k1 = r, r + 16, ... (< R2)
+    int z = blockIdx.x, h = 0;
+    if constexpr (HS == 2) {
+        z = 8 * (blockIdx.x / 16) + (blockIdx.x % 8);
+        h = (blockIdx.x / 8) % 2;
+        if (z >= a.N0) return;  // the whole workgroup: the last group of 8 planes may be short
+    }
+    const int y0 = KR * h;  // first y2 of this workgroup's rows
     const int pl = blockIdx.y;
     const int p = a.p0 + pl;
     for (int i = t; i < L; i += kXYW) twL[i] = a.twL[i];
     if (t < 32) tw32[t] = a.tw32[t];
     const int64_t plane = (int64_t)N * N;
     const LaneTw32 lt = lane_tw32(b, a.tw32);
-    float2 acc[2 * NB][NB];
+    float2 acc[KR][NB];
 #pragma unroll
-    for (int k = 0; k < 2 * NB; ++k)
+    for (int k = 0; k < KR; ++k)
 #pragma unroll
         for (int j = 0; j < NB; ++j) acc[k][j] = float2{0.f, 0.f};
     lds_barrier();
@@ -902,35 +975,37 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
         const int coil = a.coils[a.c0 + ci] + p * a.coil_pstride;
         const float2* sp = a.smaps + ((int64_t)coil * a.N0 + z) * plane;
         // this coil's image, summed over the cosets; times conj(S) once at the end
-        float2 ic[2 * NB][NB];
+        float2 ic[KR][NB];
 #pragma unroll
-        for (int k = 0; k < 2 * NB; ++k)
+        for (int k = 0; k < KR; ++k)
 #pragma unroll
             for (int j = 0; j < NB; ++j) ic[k][j] = float2{0.f, 0.f};
         const float2* Tp = a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L;
-        for (int s0 = 0; s0 < P; s0 += 2) {
-            // step 1 (thread: column m, k1 = r, both cosets): inverse DFT over k2 of
-            // X[k1 + R2 k2] -> y1, twiddle wL^{-y1 k1}, into LDS as [y1][k1][m]
-            if (r < R2) {
+        for (int s0 = 0; s0 < P; s0 += CS) {
+            // step 1 (thread: column m, k1 = r, r + 16, ..., each coset of the step):
+            // inverse DFT over k2 of X[k1 + R2 k2] -> y1, twiddle wL^{-y1 k1}, into LDS as
+            // [y1][k1][m]
 #pragma unroll 1
-                for (int cs = 0; cs < 2; ++cs) {
+            for (int k1 = r; k1 < R2; k1 += 16) {
+#pragma unroll 1
+                for (int cs = 0; cs < CS; ++cs) {
                     float2 u[16];
                     const float2* src = Tp + (int64_t)(s0 + cs) * CW + m;
 #pragma unroll
-                    for (int k2 = 0; k2 < 16; ++k2) u[k2] = src[(r + R2 * k2) * kW];
+                    for (int k2 = 0; k2 < 16; ++k2) u[k2] = src[(k1 + R2 * k2) * kW];
                     dft16<1>(u);
 #pragma unroll
-                    for (int q = 1; q < 16; ++q) u[q] = cmulc(twL[(q * r) % L], u[q]);
+                    for (int q = 1; q < 16; ++q) u[q] = cmulc(twL[(q * k1) % L], u[q]);
 #pragma unroll
-                    for (int q = 0; q < 16; ++q) buf[cs * CW + (q * R2 + r) * kW + m] = u[q];
+                    for (int q = 0; q < 16; ++q) buf[cs * CW + (q * R2 + k1) * kW + m] = u[q];
                 }
             }
             lds_barrier();
-            // step 2 (thread: rows yl + 16 k, lane b): inverse DFT over k1 -> y2 = k for
+            // step 2 (thread: rows yl + 16 y2, lane b): inverse DFT over k1 -> y2 for
             // column m = brev5(b); inverse 32-point DFT across lanes; coset twiddle,
             // conj(S) multiply, coil sum
 #pragma unroll 1
-            for (int cc = 0; cc < 2; ++cc) {
+            for (int cc = 0; cc < CS; ++cc) {
                 const int s = s0 + cc;
                 float2 u[R2];
 #pragma unroll
@@ -940,8 +1015,12 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
 #pragma unroll
                 for (int j = 0; j < NB; ++j) w[j] = twid<1>(twL, (s * (b + 32 * j)) % L);
 #pragma unroll
-                for (int k = 0; k < 2 * NB; ++k) {
-                    const float2 val = dft32_inv_r(u[k], lt);
+                for (int k = 0; k < KR; ++k) {
+                    // y2 = y0 + k; with row halves the half is chosen by a select, so the
+                    // register array is indexed with constants only
+                    const float2 xk = HS == 1 ? u[dft_slot<R2>(k)]
+                                              : (h ? u[dft_slot<R2>(KR + k)] : u[dft_slot<R2>(k)]);
+                    const float2 val = dft32_inv_r(xk, lt);
 #pragma unroll
                     for (int j = 0; j < NB; ++j) ic[k][j] = cfma(ic[k][j], w[j], val);
                 }
@@ -949,18 +1028,18 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
             lds_barrier();
         }
 #pragma unroll
-        for (int k = 0; k < 2 * NB; ++k)
+        for (int k = 0; k < KR; ++k)
 #pragma unroll
             for (int j = 0; j < NB; ++j)
-                acc[k][j] = cfma_conj(acc[k][j], sp[(yl + 16 * k) * N + b + 32 * j], ic[k][j]);
+                acc[k][j] = cfma_conj(acc[k][j], sp[(yl + 16 * (y0 + k)) * N + b + 32 * j], ic[k][j]);
     }
     float2* op = a.out + ((int64_t)p * a.N0 + z) * plane;
     const float2* sb = a.sub ? a.sub + ((int64_t)p * a.N0 + z) * plane : nullptr;
 #pragma unroll
-    for (int k = 0; k < 2 * NB; ++k)
+    for (int k = 0; k < KR; ++k)
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-            const int64_t idx = (int64_t)(yl + 16 * k) * N + b + 32 * j;
+            const int64_t idx = (int64_t)(yl + 16 * (y0 + k)) * N + b + 32 * j;
             float2 rr = acc[k][j];
             if (a.accumulate) rr = cadd(rr, op[idx]);
             else if (sb) rr = csub(rr, sb[idx]);
@@ -1570,10 +1649,18 @@ int sense_fused_get_zconv_batch() { return g_zconv_batch; }
 void sense_fused_set_zconv_variant(int v) { g_zconv_variant = v; }
 int sense_fused_get_zconv_variant() { return g_zconv_variant; }
 
+// The z-convolution has register / compact-LDS forms for 2 N0 = 64 k (k <= 5); other
+// lengths use z_conv_kernel, which holds two 32-column blocks of the line in LDS.
+static bool zconv_fits(int64_t L0)
+{
+    if (L0 % 64 == 0 && L0 <= 320) return true;
+    return sizeof(float2) * (2 * (size_t)L0 * kW + (size_t)L0) <= 160 * 1024;
+}
+
 bool sense_fused_supported(const int64_t* n3)
 {
-    return n3[1] == n3[2] && n3[1] % 32 == 0 && n3[1] >= 32 && n3[1] <= 128 && fft_size_supported((int)(2 * n3[0])) &&
-           2 * n3[0] <= 512;
+    return n3[1] == n3[2] && n3[1] % 32 == 0 && n3[1] >= 32 && n3[1] <= 160 && fft_size_supported((int)(2 * n3[0])) &&
+           zconv_fits(2 * n3[0]);
 }
 
 bool sense_fused_register_layout(const int64_t* n3) { return use_r16((int)n3[1]); }
@@ -1600,7 +1687,7 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
                        void* ws, size_t ws_bytes, int pchunk, int cchunk, hipStream_t s, int coil_pstride,
                        const int* pcoils_dev)
 {
-    HASTY_REQUIRE(sense_fused_supported(n3), "fused SENSE normal operator needs N1 == N2 in {32, 64, 96, 128}");
+    HASTY_REQUIRE(sense_fused_supported(n3), "fused SENSE normal operator needs N1 == N2 in {32, 64, 96, 128, 160}");
     const int N0 = (int)n3[0], N = (int)n3[1];
     const int NB = N / 32;
     const int L0 = 2 * N0, L = 2 * N;
@@ -1632,16 +1719,28 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
     else if (L0 == 128) zct = z_conv_ct_kernel<2>;
     else if (L0 == 192) zct = z_conv_ct_kernel<3>;
     else if (L0 == 256) zct = z_conv_ct_kernel<4>;
+    else if (L0 == 320) zct = z_conv_ct_kernel<5>;
     if (zct) allow_lds(zct, smem_zct);
     auto fwd = NB == 1 ? fwd_xy_kernel<1> : NB == 2 ? fwd_xy_kernel<2> : NB == 3 ? fwd_xy_kernel<3> : fwd_xy_kernel<4>;
     auto inv = NB == 1 ? inv_xy_kernel<1> : NB == 2 ? inv_xy_kernel<2> : NB == 3 ? inv_xy_kernel<3> : inv_xy_kernel<4>;
     int nxy = kXY;
-    if (g_xy_variant >= 1) {
-        fwd = NB == 1 ? fwd_xy_w_kernel<1> : NB == 2 ? fwd_xy_w_kernel<2> : NB == 3 ? fwd_xy_w_kernel<3> : fwd_xy_w_kernel<4>;
-        inv = NB == 1 ? inv_xy_w_kernel<1> : NB == 2 ? inv_xy_w_kernel<2> : NB == 3 ? inv_xy_w_kernel<3> : inv_xy_w_kernel<4>;
+    const int variant = g_xy_variant.load();
+    // at N = 160 only the coset kernels (one coset per step) fit in LDS
+    if (variant >= 1 || NB == 5) {
+        fwd = NB == 1   ? fwd_xy_w_kernel<1>
+              : NB == 2 ? fwd_xy_w_kernel<2>
+              : NB == 3 ? fwd_xy_w_kernel<3>
+              : NB == 4 ? fwd_xy_w_kernel<4>
+                        : fwd_xy_w_kernel<5>;
+        inv = NB == 1   ? inv_xy_w_kernel<1>
+              : NB == 2 ? inv_xy_w_kernel<2>
+              : NB == 3 ? inv_xy_w_kernel<3>
+              : NB == 4 ? inv_xy_w_kernel<4>
+                        : inv_xy_w_kernel<5>;
         nxy = kXYW;
+        smem_xy = coset_kernel_lds(NB);
     }
-    if (g_xy_variant == 3)
+    if (variant == 3 && NB <= 4)  // stage + transform buffers: the LDS of two coset blocks
         inv = NB == 1 ? inv_xy_d_kernel<1> : NB == 2 ? inv_xy_d_kernel<2> : NB == 3 ? inv_xy_d_kernel<3> : inv_xy_d_kernel<4>;
     HASTY_REQUIRE(!r16 || N == 128, "the register-kernel layout needs N1 = N2 = 128");
     if (r16) {
@@ -1652,7 +1751,9 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
     }
     allow_lds(fwd, smem_xy);
     allow_lds(inv, smem_xy);
-    allow_lds(z_conv_kernel, smem_z);
+    if (!zct && L0 != 256) allow_lds(z_conv_kernel, smem_z);
+    // at N = 160 (always the coset kernels) the inverse runs two workgroups per z-plane
+    const unsigned inv_grid_x = inv_w_grid_x(NB, N0);
     for (int p0 = 0; p0 < P; p0 += pchunk) {
         const int np = std::min(pchunk, P - p0);
         a.p0 = p0;
@@ -1695,7 +1796,7 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
                 check_launch("z_conv_kernel");
             }
             KernelTimer t("sense_inv_xy", s);
-            hipLaunchKernelGGL(inv, dim3(N0, np), dim3(nxy), smem_xy, s, a);
+            hipLaunchKernelGGL(inv, dim3(inv_grid_x, np), dim3(nxy), smem_xy, s, a);
             check_launch("inv_xy_kernel");
         }
     }

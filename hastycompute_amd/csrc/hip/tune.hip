@@ -149,7 +149,10 @@ SenseChoice autotune_sense_normal(const int64_t* n3, int ncoil, int nproblems, b
         best.zconv_variant = sense_fused_get_zconv_variant();
         double tbest = 1e300;
         if (fused) {
-            std::vector<int> variants = n3[1] == 128 ? std::vector<int>{2, 1, 3, 0} : std::vector<int>{1, 3, 0};
+            // at N = 160 every variant runs the coset kernels: one timing is enough
+            std::vector<int> variants = n3[1] == 128   ? std::vector<int>{2, 1, 3, 0}
+                                        : n3[1] == 160 ? std::vector<int>{best.xy_variant}
+                                                       : std::vector<int>{1, 3, 0};
             for (int v : variants) {
                 sense_fused_set_xy_variant(v);
                 layout = sense_kernel_layout(n3, allow_fused);

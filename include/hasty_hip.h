@@ -88,9 +88,11 @@ int hasty_toeplitz_apply(const void* in, void* out, const void* mul, const void*
  * [nk][2N0][2N1][2N2] as written by hasty_sense_normal_prepare with `layout`;
  * kernel_index: HOST int32 [P], kernel of each problem. coil_batch coils share one
  * oversampled-grid workspace; problem_batch problems go through each launch.
- * Layouts 1 and 2 (N1 == N2 in {32, 64, 96, 128}) run the fused three-kernel path
- * (2: the register plane kernels at N1 = N2 = 128, 1: the LDS or coset plane kernels);
- * layout 0 the generic five-pass path. The apply follows the kernels' layout, whatever
+ * Layouts 1 and 2 (N1 == N2 in {32, 64, 96, 128, 160}, 2 N0 2,3,5-smooth with its
+ * z-convolution in LDS: any 2 N0 = 64 k up to 320, others up to 315) run the fused
+ * three-kernel path (2: the register plane kernels at N1 = N2 = 128, 1: the LDS or
+ * coset plane kernels; at 160 always the coset kernels); layout 0 the generic
+ * five-pass path. The apply follows the kernels' layout, whatever
  * the sense_xy_variant option is at that time.
  * ------------------------------------------------------------------------------- */
 size_t hasty_sense_normal_workspace_size(const int64_t* nmodes3, int coil_batch, int problem_batch);

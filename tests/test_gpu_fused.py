@@ -37,7 +37,12 @@ def maps(rng, C, shape):
 
 
 @pytest.mark.parametrize("N0,Nn,C,cb,P", [(32, 32, 3, 2, 2), (40, 64, 4, 4, 3), (24, 96, 2, 1, 1), (20, 32, 5, 3, 4),
-                                         (8, 128, 3, 2, 3)])
+                                         (8, 128, 3, 2, 3),
+                                         # N = 160: coset kernels one coset per step, the
+                                         # inverse in two row halves (N0 = 20: a short last
+                                         # group of 8 planes); 2 N0 = 320 and 300 z lines
+                                         (8, 160, 3, 2, 3), (20, 160, 2, 1, 2), (160, 32, 2, 2, 2),
+                                         (150, 32, 2, 1, 1)])
 def test_fused_vs_oracle(N, N0, Nn, C, cb, P):
     rng = np.random.default_rng(N0 + Nn + C)
     shape = (N0, Nn, Nn)
@@ -100,6 +105,36 @@ def test_fused_repeatable(N):
     a = op.apply(x).clone()
     b = op.apply(x).clone()
     assert torch.equal(a, b)
+
+
+def test_fused_160_every_variant_runs_the_coset_kernels(N):
+    # at N1 = N2 = 160 only the coset kernels fit in LDS: every sense_xy_variant value
+    # prepares layout 1 and gives the same bits, which match the oracle
+    from hastycompute_amd import _capi as C
+
+    rng = np.random.default_rng(16)
+    shape = (12, 160, 160)
+    Sn = maps(rng, 3, shape)
+    co = rng.uniform(-np.pi, np.pi, (3, 6000))
+    _, k = N.toeplitz_build(g(co, torch.float32), list(shape), diag=False)
+    x = rnd(rng, (2,) + shape)
+    prev = C.get_option("sense_xy_variant")
+    outs = []
+    try:
+        for v in (0, 1, 2, 3):
+            C.set_option("sense_xy_variant", v)
+            op = N.SenseNormalToeplitz(g(Sn), k, coil_batch=2, problem_batch=2)
+            assert op.layout == 1
+            outs.append(op.apply(g(x)).clone())
+    finally:
+        C.set_option("sense_xy_variant", prev)
+    assert all(torch.equal(outs[0], o) for o in outs[1:])
+    diag = O.toeplitz_diagonal(co.astype(np.float32).astype(np.float64), [1, *shape])
+    got = h(outs[0])
+    for p in range(2):
+        e = O.rel_l2(got[p:p + 1], O.sense_normal_toeplitz(diag, x[p:p + 1], Sn, [0, 1, 2]))
+        print(f"160: problem {p}: {e:.2e}", flush=True)
+        assert e < 1e-5
 
 
 def test_prepared_layout_survives_option_change(N):
