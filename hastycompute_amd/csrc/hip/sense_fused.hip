@@ -1589,6 +1589,120 @@ template <int COLS, bool DB> void launch_zconv(const FusedArgs& a, int LL, int q
                        dim3(ZConvShape<COLS>::T), smem, s, a, LL, q0, zk);
 }
 
+// z-axis pass for L0 = 320 (N0 = 160) as a 20 x 16 four-step FFT in registers:
+// z = 20 n1 + n2 (n1 < 8 nonzero), k = k1 + 16 k2. 16 columns per workgroup and 20 threads
+// per column (320 threads): in the 16-point steps thread g of a column owns n2 = g, in
+// the 20-point steps (over n2 -> k2 and back) the threads g < 16 own k1 = g and the
+// other four wait at the barrier. One LDS transpose per direction; DB: separate forward
+// and inverse buffers (two barriers per coil instead of four).
+struct ZConv20 {
+    static constexpr int COLS = 16, T = 20 * COLS;
+    static constexpr int S1 = COLS + 1;  // stride of the inner index (padded)
+    static constexpr int SF = 348;       // forward buffer [k1 < 16][n2 < 20 (S1)][c]
+    static constexpr int SI = 280;       // inverse buffer [n2 < 20][k1 < 16 (S1)][c]
+    // transpose buffer(s), then the w320 table
+    static constexpr size_t lds_bytes(bool db)
+    {
+        return sizeof(float2) * ((db ? 16 * SF + 20 * SI : std::max(16 * SF, 20 * SI)) + 320);
+    }
+};
+
+template <bool DB>
+__global__ void __launch_bounds__(ZConv20::T) z_conv_r20_kernel(FusedArgs a, int LL, int pl0, ZKernels zk)
+{
+    constexpr int S1 = ZConv20::S1, SF = ZConv20::SF, SI = ZConv20::SI, COLS = ZConv20::COLS;
+    const int pl = pl0 + (int)blockIdx.y;
+    const float* __restrict__ K = zk.k[blockIdx.y];
+    extern __shared__ float2 lds_all[];  // ZConv20::lds_bytes(DB)
+    float2* ldsf = lds_all;
+    float2* ldsi = lds_all + (DB ? 16 * SF : 0);
+    float2* twz = lds_all + (DB ? 16 * SF + 20 * SI : std::max(16 * SF, 20 * SI));  // w320^k, forward sign
+    const int t = threadIdx.x;
+    const int c = t % COLS;
+    const int g = t / COLS;  // 0..19
+    const int pos0 = blockIdx.x * COLS;
+    for (int i = t; i < 320; i += ZConv20::T) twz[i] = a.twL0[i];
+    // the 20-point steps: kernel at k = g + 16 k2 of this column
+    float kv[20];
+#pragma unroll
+    for (int k2 = 0; k2 < 20; ++k2) kv[k2] = g < 16 ? K[(int64_t)(g + 16 * k2) * LL + pos0 + c] : 0.f;
+    // one coil's half grid [160 z][LL]: element (z, pos0 + c) of z = 20 n1 + g at byte
+    // voff + n1 * 20 * LL * 8
+    auto grid_rsrc = [&](int ci) {
+        return plane_rsrc(a.T + ((int64_t)pl * a.ccount + ci) * 160 * (int64_t)LL, (uint32_t)(160 * 8 * (int64_t)LL));
+    };
+    const uint32_t voff = (uint32_t)(((int64_t)g * LL + pos0 + c) * 8);
+    const uint32_t zstep = (uint32_t)(20 * 8 * (int64_t)LL);
+    float2 pf[8];
+    {
+        const auto r0 = grid_rsrc(0);
+#pragma unroll
+        for (int n1 = 0; n1 < 8; ++n1) pf[n1] = bload<true>(r0, voff, n1 * zstep);
+    }
+
+    const int ncl = coil_count(a, a.p0 + pl);
+    for (int ci = 0; ci < ncl; ++ci) {
+        const auto rc = grid_rsrc(ci);
+        float2 v[20];
+#pragma unroll
+        for (int n1 = 0; n1 < 8; ++n1) v[n1] = pf[n1];
+#pragma unroll
+        for (int n1 = 8; n1 < 16; ++n1) v[n1] = float2{0.f, 0.f};
+        if (ci + 1 < ncl) {
+            const auto rn = grid_rsrc(ci + 1);
+#pragma unroll
+            for (int n1 = 0; n1 < 8; ++n1) pf[n1] = bload<true>(rn, voff, n1 * zstep);
+        }
+        // forward, step A (n2 = g): DFT over n1 (n1 < 8 nonzero), twiddle w320^{g k1}, transpose
+        dft16<-1, true>(v);
+        if (ci == 0) lds_barrier();  // the twiddle table is written
+#pragma unroll
+        for (int k1 = 1; k1 < 16; ++k1) v[k1] = cmul(v[k1], twz[(g * k1) % 320]);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) ldsf[k1 * SF + g * S1 + c] = v[k1];
+        lds_barrier();
+        // forward, step B (k1 = g < 16): DFT over n2 -> X[g + 16 k2] (slot order), times
+        // the kernel; inverse, step B': IDFT over k2 -> n2 (slot order). Waves 0-3 hold
+        // g < 16 and wave 4 g >= 16, so the branches are uniform per wave; the barriers
+        // stay outside them.
+        float2 w[20];  // natural order k2 for the inverse
+        if (g < 16) {
+#pragma unroll
+            for (int n2 = 0; n2 < 20; ++n2) v[n2] = ldsf[g * SF + n2 * S1 + c];
+            dft20_dr<-1>(v);
+#pragma unroll
+            for (int k2 = 0; k2 < 20; ++k2) w[k2] = cscale(v[dft_slot<20>(k2)], kv[k2]);
+            dft20_dr<1>(w);
+        }
+        // one buffer: all step-B reads are done before the transpose is overwritten
+        if (!DB) lds_barrier();
+        if (g < 16) {
+            // twiddle w320^{-g n2}, transpose
+#pragma unroll
+            for (int n2 = 0; n2 < 20; ++n2) {
+                const float2 f = w[dft_slot<20>(n2)];
+                ldsi[n2 * SI + g * S1 + c] = n2 == 0 ? f : cmulc(twz[(g * n2) % 320], f);
+            }
+        }
+        lds_barrier();
+        // inverse, step A' (n2 = g): IDFT over k1 -> out[20 n1 + g], keep n1 < 8
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) v[k1] = ldsi[g * SI + k1 * S1 + c];
+        dft16<1>(v);
+#pragma unroll
+        for (int n1 = 0; n1 < 8; ++n1) bstore<true>(rc, v[n1], voff, n1 * zstep);
+        if (!DB) lds_barrier();
+    }
+}
+
+template <bool DB> void launch_zconv20(const FusedArgs& a, int LL, int q0, int nq, const ZKernels& zk, hipStream_t s)
+{
+    const size_t smem = ZConv20::lds_bytes(DB);
+    allow_lds(z_conv_r20_kernel<DB>, smem);
+    hipLaunchKernelGGL((z_conv_r20_kernel<DB>), dim3((unsigned)(LL / ZConv20::COLS), (unsigned)nq),
+                       dim3(ZConv20::T), smem, s, a, LL, q0, zk);
+}
+
 uint32_t radix_code(int L, int* npass)
 {
     uint32_t code = 0;
@@ -1766,7 +1880,9 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
                 hipLaunchKernelGGL(fwd, dim3(N0, np), dim3(nxy), smem_xy, s, a);
                 check_launch("fwd_xy_kernel");
             }
-            if (L0 == 256) {
+            const int zvar = g_zconv_variant.load();
+            const bool z20 = L0 == 320 && (zvar == 0 || zvar == 2);  // else the LDS Stockham form
+            if (L0 == 256 || z20) {
                 // one launch per up to g_zconv_batch problems of the chunk
                 const int zbatch = g_zconv_batch.load();
                 for (int q0 = 0; q0 < np; q0 += zbatch) {
@@ -1775,16 +1891,21 @@ void sense_fused_apply(const float2* x, float2* out, const float2* smaps, const 
                     for (int j = 0; j < nq; ++j)
                         zk.k[j] = kernels + (int64_t)kidx[p0 + q0 + j] * L0 * (int64_t)L * L;
                     KernelTimer t("sense_z_conv", s);
-                    switch (g_zconv_variant) {
-                    case 1: launch_zconv<8, false>(a, L * L, q0, nq, zk, s); break;
-                    case 2: launch_zconv<16, true>(a, L * L, q0, nq, zk, s); break;
-                    case 3: launch_zconv<8, true>(a, L * L, q0, nq, zk, s); break;
-                    default: launch_zconv<16, false>(a, L * L, q0, nq, zk, s); break;
+                    if (z20) {
+                        if (zvar == 2) launch_zconv20<true>(a, L * L, q0, nq, zk, s);
+                        else launch_zconv20<false>(a, L * L, q0, nq, zk, s);
+                    } else {
+                        switch (zvar) {
+                        case 1: launch_zconv<8, false>(a, L * L, q0, nq, zk, s); break;
+                        case 2: launch_zconv<16, true>(a, L * L, q0, nq, zk, s); break;
+                        case 3: launch_zconv<8, true>(a, L * L, q0, nq, zk, s); break;
+                        default: launch_zconv<16, false>(a, L * L, q0, nq, zk, s); break;
+                        }
                     }
                     check_launch("z_conv_kernel");
                 }
             }
-            for (int pl = 0; pl < np && L0 != 256; ++pl) {
+            for (int pl = 0; pl < np && L0 != 256 && !z20; ++pl) {
                 const float* K = kernels + (int64_t)kidx[p0 + pl] * L0 * (int64_t)L * L;
                 KernelTimer t("sense_z_conv", s);
                 if (zct) {

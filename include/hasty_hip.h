@@ -133,7 +133,8 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
  *     plans.
  *   "sense_zconv_variant" 0..3 (default 2): z-convolution kernel at N0 = 128 (0: 16
  *     columns per workgroup, one LDS transpose buffer; 1: 8 columns; 2: 16 columns, two
- *     buffers; 3: 8 columns, two buffers).
+ *     buffers; 3: 8 columns, two buffers) and at N0 = 160 (0: 20 x 16 register FFT, one
+ *     buffer; 2: the same with two buffers; 1, 3: LDS Stockham passes).
  *   "llr_max_sweeps" 1..15 (default 15): cap on the Jacobi sweeps of the LLR eigensolver.
  *   "nufft_spread_variant" 0..3 (default 3): 3-D type-1 spreader; 0 one lane per
  *     point with LDS atomics, 1 window rows with one wave per tile, 2 window rows with four

@@ -165,14 +165,17 @@ def test_prepared_layout_survives_option_change(N):
     assert O.rel_l2(a, b) < 1e-5 and O.rel_l2(c, b) < 1e-5
 
 
+@pytest.mark.parametrize("N0", [128, 160])
 @pytest.mark.parametrize("variant", [0, 1, 2, 3])
-def test_zconv_variants_vs_oracle(N, variant):
+def test_zconv_variants_vs_oracle(N, variant, N0):
     # every z-convolution kernel at N0 = 128 (2N0 = 256): 16 or 8 columns per workgroup,
-    # one or two LDS transpose buffers; three problems with their own kernels
+    # one or two LDS transpose buffers; at N0 = 160 (2N0 = 320) the 20 x 16 register form
+    # with one (0) or two (2) buffers and the LDS Stockham form (1, 3); three problems with
+    # their own kernels
     from hastycompute_amd import _capi as C
 
     rng = np.random.default_rng(40 + variant)
-    shape = (128, 32, 32)
+    shape = (N0, 32, 32)
     S = maps(rng, 3, shape)
     cos = [rng.uniform(-np.pi, np.pi, (3, 3000)) for _ in range(2)]
     kern, diags = [], []
