@@ -1003,7 +1003,11 @@ at::Tensor BatchedConjugateGradient::run(at::Tensor& x, int iter, double tol)
     at::Tensor r = at::empty_like(_b), p = at::empty_like(_b), Ap = at::empty_like(_b);
     const size_t ws = hasty_cg_workspace_size(P, n);
     at::Tensor w = at::empty({(int64_t)ws}, _b.options().dtype(at::kByte));
-    _A(x, Ap);
+    // A 0 is exactly 0: a zero start gives r = b to the bit without an apply
+    if (at::count_nonzero(at::view_as_real(x)).item<int64_t>() == 0)
+        Ap.zero_();
+    else
+        _A(x, Ap);
     check(hasty_cg_init(P, n, _b.data_ptr(), Ap.data_ptr(), r.data_ptr(), p.data_ptr(), state.data_ptr<double>(),
                         w.data_ptr(), detail::stream()));
     for (int i = 0; i < iter; ++i) {

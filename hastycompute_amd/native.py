@@ -297,7 +297,8 @@ def cg_solve(apply_A, b: torch.Tensor, x0: torch.Tensor, iters: int, tol: float 
     preconditioned P v (the _P branch). With graph, the first iteration runs eagerly and
     the later ones replay a HIP graph of one iteration (apply_A + the CG step), so
     apply_A must not allocate. l2 > 0 solves with A + l2 I (l2_reg_linop,
-    python/python/hastypy/base/opalg.py:626-627) without a pass for l2 p. Returns
+    python/python/hastypy/base/opalg.py:626-627) without a pass for l2 p. A start x0 of
+    all zeros skips the first apply (A 0 is exactly 0, so r = b to the bit). Returns
     (x, state)."""
     _require(l2 >= 0.0, "l2 must be >= 0")
     _require(not (l2 and apply_P is not None), "l2 covers the unpreconditioned loop")
@@ -310,15 +311,23 @@ def cg_solve(apply_A, b: torch.Tensor, x0: torch.Tensor, iters: int, tol: float 
     cg = DeviceCG(P, n, b.device)
     if iters < 1:
         return x, cg.state
+    zero_start = int(torch.count_nonzero(torch.view_as_real(x0) if x0.is_complex() else x0)) == 0
+
+    def residual_start():  # Ap = A x0 (+ l2 x0) for r = b - A x0
+        if zero_start:
+            Ap.zero_()
+            return
+        apply_A(x, Ap)
+        if l2:
+            Ap.add_(x, alpha=l2)
+
     if graph:
         _require(apply_P is None, "graph replay covers the unpreconditioned loop")
         main = torch.cuda.current_stream()
         side = torch.cuda.Stream()
         side.wait_stream(main)
         with torch.cuda.stream(side):
-            apply_A(x, Ap)
-            if l2:
-                Ap.add_(x, alpha=l2)
+            residual_start()
             cg.init(b, Ap, r, p)
             apply_A(p, Ap)
             cg.iterate(x, r, p, Ap, tol, l2)
@@ -336,15 +345,13 @@ def cg_solve(apply_A, b: torch.Tensor, x0: torch.Tensor, iters: int, tol: float 
             side.synchronize()
             g.close()
     elif apply_P is None:
-        apply_A(x, Ap)
-        if l2:
-            Ap.add_(x, alpha=l2)
+        residual_start()
         cg.init(b, Ap, r, p)
         for _ in range(iters):
             apply_A(p, Ap)
             cg.iterate(x, r, p, Ap, tol, l2)
     else:
-        apply_A(x, Ap)
+        residual_start()
         cg.init(b, Ap, r, p)
         z = torch.empty_like(b)
         apply_P(r, z)
