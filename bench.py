@@ -91,6 +91,8 @@ def parse():
     ap.add_argument("--llr-fraction", type=float, default=0.25,
                     help="serve mode: share of multi-frame requests with the LLR prox")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--set-option", action="append", default=[], metavar="NAME=VALUE",
+                    help="library option (hasty_set_option) for the whole run; repeatable")
     return ap.parse_args()
 
 
@@ -762,6 +764,11 @@ def main():
     from hastycompute_amd import native
 
     torch.cuda.set_device(0)
+    if args.set_option:
+        from hastycompute_amd import _capi
+        for item in args.set_option:
+            name, _, value = item.partition("=")
+            _capi.set_option(name, int(value))
     if args.copy_calibration:
         copy_calibration(torch)
         return

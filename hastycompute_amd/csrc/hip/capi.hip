@@ -382,6 +382,14 @@ int hasty_set_option(const char* name, int64_t value)
         nufft_set_locality((int)value);
         return 0;
     }
+    if (name && std::strcmp(name, "nufft_spread_tile") == 0 && (value == 0 || (value >= 12 && value <= 28))) {
+        nufft_set_tile((int)value);
+        return 0;
+    }
+    if (name && std::strcmp(name, "nufft_spread_chunk") == 0 && (value == 0 || (value >= 32 && value <= 8192))) {
+        nufft_set_chunk((int)value);
+        return 0;
+    }
     set_last_error(std::string("unknown option ") + (name ? name : "(null)"));
     return 1;
 }
@@ -398,6 +406,8 @@ int hasty_get_option(const char* name, int64_t* value)
     else if (name && std::strcmp(name, "sense_zconv_batch") == 0) *value = sense_fused_get_zconv_batch();
     else if (name && std::strcmp(name, "llr_max_sweeps") == 0) *value = llr_get_max_sweeps();
     else if (name && std::strcmp(name, "sense_zconv_variant") == 0) *value = sense_fused_get_zconv_variant();
+    else if (name && std::strcmp(name, "nufft_spread_tile") == 0) *value = nufft_get_tile();
+    else if (name && std::strcmp(name, "nufft_spread_chunk") == 0) *value = nufft_get_chunk();
     else if (name && std::strcmp(name, "nufft_locality") == 0) *value = nufft_get_locality();
     else {
         set_last_error(std::string("unknown option ") + (name ? name : "(null)"));

@@ -30,8 +30,12 @@ namespace {
 
 // points per spreader workgroup: complex64 chunks stay small so that a crowded tile
 // (the k-space centre of a radial trajectory) spreads over many CUs; complex128 tiles
-// (20^3 cells, one workgroup per CU) are costlier to clear and flush, so they take more
-inline int chunk_cap(bool fp64) { return fp64 ? 1024 : 256; }
+// (one workgroup per CU) are costlier to clear and flush, so they take more. On a 128^3
+// kooshball (ns = 7, M = 147 200) complex64 spreading took 0.446 ms with 16^3 tiles and
+// 256 points, 0.428 ms with 20^3 tiles and 128 points, 0.683 ms with 24^3 and 256.
+inline int chunk_cap(bool fp64) { return fp64 ? 1024 : 128; }
+std::atomic<int> g_tile{0};   // 3-D tile edge with halo of new plans (0: by precision)
+std::atomic<int> g_chunk{0};  // points per subproblem of new plans (0: chunk_cap)
 std::atomic<int> g_spread_variant{3};  // 3-D spreader, values as nufft_set_spread_variant
 // point ordering of new plans: bit 0 super-tile bin order, bit 1 XCD-aware block order;
 // 4 (default): 3 for type-1 plans (spreading), 0 for type-2 plans (interpolation)
@@ -718,6 +722,10 @@ void nufft_set_spread_variant(int v) { g_spread_variant = v; }
 int nufft_get_spread_variant() { return g_spread_variant; }
 void nufft_set_locality(int v) { g_locality = v; }
 int nufft_get_locality() { return g_locality; }
+void nufft_set_tile(int v) { g_tile = v; }
+int nufft_get_tile() { return g_tile; }
+void nufft_set_chunk(int v) { g_chunk = v; }
+int nufft_get_chunk() { return g_chunk; }
 
 void es_params(double tol, int* ns, double* beta)
 {
@@ -765,10 +773,12 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int iflag, i
             p->nf[d] = next235even(std::max<int64_t>(2 * p->n[d], 2 * p->ns));
             HASTY_REQUIRE(fft_size_supported((int)p->nf[d]), "fine grid too large");
         }
-        // Tile sizes: tile + halo fits the LDS budget (32 KiB for complex64, 128 KiB for complex128 in 3-D).
+        // Tile sizes in 3-D: 20^3 cells with the halo (64 KiB of LDS for complex64, 128 KiB
+        // for complex128); the nufft_spread_tile option overrides the edge.
         if (dim == 3) {
-            const double cells = p->fp64 ? 8000.0 : 4096.0;
+            const double cells = 8000.0;
             int t = (int)std::floor(std::cbrt(cells) + 1e-9);
+            if (g_tile.load() > 0) t = g_tile.load();
             int b = t - p->ns - 1;
             b = std::max(2, b);
             for (int d = 0; d < 3; ++d) p->bs[d] = b;
@@ -793,6 +803,7 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int iflag, i
             else
                 twiddles<float>((int)p->nf[d]);
         }
+        p->chunk = g_chunk.load() > 0 ? g_chunk.load() : chunk_cap(p->fp64 != 0);
         const int loc = g_locality.load();
         p->locality = loc <= 3 ? loc : (type == 1 ? 3 : 0);
         const int64_t nb = bin_range(p->nbins, (p->locality & 1) ? kSuper : 1);
@@ -845,7 +856,7 @@ template <typename R> void setpts_typed(NufftPlan* p, hipStream_t s)
     // `rank` (free again after the scatter) to avoid another buffer.
     int* nsub = p->rank;
     hipLaunchKernelGGL(subcount_kernel, dim3((unsigned)ceil_div(nb, kThreads)), dim3(kThreads), 0, s,
-                       (const int*)p->counts, nsub, nb, chunk_cap(p->fp64 != 0));
+                       (const int*)p->counts, nsub, nb, p->chunk);
     check_launch("subcount_kernel");
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, (const int*)nsub, p->nsubbin, nb);
     check_launch("scan_kernel");
@@ -865,7 +876,7 @@ template <typename R> void setpts_typed(NufftPlan* p, hipStream_t s)
     if (total > 0) {
         hipLaunchKernelGGL(subfill_kernel, dim3((unsigned)ceil_div(nb, kThreads)), dim3(kThreads), 0, s,
                            (const int*)p->counts, (const int*)p->nsubbin, p->subbin, p->suboff, nb,
-                           chunk_cap(p->fp64 != 0));
+                           p->chunk);
         check_launch("subfill_kernel");
     }
 }
@@ -889,7 +900,7 @@ template <typename R> SpreadArgs<R> spread_args(const NufftPlan* p)
     int T[3];
     for (int d = 0; d < 3; ++d) T[d] = (d >= 3 - p->dim) ? p->bs[d] + p->ns + 1 : 1;
     tile_pitches(T, p->ns, (int)sizeof(cplx<R>), pad_tile(p), a.rp, a.pp);
-    a.maxsub = chunk_cap(p->fp64 != 0);
+    a.maxsub = p->chunk;
     a.nsub = p->nsub;
     a.xcd = (p->locality & 2) ? 1 : 0;
     return a;

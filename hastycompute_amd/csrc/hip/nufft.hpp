@@ -40,6 +40,7 @@ struct NufftPlan {
     int device = 0;
     int spread_variant = -1;  // 3-D spreader of this plan (-1: the library option)
     int locality = 3;         // bit 0 super-tile bin order, bit 1 XCD-aware blocks (fixed at makeplan)
+    int chunk = 256;          // points per spreader subproblem at most (fixed at makeplan)
 };
 
 // nmodes in FINUFFT order (fastest-varying first), as the reference passes
@@ -61,6 +62,12 @@ int nufft_get_spread_variant();
 // type-1 plans, 0 for type-2 plans (measured best for spreading and for interpolation).
 void nufft_set_locality(int v);
 int nufft_get_locality();
+// 3-D tile edge including the halo (0: 20 cells) and points per spreader subproblem
+// (0: 128 for complex64, 1024 for complex128) of new plans
+void nufft_set_tile(int v);
+int nufft_get_tile();
+void nufft_set_chunk(int v);
+int nufft_get_chunk();
 
 // Time the 3-D type-1 spreaders on the plan's points (c -> f, f overwritten) and keep
 // the fastest for this plan; returns it and its execute time in *ms. Other plans: no-op

@@ -136,6 +136,10 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
  *     buffers; 3: 8 columns, two buffers) and at N0 = 160 (0: 20 x 16 register FFT, one
  *     buffer; 2: the same with two buffers; 1, 3: LDS Stockham passes).
  *   "llr_max_sweeps" 1..15 (default 15): cap on the Jacobi sweeps of the LLR eigensolver.
+ *   "nufft_spread_tile" 0 or 12..28 (default 0 = 20): 3-D spreading tile edge in fine-grid
+ *     cells, halo included, of new plans (the tile must fit in LDS: 20 for complex128).
+ *   "nufft_spread_chunk" 0 or 32..8192 (default 0 = 128 for complex64, 1024 for
+ *     complex128): points per spreading workgroup of new plans.
  *   "nufft_spread_variant" 0..3 (default 3): 3-D type-1 spreader; 0 one lane per
  *     point with LDS atomics, 1 window rows with one wave per tile, 2 window rows with four
  *     waves and LDS atomics, 3 four waves clearing and flushing the tile while one wave
