@@ -25,6 +25,7 @@ import sys
 KERNEL_NAMES = {"fwd_xy_kernel": "sense_fwd_xy", "inv_xy_kernel": "sense_inv_xy", "fwd_xy_w_kernel": "sense_fwd_xy",
                 "inv_xy_w_kernel": "sense_inv_xy", "inv_xy_d_kernel": "sense_inv_xy", "fwd_xy_r16_kernel": "sense_fwd_xy",
                 "inv_xy_r16_kernel": "sense_inv_xy", "z_conv_r16_kernel": "sense_z_conv",
+                "z_conv_r20_kernel": "sense_z_conv",
                 "z_conv_ct_kernel": "sense_z_conv", "z_conv_kernel": "sense_z_conv", "spread_kernel": "nufft_spread",
                 "spread_rows_kernel": "nufft_spread", "zero_kernel": "fill_zero",
                 "interp_kernel": "nufft_interp", "axis_fft_kernel": "fft_axis", "gram_kernel": "llr_gram",
@@ -57,7 +58,8 @@ def counter_per_kernel(path: str):
 KERNEL_SOURCES = {"fwd_xy_r16_kernel": "sense_fused.hip", "inv_xy_r16_kernel": "sense_fused.hip",
                   "z_conv_r16_kernel": "sense_fused.hip", "fwd_xy_w_kernel": "sense_fused.hip",
                   "inv_xy_w_kernel": "sense_fused.hip", "fwd_xy_kernel": "sense_fused.hip",
-                  "inv_xy_kernel": "sense_fused.hip", "spread_kernel": "nufft.hip", "interp_kernel": "nufft.hip",
+                  "inv_xy_kernel": "sense_fused.hip", "z_conv_r20_kernel": "sense_fused.hip",
+                  "spread_kernel": "nufft.hip", "interp_kernel": "nufft.hip",
                   "spread_rows_kernel": "nufft.hip", "zero_kernel": "nufft.hip",
                   "axis_fft_kernel": "fft.hip", "gram_kernel": "llr.hip", "eig_w_kernel": "llr.hip",
                   "apply_kernel": "llr.hip"}
