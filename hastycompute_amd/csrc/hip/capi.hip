@@ -631,6 +631,38 @@ int hasty_llr_blocks(void* data, const int64_t* dims5, const int64_t* block_shap
     });
 }
 
+int hasty_llr_grid_corners(const int64_t* dims5, const int64_t* block_strides, const int64_t* block_shape,
+                           const int32_t* shift3, int32_t* corners_out, int64_t capacity, int64_t* count)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(dims5 && block_strides && block_shape && shift3 && count, "null argument");
+        for (int d = 0; d < 3; ++d) HASTY_REQUIRE(block_strides[d] >= 1, "block strides must be >= 1");
+        check_dims5(dims5, block_shape);
+        const std::vector<int32_t> b = grid_blocks(dims5, block_strides, block_shape, shift3[0], shift3[1], shift3[2]);
+        *count = (int64_t)b.size() / 3;
+        HASTY_REQUIRE(!corners_out || capacity >= *count, "corner buffer too small");
+        if (corners_out) std::memcpy(corners_out, b.data(), sizeof(int32_t) * b.size());
+    });
+}
+
+int hasty_llr_blocks_dev(void* data, const int64_t* dims5, const int64_t* block_shape, const int32_t* corners_dev,
+                         int64_t nblk, double thresh, int soft, void* workspace, size_t workspace_bytes, void* stream)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(data && dims5 && block_shape && workspace, "null argument");
+        HASTY_REQUIRE(nblk >= 0 && (nblk == 0 || corners_dev), "null corners");
+        HASTY_REQUIRE(thresh >= 0.0, "threshold must be >= 0");
+        check_dims5(dims5, block_shape);
+        if (nblk == 0) return;
+        const LlrGeom g = image_geom(dims5, block_shape);
+        const int nsplit = choose_nsplit(nblk, g.E * g.bx * g.by * g.bz, g.F);
+        HASTY_REQUIRE(workspace_bytes >= llr_workspace_bytes(nblk, g.F, nsplit), "LLR workspace too small");
+        const int rank = soft ? 0 : (int)(thresh + 0.5);
+        llr_blocks((float2*)data, g, reinterpret_cast<const int3*>(corners_dev), nblk, (float)thresh, soft, rank,
+                   workspace, workspace_bytes, nsplit, nullptr, as_stream(stream));
+    });
+}
+
 size_t hasty_svt_workspace_size(int64_t R, int64_t F) { return llr_ws_bound(1, F, R); }
 
 size_t hasty_llr_blocks_workspace_size(const int64_t* dims5, const int64_t* block_shape, int64_t nblk)

@@ -107,6 +107,13 @@ template <int NT, int WAVE, int NTW> struct GramTiles<NT, WAVE, NTW, NTW> {
     __device__ __forceinline__ static void mfma(const float2 (&)[NT][4], floatx4*, floatx4*, floatx4*) {}
 };
 
+// A block lies inside the image: corners handed over on the device are not seen by the
+// host, so the kernels skip a block that does not (its Gram is zero, its rows untouched).
+__device__ __forceinline__ bool block_inside(const LlrGeom& g, int3 c)
+{
+    return c.x >= 0 && c.y >= 0 && c.z >= 0 && c.x + g.bx <= g.X && c.y + g.by <= g.Y && c.z + g.bz <= g.Z;
+}
+
 template <int NT, int WAVE>
 __device__ __forceinline__ void gram_wave(const float2* __restrict__ data, const LlrGeom& g, int3 c, int64_t r_begin,
                                           int64_t r_end, float2* __restrict__ G)
@@ -211,7 +218,8 @@ __global__ void __launch_bounds__(kThreads) gram_kernel(const float2* __restrict
     const int3 c = corners[blk];
     const int64_t R = g.E * g.bx * g.by * g.bz;
     const int64_t r_begin = split * rows_per_split;
-    const int64_t r_end = min(R, r_begin + rows_per_split);
+    // a block outside the image gets an empty row range: its partial Gram is zero
+    const int64_t r_end = block_inside(g, c) ? min(R, r_begin + rows_per_split) : r_begin;
     float2* G = gpart + ((int64_t)blk * nsplit + split) * Fp * Fp;
     switch (threadIdx.x >> 6) {
     case 0: gram_wave<NT, 0>(data, g, c, r_begin, r_end, G); break;
@@ -649,7 +657,7 @@ __global__ void __launch_bounds__(kApplyThreads) apply_kernel(float2* __restrict
     const int64_t R = g.E * g.bx * g.by * g.bz;
     const int64_t r_begin = (int64_t)blockIdx.y * rows_per_wg;
     const int64_t r_end = min(R, r_begin + rows_per_wg);
-    if (r_begin >= r_end) return;
+    if (r_begin >= r_end || !block_inside(g, c)) return;  // the whole workgroup
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int n = lane & 15;   // slab row of this lane

@@ -477,6 +477,41 @@ def llr_blocks(x: torch.Tensor, block_shape, corners, thresh: float, soft: bool)
     return x
 
 
+def llr_grid_corners(dims, block_strides, block_shape, shift) -> list:
+    """Corners [[x, y, z], ...] of the blocks one pass of llr_normal_blocks visits for one
+    shift (the reference's strided grid and skip rule, svt.cpp:378-404); host only."""
+    count = ctypes.c_int64(0)
+    args = (C.i64arr(list(dims)), C.i64arr(list(block_strides)), C.i64arr(list(block_shape)),
+            C.i32arr([int(v) for v in shift]))
+    C.call("hasty_llr_grid_corners", *args, None, 0, ctypes.byref(count))
+    out = C.i32arr([0] * (3 * max(1, count.value)))
+    C.call("hasty_llr_grid_corners", *args, out, count.value, ctypes.byref(count))
+    return [[out[3 * b + d] for d in range(3)] for b in range(count.value)]
+
+
+def llr_blocks_workspace(x: torch.Tensor, block_shape, nblk: int) -> torch.Tensor:
+    """Workspace for llr_blocks_dev on x with up to nblk blocks."""
+    n = C.lib().hasty_llr_blocks_workspace_size(C.i64arr(list(x.shape)), C.i64arr(list(block_shape)), nblk)
+    return torch.empty(n, dtype=torch.uint8, device=x.device)
+
+
+def llr_blocks_dev(x: torch.Tensor, block_shape, corners: torch.Tensor, thresh: float, soft: bool,
+                   workspace: torch.Tensor) -> torch.Tensor:
+    """SVT of the disjoint blocks whose corners (int32 [nblk, 3]) are already on the GPU,
+    in place on x [F, E, X, Y, Z]: no host copy and no synchronisation, so the call can
+    be recorded in a Graph. Blocks outside the image are skipped."""
+    _cuda(x, "input")
+    _cuda(corners, "corners")
+    _require(x.dtype == torch.complex64 and x.dim() == 5 and x.is_contiguous(),
+             "input must be contiguous complex64 [F, E, X, Y, Z]")
+    _require(corners.dtype == torch.int32 and corners.dim() == 2 and corners.shape[1] == 3 and corners.is_contiguous(),
+             "corners must be contiguous int32 [nblk, 3]")
+    C.call("hasty_llr_blocks_dev", x.data_ptr(), C.i64arr(list(x.shape)), C.i64arr(list(block_shape)),
+           corners.data_ptr(), corners.shape[0], float(thresh), int(bool(soft)), workspace.data_ptr(),
+           workspace.numel(), C.stream_ptr())
+    return x
+
+
 def svt_matrix(A: torch.Tensor, thresh: float, soft: bool) -> torch.Tensor:
     """svt_soft / svt_hard on a contiguous [R, F] complex64 GPU matrix, in place."""
     _cuda(A, "A")

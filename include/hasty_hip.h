@@ -318,6 +318,19 @@ int hasty_llr_blocks(void* data, const int64_t* dims5, const int64_t* block_shap
                      int64_t nblk, double thresh, int soft, void* workspace, size_t workspace_bytes, void* stream);
 size_t hasty_llr_blocks_workspace_size(const int64_t* dims5, const int64_t* block_shape, int64_t nblk);
 
+/* The blocks one pass of hasty_llr_normal_blocks visits for one shift (shift3: int32
+ * [3]): *count blocks, their corners written to corners_out (HOST int32 [capacity][3])
+ * when it is not NULL; no GPU work. With hasty_llr_blocks_dev it splits the prox into a
+ * host step done once (the grid of each shift of a reconstruction) and launches that a
+ * HIP graph can capture. */
+int hasty_llr_grid_corners(const int64_t* dims5, const int64_t* block_strides, const int64_t* block_shape,
+                           const int32_t* shift3, int32_t* corners_out, int64_t capacity, int64_t* count);
+/* hasty_llr_blocks with the corners already on the device (DEVICE int32 [nblk][3],
+ * disjoint blocks): no host copy and no synchronisation, so it can run under graph
+ * capture. A block that does not lie inside the image is skipped (left unchanged). */
+int hasty_llr_blocks_dev(void* data, const int64_t* dims5, const int64_t* block_shape, const int32_t* corners_dev,
+                         int64_t nblk, double thresh, int soft, void* workspace, size_t workspace_bytes, void* stream);
+
 /* svt_soft / svt_hard (svt.cpp:62-169) on one row-major [R][F] complex64 matrix, in
  * place. workspace >= hasty_svt_workspace_size(R, F). */
 size_t hasty_svt_workspace_size(int64_t R, int64_t F);

@@ -114,3 +114,20 @@ def test_serving_request_stream_host_logic():
     assert all(r.n in (96, 128, 160) and 8 <= r.ncoils <= 32 and 1000 <= r.nspokes <= 10000 for r in a)
     assert all(x.arrival < y.arrival for x, y in zip(a, a[1:]))
     assert serving.resident_bytes(128, 16) == 8 * 16 * 128 ** 3 + 4 * 256 ** 3 + 40 * 128 ** 3
+
+
+def test_llr_grid_corners_match_the_oracle_grid():
+    # the host enumeration behind hasty_llr_blocks_dev follows the reference's strided
+    # grid and skip rule (svt.cpp:378-404), as the oracle restates it
+    import numpy as np
+
+    from hastycompute_amd import native
+    from oracle import oracle as O
+
+    for dims, strides, shape in (([3, 1, 48, 40, 36], [16, 16, 16], [16, 16, 16]),
+                                 ([2, 2, 20, 24, 28], [4, 6, 5], [8, 8, 8])):
+        shifts = np.array(O.llr_shifts(4, shape), dtype=np.int32).reshape(3, 4)
+        ref = O.normal_blocks(dims[2:], strides, shape, 4, shifts)
+        for it in range(4):
+            got = native.llr_grid_corners(dims, strides, shape, shifts[:, it])
+            assert [tuple(c) for c in got] == [tuple(c) for c in ref[it]], (dims, it)

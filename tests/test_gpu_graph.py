@@ -132,3 +132,62 @@ def test_cg_solve_zero_start_skips_the_first_apply(N):
             cg.iterate(x, r, p, Ap, 0.0, l2)
         torch.cuda.synchronize()
         assert torch.equal(x, x_skip) and torch.equal(cg.state, st_skip)
+
+
+def test_llr_prox_graph_replays_equal_eager(N):
+    # a prox-gradient step recorded once as a HIP graph: the normal operator (gradient
+    # T x - b through the sub path), the step and the LLR prox on device-resident block
+    # corners (hasty_llr_blocks_dev) of two shifts; every replay equals the same step run
+    # eagerly through hasty_llr_normal_blocks; a corner outside the image is skipped
+    F, n = 6, 32
+    shape = (n, n, n)
+    K, S, b = _problem(N, shape, 3, F, seed=12)
+    op = N.SenseNormalToeplitz(S, K[:1], coil_batch=3, problem_batch=F)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(13)
+    x0 = torch.randn((F,) + shape, dtype=torch.complex64, device="cuda", generator=gen)
+    blk = [8, 8, 8]
+    shifts = [[0, 3], [0, 5], [0, 2]]
+    corner_sets = [N.llr_grid_corners([F, 1, n, n, n], blk, blk, [shifts[d][it] for d in range(3)]) for it in range(2)]
+    cdev = [torch.tensor(c, dtype=torch.int32, device="cuda") for c in corner_sets]
+    ws = N.llr_blocks_workspace(x0.view(F, 1, n, n, n), blk, max(len(c) for c in corner_sets))
+    thresh, step = 0.5, 1e-3
+    kidx = [0] * F
+
+    def eager(x):
+        grad = op.apply(x, kernel_index=kidx, sub=b)
+        y = x - step * grad
+        for it in range(2):
+            N.llr_normal_blocks(y.view(F, 1, n, n, n), blk, blk, 1, thresh, True,
+                                shifts=[[shifts[d][it]] for d in range(3)])
+        return y
+
+    x = x0.clone()
+    grad = torch.empty_like(x)
+
+    def record():
+        op.apply(x, out=grad, kernel_index=kidx, sub=b)
+        x.sub_(grad, alpha=step)
+        for it in range(2):
+            N.llr_blocks_dev(x.view(F, 1, n, n, n), blk, cdev[it], thresh, True, ws)
+
+    record()  # eager run on the same buffers first
+    ref = eager(x0.clone())
+    torch.cuda.synchronize()
+    assert torch.equal(x, ref)
+    gr = N.Graph(record)
+    for _ in range(3):
+        ref = eager(ref)
+        gr.replay()
+        gr.stream.synchronize()
+        torch.cuda.synchronize()
+        assert torch.equal(x, ref)
+    gr.close()
+    # a block outside the image is left unchanged, the one inside is thresholded to zero
+    y = x0.clone().view(F, 1, n, n, n)
+    before = y.clone()
+    bad = torch.tensor([[0, 0, 0], [n - 4, 0, 0]], dtype=torch.int32, device="cuda")
+    N.llr_blocks_dev(y, blk, bad, 1e6, True, ws)
+    torch.cuda.synchronize()
+    assert torch.count_nonzero(y[:, :, :8, :8, :8]) == 0
+    assert torch.equal(y[:, :, 8:], before[:, :, 8:])
