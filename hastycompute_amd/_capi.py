@@ -76,6 +76,7 @@ _SIGS = {
     "hasty_llr_normal_blocks": ([c_vp, P_i64, P_i64, P_i64, c_i64, P_i32, c_dbl, c_int, c_vp, c_sz, c_vp], c_int),
     "hasty_llr_blocks": ([c_vp, P_i64, P_i64, P_i32, c_i64, c_dbl, c_int, c_vp, c_sz, c_vp], c_int),
     "hasty_llr_blocks_workspace_size": ([P_i64, P_i64, c_i64], c_sz),
+    "hasty_nufft_plan_shape": ([c_vp, P_i64, P_i64, P_i64], c_int),
     "hasty_llr_grid_corners": ([P_i64, P_i64, P_i64, P_i32, P_i32, c_i64, P_i64], c_int),
     "hasty_llr_blocks_dev": ([c_vp, P_i64, P_i64, c_vp, c_i64, c_dbl, c_int, c_vp, c_sz, c_vp], c_int),
     "hasty_svt_workspace_size": ([c_i64, c_i64], c_sz),

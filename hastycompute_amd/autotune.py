@@ -30,13 +30,17 @@ def sense_normal(nmodes3: Sequence[int], ncoil: int, nproblems: int) -> dict:
 
 
 def nufft(plan, c, f) -> dict:
-    """The 3-D type-1 spreader of a NufftPlan, timed on its own points (c -> f; f is
-    overwritten). The plan keeps the fastest."""
+    """The 3-D type-1 spreader, spreading tile edge and points per workgroup of a
+    NufftPlan, timed on its own points (c -> f; f is overwritten). The plan keeps the
+    fastest."""
     v = ctypes.c_int64()
     ms = ctypes.c_double()
     C.call("hasty_nufft_autotune", plan._h, c.data_ptr(), f.data_ptr(), ctypes.byref(v), ctypes.byref(ms),
            C.stream_ptr())
-    return {"nufft_spread_variant": int(v.value), "execute_ms": ms.value}
+    tile, chunk = ctypes.c_int64(), ctypes.c_int64()
+    C.call("hasty_nufft_plan_shape", plan._h, ctypes.byref(tile), ctypes.byref(chunk), None)
+    return {"nufft_spread_variant": int(v.value), "nufft_spread_tile": int(tile.value),
+            "nufft_spread_chunk": int(chunk.value), "execute_ms": ms.value}
 
 
 def llr(dims5: Sequence[int], block_strides: Sequence[int], block_shape: Sequence[int]) -> dict:

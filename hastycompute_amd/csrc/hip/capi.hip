@@ -759,6 +759,17 @@ int hasty_nufft_autotune(hasty_nufft_plan plan, void* c, void* f, int64_t* sprea
     });
 }
 
+int hasty_nufft_plan_shape(hasty_nufft_plan plan, int64_t* tile, int64_t* chunk, int64_t* spread_variant)
+{
+    return guarded([&] {
+        HASTY_REQUIRE(plan != nullptr, "null plan");
+        const NufftPlan* p = plan->impl;
+        if (tile) *tile = p->dim == 3 ? nufft_tile_edge(p) : 0;
+        if (chunk) *chunk = p->chunk;
+        if (spread_variant) *spread_variant = p->spread_variant >= 0 ? p->spread_variant : nufft_get_spread_variant();
+    });
+}
+
 int hasty_autotune_llr(const int64_t* dims5, const int64_t* block_strides, const int64_t* block_shape,
                        int64_t* nsplit, double* ms, void* stream)
 {

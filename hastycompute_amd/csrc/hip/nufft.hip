@@ -1046,6 +1046,30 @@ template <typename R> void execute_type1(NufftPlan* p, const cplx<R>* c, cplx<R>
 
 }  // namespace
 
+int nufft_tile_edge(const NufftPlan* p) { return p->bs[2] + p->ns + 1; }
+
+void nufft_retile(NufftPlan* p, int tile, int chunk, hipStream_t s)
+{
+    HASTY_REQUIRE(p != nullptr && p->dim == 3, "tiles are chosen for 3-D plans");
+    HASTY_REQUIRE(tile >= 12 && tile <= 28 && chunk >= 32 && chunk <= 8192, "tile or chunk out of range");
+    HASTY_HIP_CHECK(hipStreamSynchronize(s));  // the bin arrays may still be in use
+    const int b = std::max(2, tile - p->ns - 1);
+    for (int d = 0; d < 3; ++d) {
+        p->bs[d] = b;
+        p->nbins[d] = ceil_div(p->nf[d], b);
+    }
+    p->chunk = chunk;
+    const int64_t nb = bin_range(p->nbins, (p->locality & 1) ? kSuper : 1);
+    dev_free(p->counts);
+    dev_free(p->binstart);
+    dev_free(p->nsubbin);
+    p->counts = p->binstart = p->nsubbin = nullptr;
+    p->counts = (int*)dev_alloc(sizeof(int) * nb);
+    p->binstart = (int*)dev_alloc(sizeof(int) * (nb + 1));
+    p->nsubbin = (int*)dev_alloc(sizeof(int) * (nb + 1));
+    nufft_setpts(p, p->M, p->pts[2], p->pts[1], p->pts[0], s);
+}
+
 void nufft_setpts(NufftPlan* p, int64_t M, const void* x, const void* y, const void* z, hipStream_t s)
 {
     HASTY_REQUIRE(p != nullptr, "null plan");

@@ -68,6 +68,10 @@ void nufft_set_tile(int v);
 int nufft_get_tile();
 void nufft_set_chunk(int v);
 int nufft_get_chunk();
+// Re-bin a 3-D plan's points (as last given to setpts) with another tile edge (halo
+// included) and subproblem size; synchronises the stream.
+void nufft_retile(NufftPlan* p, int tile, int chunk, hipStream_t s);
+int nufft_tile_edge(const NufftPlan* p);
 
 // Time the 3-D type-1 spreaders on the plan's points (c -> f, f overwritten) and keep
 // the fastest for this plan; returns it and its execute time in *ms. Other plans: no-op

@@ -252,6 +252,24 @@ int nufft_autotune(NufftPlan* p, void* c, void* f, hipStream_t s, double* ms)
         throw;
     }
     p->spread_variant = best;
+    // spreading tile (edge with halo) and points per workgroup, complex64 plans: the
+    // points are re-binned for each candidate; the current shape is timed first
+    if (!p->fp64) {
+        const int t0 = nufft_tile_edge(p), c0 = p->chunk;
+        int bt = t0, bc = c0;
+        const std::pair<int, int> cands[] = {{20, 128}, {16, 256}, {20, 256}, {16, 128}, {20, 64}};
+        for (const auto& tc : cands) {
+            if (tc.first == t0 && tc.second == c0) continue;
+            nufft_retile(p, tc.first, tc.second, s);
+            const double t = time_best([&] { nufft_execute(p, c, f, s); }, s, 3);
+            if (t < kGain * tbest) {
+                tbest = t;
+                bt = tc.first;
+                bc = tc.second;
+            }
+        }
+        if (nufft_tile_edge(p) != bt || p->chunk != bc) nufft_retile(p, bt, bc, s);
+    }
     if (ms) *ms = tbest;
     return best;
 }

@@ -164,15 +164,18 @@ int hasty_get_option(const char* name, int64_t* value);
  *   sense_zconv_batch, problem_batch, coil_batch, sense_zconv_variant} (size the
  *   workspace with problem_batch and coil_batch) and the time per problem-apply.
  * hasty_nufft_autotune: the 3-D type-1 spreaders (nufft_spread_variant 0..3) of `plan`
- *   on its own points, executing c -> f (f is overwritten); the plan keeps the fastest.
- *   Other plans keep their spreader. *spread_variant receives the choice, *ms the
- *   execute time.
+ *   on its own points, executing c -> f (f is overwritten), then (complex64 plans) the
+ *   spreading tile edge and points per workgroup (the points are re-binned for each;
+ *   the stream is synchronised); the plan keeps the fastest. Other plans keep theirs.
+ *   *spread_variant receives the spreader, *ms the execute time;
+ *   hasty_nufft_plan_shape reports a plan's tile edge, chunk and spreader.
  * hasty_autotune_llr: the Gram row split of hasty_llr_normal_blocks for this image and
  *   block grid (threshold 0 on synthetic data); later calls with the same number of
  *   blocks per pass, rows and frames use it (size workspaces after this call). */
 int hasty_autotune_sense_normal(const int64_t* nmodes3, int ncoil, int nproblems, int64_t* choice,
                                 double* ms_per_problem, void* stream);
 int hasty_nufft_autotune(hasty_nufft_plan plan, void* c, void* f, int64_t* spread_variant, double* ms, void* stream);
+int hasty_nufft_plan_shape(hasty_nufft_plan plan, int64_t* tile, int64_t* chunk, int64_t* spread_variant);
 int hasty_autotune_llr(const int64_t* dims5, const int64_t* block_strides, const int64_t* block_shape,
                        int64_t* nsplit, double* ms, void* stream);
 

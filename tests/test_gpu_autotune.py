@@ -23,7 +23,8 @@ def N():
 def restore_options():
     from hastycompute_amd import _capi as C
 
-    names = ("sense_xy_variant", "sense_zconv_batch", "sense_zconv_variant", "nufft_spread_variant")
+    names = ("sense_xy_variant", "sense_zconv_batch", "sense_zconv_variant", "nufft_spread_variant",
+             "nufft_spread_tile", "nufft_spread_chunk")
     saved = {k: C.get_option(k) for k in names}
     yield
     for k, v in saved.items():
@@ -70,6 +71,7 @@ def test_autotune_nufft_type1(N, restore_options):
     t = autotune.nufft(plan, c, f)
     print(f"autotune nufft: {t}", flush=True)
     assert t["nufft_spread_variant"] in (0, 1, 2, 3) and t["execute_ms"] > 0
+    assert t["nufft_spread_tile"] in (16, 20) and t["nufft_spread_chunk"] in (64, 128, 256)
     plan.execute(c, f)
     torch.cuda.synchronize()
     assert O.rel_l2(f.cpu().numpy(), f0.cpu().numpy()) < 1e-5
@@ -90,3 +92,28 @@ def test_autotune_llr_row_split(N):
     after = N.llr_normal_blocks(x.clone(), (16, 16, 16), (16, 16, 16), 1, 2.0, True)
     torch.cuda.synchronize()
     assert O.rel_l2(after.cpu().numpy(), before.cpu().numpy()) < 1e-4
+
+
+@pytest.mark.parametrize("tile,chunk", [(16, 64), (24, 1024), (13, 256)])
+def test_nufft_spread_tile_options_vs_direct_sum(N, restore_options, tile, chunk):
+    # plans made under the nufft_spread_tile / nufft_spread_chunk options (tiles from
+    # 13^3 to 24^3 cells with halo, 64..1024 points per workgroup) spread exactly what the
+    # direct sum gives, on a radial trajectory (a crowded centre) at ns = 7
+    from hastycompute_amd import _capi as C
+
+    n = 24
+    co = O.kooshball(200, 48)
+    M = co.shape[1]
+    rng = np.random.default_rng(tile + chunk)
+    c = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
+    C.set_option("nufft_spread_tile", tile)
+    C.set_option("nufft_spread_chunk", chunk)
+    plan = N.NufftPlan(torch.as_tensor(co.astype(np.float32)).cuda(), [n, n, n], 1, 1, 1, 1e-6)
+    f = torch.empty((1, n, n, n), dtype=torch.complex64, device="cuda")
+    plan.execute(torch.as_tensor(c[None]).cuda(), f)
+    torch.cuda.synchronize()
+    plan.close()
+    ref = O.nudft_type1(co.astype(np.float32).astype(np.float64), c.astype(np.complex128), [n, n, n], 1)
+    e = O.rel_l2(f.cpu().numpy()[0].astype(np.complex128), ref)
+    print(f"tile {tile} chunk {chunk}: {e:.2e}", flush=True)
+    assert e < 1e-5
