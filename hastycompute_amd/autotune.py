@@ -44,9 +44,10 @@ def nufft(plan, c, f) -> dict:
 
 
 def llr(dims5: Sequence[int], block_strides: Sequence[int], block_shape: Sequence[int]) -> dict:
-    """The Gram row split of the LLR prox for this image and block grid."""
+    """The Gram row split of the LLR prox for this image and block grid, then the
+    eigensolver's workgroup size (the process-wide llr_eig_threads option)."""
     ns = ctypes.c_int64()
     ms = ctypes.c_double()
     C.call("hasty_autotune_llr", C.i64arr(dims5), C.i64arr(block_strides), C.i64arr(block_shape), ctypes.byref(ns),
            ctypes.byref(ms), C.stream_ptr())
-    return {"llr_row_split": int(ns.value), "pass_ms": ms.value}
+    return {"llr_row_split": int(ns.value), "llr_eig_threads": C.get_option("llr_eig_threads"), "pass_ms": ms.value}

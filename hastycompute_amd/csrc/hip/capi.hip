@@ -370,6 +370,10 @@ int hasty_set_option(const char* name, int64_t value)
         sense_fused_set_zconv_batch((int)value);
         return 0;
     }
+    if (name && std::strcmp(name, "llr_eig_threads") == 0 && (value == 256 || value == 512 || value == 768)) {
+        llr_set_eig_threads((int)value);
+        return 0;
+    }
     if (name && std::strcmp(name, "llr_max_sweeps") == 0 && value >= 1 && value <= 15) {
         llr_set_max_sweeps((int)value);
         return 0;
@@ -405,6 +409,7 @@ int hasty_get_option(const char* name, int64_t* value)
     else if (name && std::strcmp(name, "nufft_spread_variant") == 0) *value = nufft_get_spread_variant();
     else if (name && std::strcmp(name, "sense_zconv_batch") == 0) *value = sense_fused_get_zconv_batch();
     else if (name && std::strcmp(name, "llr_max_sweeps") == 0) *value = llr_get_max_sweeps();
+    else if (name && std::strcmp(name, "llr_eig_threads") == 0) *value = llr_get_eig_threads();
     else if (name && std::strcmp(name, "sense_zconv_variant") == 0) *value = sense_fused_get_zconv_variant();
     else if (name && std::strcmp(name, "nufft_spread_tile") == 0) *value = nufft_get_tile();
     else if (name && std::strcmp(name, "nufft_spread_chunk") == 0) *value = nufft_get_chunk();
@@ -820,6 +825,29 @@ int hasty_autotune_llr(const int64_t* dims5, const int64_t* block_strides, const
                     best = v;
                 }
             }
+            // the eigensolver's workgroup size (a process-wide option) with that split
+            {
+                std::lock_guard<std::mutex> lock(g_nsplit_mu);
+                g_nsplit_tuned[key] = best;
+            }
+            const int nth0 = llr_get_eig_threads();
+            int best_nth = nth0;
+            // the workspace of the chosen split (the loop above left the last tried one's)
+            const size_t wsb = llr_ws_bytes_for(nblk, g.F, R);
+            dev_free(ws);
+            ws = nullptr;
+            ws = dev_alloc(wsb);
+            for (int nth : {256, 512, 768}) {
+                if (nth == nth0) continue;
+                llr_set_eig_threads(nth);
+                const double t = time_best_ms(
+                    [&] { run_llr_batch((float2*)img, g, blocks, 0.f, 1, ws, wsb, s); }, s, 3);
+                if (t < 0.98 * tbest) {
+                    tbest = t;
+                    best_nth = nth;
+                }
+            }
+            llr_set_eig_threads(best_nth);
         } catch (...) {
             dev_free(ws);
             dev_free(img);

@@ -26,6 +26,12 @@ namespace {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kThreads = 256;
+// The eigensolver's workgroup: 256, 512 or 768 threads (option llr_eig_threads); every
+// thread owns up to ceil(n (n + 2) / 8 / threads) upper 2x2 blocks of the n x n problem.
+// A Jacobi round runs those block updates between two barriers, so more threads shorten
+// it until the padded block slots and the waves meeting at each barrier cost more than
+// they save. 100 frames, 343 blocks: 256 threads 1.16 ms, 512 1.04 ms, 1024 1.40 ms.
+std::atomic<int> g_eig_threads{512};
 constexpr int kMaxTilesPerWave = 9;  // F <= 128 -> 8x8 tiles -> 36 upper tiles / 4 waves
 constexpr int kMaxSweeps = 15;       // Jacobi sweeps per block (4-6 reach FP32 rounding)
 std::atomic<int> g_max_sweeps{kMaxSweeps};  // cap in force (option "llr_max_sweeps")
@@ -250,12 +256,19 @@ __device__ __forceinline__ void wave_lds_sync()
 }
 
 // Sum over the workgroup, returned to every thread.
-__device__ double block_sum(double v, double* red)
+template <int NTH> __device__ double block_sum(double v, double* red)
 {
+    // the largest power of two <= NTH: the threads above it fold in first
+    constexpr int P2 = NTH >= 512 ? 512 : (NTH >= 256 ? 256 : 128);
+    static_assert(NTH >= P2 && NTH - P2 <= P2, "workgroup size out of range");
     const int tid = threadIdx.x;
     red[tid] = v;
     __syncthreads();
-    for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if (NTH > P2) {
+        if (tid < NTH - P2) red[tid] += red[tid + P2];
+        __syncthreads();
+    }
+    for (int o = P2 / 2; o > 0; o >>= 1) {
         if (tid < o) red[tid] += red[tid + o];
         __syncthreads();
     }
@@ -304,13 +317,13 @@ struct EigLayout {
 };
 
 // Upper 2x2 blocks (pairs k <= l) one thread updates per round, at most
-// ceil(64 * 65 / 2 / kThreads) = 9 (n = 128); the kernel is instantiated per count so the
-// registers it holds fit two workgroups per CU.
+// ceil(64 * 65 / 2 / threads) = 9 (n = 128, 256 threads); the kernel is instantiated per
+// (threads, count).
 constexpr int kMaxBlkPerThread = 9;
-__host__ __device__ inline int eig_blk_per_thread(int n)
+__host__ __device__ inline int eig_blk_per_thread(int n, int nth)
 {
     const int half = n / 2;
-    return (half * (half + 1) / 2 + kThreads - 1) / kThreads;
+    return (half * (half + 1) / 2 + nth - 1) / nth;
 }
 
 // Complex index of element el = 2 i + j of block b in Gb: the blocks' top rows (elements
@@ -340,8 +353,8 @@ __device__ __forceinline__ int moved_loc(int s, int t, int h, int nb1)
 // v_k = J_1 J_2 ... J_R e_k by replaying the recorded rotations backwards, one wave per
 // vector (its lanes are the pairs, so a round needs no workgroup barrier); that pass
 // scales with the number of singular values that survive the threshold.
-template <int kBlkPerThread>
-__global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restrict__ gpart, int nsplit, int F, int Fp,
+template <int NTH, int kBlkPerThread>
+__global__ void __launch_bounds__(NTH) eig_w_kernel(const float2* __restrict__ gpart, int nsplit, int F, int Fp,
                                                         float thresh, int soft, int rank, int max_sweeps,
                                                         float4* rot_all, float2* __restrict__ Wout,
                                                         float* __restrict__ eigout)
@@ -355,7 +368,7 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
     float4* rcs = reinterpret_cast<float4*>(smem_raw + L.rc);
     float* gk = reinterpret_cast<float*>(smem_raw + L.gk);
     int* kl = reinterpret_cast<int*>(smem_raw + L.kl);
-    __shared__ double red[kThreads];
+    __shared__ double red[NTH];
     __shared__ int nk_s;
     const int blk = blockIdx.x;
     const int tid = threadIdx.x;
@@ -364,7 +377,7 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
     // G in slot order (sum of the partial Grams in split order; the partials are full
     // Hermitian matrices).
     const float2* Gb = gpart + (int64_t)blk * nsplit * Fp * Fp;
-    for (int e = tid; e < n * n; e += kThreads) {
+    for (int e = tid; e < n * n; e += NTH) {
         const int s = e / n, t = e - (e / n) * n;
         if (t < s) continue;
         const int i = slot0_index(s, n), j = slot0_index(t, n);
@@ -380,14 +393,14 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
         Gs[gel(tri_idx(s >> 1, t >> 1, half), ((s & 1) << 1) + (t & 1), nb1)] = v;
     }
 
-    // This thread's blocks: e = tid + kThreads i in row-major order over the upper
+    // This thread's blocks: e = tid + NTH i in row-major order over the upper
     // triangle of pairs (k, l), k <= l; and where their elements go after a round.
     int bk[kBlkPerThread], bl[kBlkPerThread], src[kBlkPerThread], dst[kBlkPerThread][4];
     {
         int k = 0, rem = tid;
 #pragma unroll
         for (int i = 0; i < kBlkPerThread; ++i) {
-            if (i > 0) rem += kThreads;
+            if (i > 0) rem += NTH;
             while (k < half && rem >= half - k) {
                 rem -= half - k;
                 ++k;
@@ -412,13 +425,13 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
     // |G|_F^2 and the off-diagonal mass from the kept elements (off-diagonal ones count twice)
     auto mass = [&](bool off_only) {
         double a = 0.0;
-        for (int e = tid; e < n * n; e += kThreads) {
+        for (int e = tid; e < n * n; e += NTH) {
             const int s = e / n, t = e - (e / n) * n;
             if (t < s || (off_only && s == t)) continue;
             const float2 v = Gs[gel(tri_idx(s >> 1, t >> 1, half), ((s & 1) << 1) + (t & 1), nb1)];
             a += (s == t ? 1.0 : 2.0) * ((double)v.x * v.x + (double)v.y * v.y);
         }
-        return block_sum(a, red);
+        return block_sum<NTH>(a, red);
     };
     const double total = mass(false);
 
@@ -511,7 +524,7 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
         const int s = index_slot0(k, n);
         return Gs[gel(tri_idx(s >> 1, s >> 1, half), 3 * (s & 1), nb1)].x;
     };
-    for (int k = tid; k < n; k += kThreads) {
+    for (int k = tid; k < n; k += NTH) {
         const float lam = eig_of(k);
         float gv = 0.f;
         if (k < F) {
@@ -554,7 +567,7 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
     const int cap = half;
     for (int c0 = 0; c0 < max(nk, 1); c0 += cap) {
         const int nkc = min(cap, nk - c0);
-        for (int e = tid; e < nkc * n; e += kThreads) {
+        for (int e = tid; e < nkc * n; e += NTH) {
             const int j = e / n, x = e - (e / n) * n;
             X[e] = float2{x == kl[c0 + j] ? 1.f : 0.f, 0.f};
         }
@@ -587,7 +600,7 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
                         const int q = 1 + B;
                         const float c = cur[j].x, s = cur[j].y;
                         const float2 eph = float2{cur[j].z, -cur[j].w};
-                        for (int v = wave; v < nkc; v += kThreads / 64) {
+                        for (int v = wave; v < nkc; v += NTH / 64) {
                             float2* xv = X + v * n;
                             const float2 xp = xv[p], xq = xv[q];
                             xv[p] = float2{c * xp.x + s * xq.x, c * xp.y + s * xq.y};
@@ -604,7 +617,7 @@ __global__ void __launch_bounds__(kThreads) eig_w_kernel(const float2* __restric
         }
         __syncthreads();
         // W (+)= sum_v g_v X_v X_v^H
-        for (int e = tid; e < Fp * Fp; e += kThreads) {
+        for (int e = tid; e < Fp * Fp; e += NTH) {
             const int i = e / Fp, j = e - (e / Fp) * Fp;
             float2 acc = c0 == 0 ? float2{0.f, 0.f} : W[e];
             if (i < F && j < F) {
@@ -786,13 +799,13 @@ void launch_apply(float2* data, const LlrGeom& g, const int3* corners, int64_t n
     check_launch("apply_kernel");
 }
 
-template <int B>
+template <int NTH, int B>
 void launch_eig(const float2* gpart, int nsplit, const LlrGeom& g, int Fp, float thresh, int soft, int rank,
                 float4* rot, float2* W, float* eig_out, int64_t nblk, hipStream_t s)
 {
     const size_t smem = (size_t)EigLayout((int)(g.F + (g.F & 1))).bytes;
-    allow_lds(eig_w_kernel<B>, smem);
-    hipLaunchKernelGGL(eig_w_kernel<B>, dim3((unsigned)nblk), dim3(kThreads), smem, s, gpart, nsplit, (int)g.F, Fp,
+    allow_lds(eig_w_kernel<NTH, B>, smem);
+    hipLaunchKernelGGL((eig_w_kernel<NTH, B>), dim3((unsigned)nblk), dim3(NTH), smem, s, gpart, nsplit, (int)g.F, Fp,
                        thresh, soft, rank, g_max_sweeps, rot, W, eig_out);
     check_launch("eig_w_kernel");
 }
@@ -827,11 +840,19 @@ void llr_set_max_sweeps(int sweeps)
 
 int llr_get_max_sweeps() { return g_max_sweeps; }
 
+void llr_set_eig_threads(int threads)
+{
+    HASTY_REQUIRE(threads == 256 || threads == 512 || threads == 768, "llr_eig_threads must be 256, 512 or 768");
+    g_eig_threads = threads;
+}
+
+int llr_get_eig_threads() { return g_eig_threads; }
+
 void llr_check(int64_t F)
 {
     HASTY_REQUIRE(F >= 1 && F <= 128, "LLR supports 1..128 columns (frames)");
-    // dynamic LDS of the eigensolver plus its 2 KiB static reduction buffer
-    HASTY_REQUIRE(llr_eig_smem(F) + sizeof(double) * kThreads + 16 <= 160 * 1024, "LLR eigensolver tile exceeds LDS");
+    // dynamic LDS of the eigensolver plus its static reduction buffer (768 threads at most)
+    HASTY_REQUIRE(llr_eig_smem(F) + sizeof(double) * 768 + 16 <= 160 * 1024, "LLR eigensolver tile exceeds LDS");
 }
 
 // Threshold the blocks at `corners_dev` (all disjoint) in place.
@@ -867,17 +888,37 @@ void llr_blocks(float2* data, const LlrGeom& g, const int3* corners_dev, int64_t
         KernelTimer t("llr_eig", s);
         const int n = (int)(g.F + (g.F & 1));
         const float2* gp = gpart;
-        switch (eig_blk_per_thread(n)) {
-        case 1: launch_eig<1>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
-        case 2: launch_eig<2>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
-        case 3: launch_eig<3>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
-        case 4: launch_eig<4>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
-        case 5: launch_eig<5>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
-        case 6: launch_eig<6>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
-        case 7: launch_eig<7>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
-        case 8: launch_eig<8>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
-        default: launch_eig<kMaxBlkPerThread>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s); break;
+        const int nth = g_eig_threads.load();
+        const int bpt = eig_blk_per_thread(n, nth);
+#define HASTY_EIG(NTH, B) launch_eig<NTH, B>(gp, nsplit, g, Fp, thresh, soft, rank, rot, W, eig_out, nblk, s)
+        if (nth == 768) {
+            switch (bpt) {
+            case 1: HASTY_EIG(768, 1); break;
+            case 2: HASTY_EIG(768, 2); break;
+            default: HASTY_EIG(768, 3); break;
+            }
+        } else if (nth == 512) {
+            switch (bpt) {
+            case 1: HASTY_EIG(512, 1); break;
+            case 2: HASTY_EIG(512, 2); break;
+            case 3: HASTY_EIG(512, 3); break;
+            case 4: HASTY_EIG(512, 4); break;
+            default: HASTY_EIG(512, 5); break;
+            }
+        } else {
+            switch (bpt) {
+            case 1: HASTY_EIG(256, 1); break;
+            case 2: HASTY_EIG(256, 2); break;
+            case 3: HASTY_EIG(256, 3); break;
+            case 4: HASTY_EIG(256, 4); break;
+            case 5: HASTY_EIG(256, 5); break;
+            case 6: HASTY_EIG(256, 6); break;
+            case 7: HASTY_EIG(256, 7); break;
+            case 8: HASTY_EIG(256, 8); break;
+            default: HASTY_EIG(256, kMaxBlkPerThread); break;
+            }
         }
+#undef HASTY_EIG
     }
     {
         const int64_t rows_per_wg = 16 * (kApplyThreads / 64) * kApplySlabsPerWave;

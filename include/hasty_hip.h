@@ -136,6 +136,7 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
  *     buffers; 3: 8 columns, two buffers) and at N0 = 160 (0: 20 x 16 register FFT, one
  *     buffer; 2: the same with two buffers; 1, 3: LDS Stockham passes).
  *   "llr_max_sweeps" 1..15 (default 15): cap on the Jacobi sweeps of the LLR eigensolver.
+ *   "llr_eig_threads" 256, 512 or 768 (default 512): the LLR eigensolver's workgroup.
  *   "nufft_spread_tile" 0 or 12..28 (default 0 = 20): 3-D spreading tile edge in fine-grid
  *     cells, halo included, of new plans (the tile must fit in LDS: 20 for complex128).
  *   "nufft_spread_chunk" 0 or 32..8192 (default 0 = 128 for complex64, 1024 for
@@ -171,7 +172,8 @@ int hasty_get_option(const char* name, int64_t* value);
  *   hasty_nufft_plan_shape reports a plan's tile edge, chunk and spreader.
  * hasty_autotune_llr: the Gram row split of hasty_llr_normal_blocks for this image and
  *   block grid (threshold 0 on synthetic data); later calls with the same number of
- *   blocks per pass, rows and frames use it (size workspaces after this call). */
+ *   blocks per pass, rows and frames use it (size workspaces after this call); then the
+ *   eigensolver's workgroup size, which it sets as the llr_eig_threads option. */
 int hasty_autotune_sense_normal(const int64_t* nmodes3, int ncoil, int nproblems, int64_t* choice,
                                 double* ms_per_problem, void* stream);
 int hasty_nufft_autotune(hasty_nufft_plan plan, void* c, void* f, int64_t* spread_variant, double* ms, void* stream);

@@ -24,7 +24,7 @@ def restore_options():
     from hastycompute_amd import _capi as C
 
     names = ("sense_xy_variant", "sense_zconv_batch", "sense_zconv_variant", "nufft_spread_variant",
-             "nufft_spread_tile", "nufft_spread_chunk")
+             "nufft_spread_tile", "nufft_spread_chunk", "llr_eig_threads")
     saved = {k: C.get_option(k) for k in names}
     yield
     for k, v in saved.items():
@@ -78,7 +78,7 @@ def test_autotune_nufft_type1(N, restore_options):
     plan.close()
 
 
-def test_autotune_llr_row_split(N):
+def test_autotune_llr_row_split(N, restore_options):
     from hastycompute_amd import autotune
 
     g = torch.Generator(device="cuda")
@@ -88,7 +88,7 @@ def test_autotune_llr_row_split(N):
     before = N.llr_normal_blocks(x.clone(), (16, 16, 16), (16, 16, 16), 1, 2.0, True)
     t = autotune.llr(dims, (16, 16, 16), (16, 16, 16))
     print(f"autotune llr: {t}", flush=True)
-    assert 1 <= t["llr_row_split"] <= 8
+    assert 1 <= t["llr_row_split"] <= 8 and t["llr_eig_threads"] in (256, 512, 768)
     after = N.llr_normal_blocks(x.clone(), (16, 16, 16), (16, 16, 16), 1, 2.0, True)
     torch.cuda.synchronize()
     assert O.rel_l2(after.cpu().numpy(), before.cpu().numpy()) < 1e-4
@@ -117,3 +117,26 @@ def test_nufft_spread_tile_options_vs_direct_sum(N, restore_options, tile, chunk
     e = O.rel_l2(f.cpu().numpy()[0].astype(np.complex128), ref)
     print(f"tile {tile} chunk {chunk}: {e:.2e}", flush=True)
     assert e < 1e-5
+
+
+@pytest.mark.parametrize("threads", [256, 512, 768])
+def test_llr_eig_threads_vs_oracle(N, restore_options, threads):
+    # the LLR prox with each eigensolver workgroup size (3, 5 or 9 blocks per thread at
+    # 100 frames) against the oracle's LAPACK SVT of the same blocks
+    from hastycompute_amd import _capi as C
+
+    rng = np.random.default_rng(threads)
+    F, n = 100, 32
+    U = rng.standard_normal((5, n ** 3)) + 1j * rng.standard_normal((5, n ** 3))
+    V = rng.standard_normal((F, 5)) + 1j * rng.standard_normal((F, 5))
+    x = ((V @ U).reshape(F, 1, n, n, n) + 0.05 * (rng.standard_normal((F, 1, n, n, n)) +
+                                                 1j * rng.standard_normal((F, 1, n, n, n)))).astype(np.complex64)
+    C.set_option("llr_eig_threads", threads)
+    blk = (16, 16, 16)
+    sig = np.linalg.svd(x[:, 0, :16, :16, :16].reshape(F, -1).T.astype(np.complex128), compute_uv=False)[0]
+    lam = 0.01 * sig
+    out = N.llr_normal_blocks(torch.as_tensor(x).cuda(), blk, blk, 1, float(lam), True).cpu().numpy()
+    ref = O.normal_3d_blocks_svt(x.astype(np.complex128), list(blk), list(blk), 1, lam, True)
+    e = O.rel_l2(out.astype(np.complex128), ref)
+    print(f"llr eig threads {threads}: {e:.2e}", flush=True)
+    assert e < 1e-4
