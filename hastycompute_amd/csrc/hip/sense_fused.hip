@@ -806,6 +806,26 @@ template <int R, int S, bool HALF = false> __device__ __forceinline__ void dftn(
     else dft20_dr<S, HALF>(v);
 }
 
+// Buffer loads and stores on one plane: a descriptor in SGPRs (base, size in bytes;
+// accesses past the size read zero and write nothing), a per-lane 32-bit byte offset
+// and a uniform one, so a thread needs one VGPR of address for all its accesses.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+// ST: streaming data read or written once (non-temporal: kept out of the caches' way of
+// data that is used again, such as the image plane re-read for every coil).
+template <bool ST = false> __device__ __forceinline__ float2 bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, ST ? 2 : 0));
+}
+template <bool ST = false>
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, float2 v, uint32_t voff, uint32_t soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, ST ? 2 : 0);
+}
+
 // Plane kernels with the L-point y transforms as register four-step FFTs,
 // L = 16 * R2 (R2 = L / 16 = 4 NB): row y = y1 + 16 y2 and frequency ky = k1 + R2 k2, so
 //   X[k1 + R2 k2] = sum_{y1} w16^{y1 k2} [ wL^{y1 k1} sum_{y2} x[y1 + 16 y2] wR2^{y2 k1} ].
@@ -848,21 +868,24 @@ template <int NB> __global__ void __launch_bounds__(kXYW) fwd_xy_w_kernel(FusedA
     for (int i = t; i < L; i += kXYW) twL[i] = a.twL[i];
     if (t < 32) tw32[t] = a.tw32[t];
     const int64_t plane = (int64_t)N * N;
-    const float2* xp = a.x + ((int64_t)p * a.N0 + z) * plane;
+    // x and the coil maps through buffer descriptors: one VGPR of address per thread
+    // (pixel (yl + 16 k, b + 32 j) at byte xoff + (16 k N + 32 j) * 8)
+    const auto xr = plane_rsrc(a.x + ((int64_t)p * a.N0 + z) * plane, sizeof(float2) * N * N);
+    const uint32_t xoff = (uint32_t)(yl * N + b) * 8u;
     const LaneTw32 lt = lane_tw32(b, a.tw32);
     lds_barrier();
 
     const int ncl = coil_count(a, a.p0 + pl);
     for (int ci = 0; ci < ncl; ++ci) {
         const int coil = a.coils[a.c0 + ci] + p * a.coil_pstride;
-        const float2* sp = a.smaps + ((int64_t)coil * a.N0 + z) * plane;
+        const auto sr = plane_rsrc(a.smaps + ((int64_t)coil * a.N0 + z) * plane, sizeof(float2) * N * N);
         float2 v[2 * NB][NB];
 #pragma unroll
         for (int k = 0; k < 2 * NB; ++k)
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
-                const int64_t idx = (int64_t)(yl + 16 * k) * N + b + 32 * j;
-                v[k][j] = cmul(xp[idx], sp[idx]);
+                const uint32_t so = (uint32_t)(16 * k * N + 32 * j) * 8u;
+                v[k][j] = cmul(bload(xr, xoff, so), bload<true>(sr, xoff, so));
             }
         float2* Tp = a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L;
         for (int s0 = 0; s0 < P; s0 += CS) {
@@ -962,6 +985,9 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
     for (int i = t; i < L; i += kXYW) twL[i] = a.twL[i];
     if (t < 32) tw32[t] = a.tw32[t];
     const int64_t plane = (int64_t)N * N;
+    // coil maps, output and sub through buffer descriptors: pixel (yl + 16 y2, b + 32 j)
+    // at byte xoff + (16 y2 N + 32 j) * 8
+    const uint32_t xoff = (uint32_t)(yl * N + b) * 8u;
     const LaneTw32 lt = lane_tw32(b, a.tw32);
     float2 acc[KR][NB];
 #pragma unroll
@@ -973,7 +999,7 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
     const int ncl = coil_count(a, a.p0 + pl);
     for (int ci = 0; ci < ncl; ++ci) {
         const int coil = a.coils[a.c0 + ci] + p * a.coil_pstride;
-        const float2* sp = a.smaps + ((int64_t)coil * a.N0 + z) * plane;
+        const auto sr = plane_rsrc(a.smaps + ((int64_t)coil * a.N0 + z) * plane, sizeof(float2) * N * N);
         // this coil's image, summed over the cosets; times conj(S) once at the end
         float2 ic[KR][NB];
 #pragma unroll
@@ -1031,19 +1057,20 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
         for (int k = 0; k < KR; ++k)
 #pragma unroll
             for (int j = 0; j < NB; ++j)
-                acc[k][j] = cfma_conj(acc[k][j], sp[(yl + 16 * (y0 + k)) * N + b + 32 * j], ic[k][j]);
+                acc[k][j] = cfma_conj(acc[k][j], bload<true>(sr, xoff, (uint32_t)(16 * (y0 + k) * N + 32 * j) * 8u),
+                                      ic[k][j]);
     }
-    float2* op = a.out + ((int64_t)p * a.N0 + z) * plane;
-    const float2* sb = a.sub ? a.sub + ((int64_t)p * a.N0 + z) * plane : nullptr;
+    const auto orr = plane_rsrc(a.out + ((int64_t)p * a.N0 + z) * plane, sizeof(float2) * N * N);
+    const auto sbr = plane_rsrc(a.sub ? a.sub + ((int64_t)p * a.N0 + z) * plane : a.out, sizeof(float2) * N * N);
 #pragma unroll
     for (int k = 0; k < KR; ++k)
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-            const int64_t idx = (int64_t)(yl + 16 * (y0 + k)) * N + b + 32 * j;
+            const uint32_t so = (uint32_t)(16 * (y0 + k) * N + 32 * j) * 8u;
             float2 rr = acc[k][j];
-            if (a.accumulate) rr = cadd(rr, op[idx]);
-            else if (sb) rr = csub(rr, sb[idx]);
-            op[idx] = rr;
+            if (a.accumulate) rr = cadd(rr, bload(orr, xoff, so));
+            else if (a.sub) rr = csub(rr, bload(sbr, xoff, so));
+            bstore(orr, rr, xoff, so);
         }
 }
 
@@ -1211,26 +1238,6 @@ __host__ __device__ __forceinline__ int64_t r16_pos(int ky, int kx)
     const int q = kx & 15, k2 = kx >> 4, h = k2 >> 2, c = k2 & 3;
     const int ka = ky & 7, kb = ky >> 3;
     return ((int64_t)(h * 32 + kb) * 8 + ka) * 64 + q + 16 * c;
-}
-
-// Buffer loads and stores on one plane: a descriptor in SGPRs (base, size in bytes;
-// accesses past the size read zero and write nothing), a per-lane 32-bit byte offset
-// and a uniform one, so a thread needs one VGPR of address for all its accesses.
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, uint32_t bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
-}
-// ST: streaming data read or written once (non-temporal: kept out of the caches' way of
-// data that is used again, such as the image plane re-read for every coil).
-template <bool ST = false> __device__ __forceinline__ float2 bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
-{
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, ST ? 2 : 0));
-}
-template <bool ST = false>
-__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, float2 v, uint32_t voff, uint32_t soff)
-{
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, ST ? 2 : 0);
 }
 
 // Order of LDS operations inside one wavefront (the hardware keeps it; this keeps the
