@@ -887,7 +887,9 @@ template <int NB> __global__ void __launch_bounds__(kXYW) fwd_xy_w_kernel(FusedA
                 const uint32_t so = (uint32_t)(16 * k * N + 32 * j) * 8u;
                 v[k][j] = cmul(bload(xr, xoff, so), bload<true>(sr, xoff, so));
             }
-        float2* Tp = a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L;
+        // this coil's T plane [s][ky][m]: element (s, ky, m) at byte (s CW + ky kW + m) * 8
+        const auto tr = plane_rsrc(a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L,
+                                   sizeof(float2) * L * L);
         for (int s0 = 0; s0 < P; s0 += CS) {
             // x step: for coset s the 32 frequencies P m + s of each row, written as rows
             // y < N of the coset block (column m)
@@ -935,9 +937,10 @@ template <int NB> __global__ void __launch_bounds__(kXYW) fwd_xy_w_kernel(FusedA
 #pragma unroll
                     for (int y1 = 0; y1 < 16; ++y1) u[y1] = buf[cs * CW + (k1 * 16 + y1) * kW + m];
                     dft16<-1>(u);
-                    float2* dst = Tp + (int64_t)(s0 + cs) * CW + m;
+                    const uint32_t voff = (uint32_t)(k1 * kW + m) * 8u;
 #pragma unroll
-                    for (int k2 = 0; k2 < 16; ++k2) dst[(k1 + R2 * k2) * kW] = u[k2];
+                    for (int k2 = 0; k2 < 16; ++k2)
+                        bstore(tr, u[k2], voff, (uint32_t)((s0 + cs) * CW + R2 * k2 * kW) * 8u);
                 }
             }
             lds_barrier();
@@ -1006,7 +1009,9 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
         for (int k = 0; k < KR; ++k)
 #pragma unroll
             for (int j = 0; j < NB; ++j) ic[k][j] = float2{0.f, 0.f};
-        const float2* Tp = a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L;
+        // this coil's T plane [s][ky][m]: element (s, ky, m) at byte (s CW + ky kW + m) * 8
+        const auto tr = plane_rsrc(a.T + (((int64_t)pl * a.ccount + ci) * a.N0 + z) * (int64_t)L * L,
+                                   sizeof(float2) * L * L);
         for (int s0 = 0; s0 < P; s0 += CS) {
             // step 1 (thread: column m, k1 = r, r + 16, ..., each coset of the step):
             // inverse DFT over k2 of X[k1 + R2 k2] -> y1, twiddle wL^{-y1 k1}, into LDS as
@@ -1016,9 +1021,10 @@ template <int NB> __global__ void __launch_bounds__(kXYW) inv_xy_w_kernel(FusedA
 #pragma unroll 1
                 for (int cs = 0; cs < CS; ++cs) {
                     float2 u[16];
-                    const float2* src = Tp + (int64_t)(s0 + cs) * CW + m;
+                    const uint32_t voff = (uint32_t)(k1 * kW + m) * 8u;
 #pragma unroll
-                    for (int k2 = 0; k2 < 16; ++k2) u[k2] = src[(k1 + R2 * k2) * kW];
+                    for (int k2 = 0; k2 < 16; ++k2)
+                        u[k2] = bload(tr, voff, (uint32_t)((s0 + cs) * CW + R2 * k2 * kW) * 8u);
                     dft16<1>(u);
 #pragma unroll
                     for (int q = 1; q < 16; ++q) u[q] = cmulc(twL[(q * k1) % L], u[q]);
