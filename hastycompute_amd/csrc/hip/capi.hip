@@ -12,6 +12,7 @@
 
 #include "../../../include/hasty_hip.h"
 #include "common.hpp"
+#include "fft.hpp"
 #include "nufft.hpp"
 #include "toeplitz.hpp"
 #include "cg.hpp"
@@ -362,7 +363,7 @@ int hasty_set_option(const char* name, int64_t value)
         sense_fused_set_xy_variant((int)value);
         return 0;
     }
-    if (name && std::strcmp(name, "nufft_spread_variant") == 0 && value >= 0 && value <= 3) {
+    if (name && std::strcmp(name, "nufft_spread_variant") == 0 && value >= 0 && value <= 6) {
         nufft_set_spread_variant((int)value);
         return 0;
     }
@@ -394,6 +395,10 @@ int hasty_set_option(const char* name, int64_t value)
         nufft_set_chunk((int)value);
         return 0;
     }
+    if (name && std::strcmp(name, "fft_strided_lines") == 0 && value >= 0 && value <= 64) {
+        fft_set_strided_lines((int)value);
+        return 0;
+    }
     set_last_error(std::string("unknown option ") + (name ? name : "(null)"));
     return 1;
 }
@@ -414,6 +419,7 @@ int hasty_get_option(const char* name, int64_t* value)
     else if (name && std::strcmp(name, "nufft_spread_tile") == 0) *value = nufft_get_tile();
     else if (name && std::strcmp(name, "nufft_spread_chunk") == 0) *value = nufft_get_chunk();
     else if (name && std::strcmp(name, "nufft_locality") == 0) *value = nufft_get_locality();
+    else if (name && std::strcmp(name, "fft_strided_lines") == 0) *value = fft_get_strided_lines();
     else {
         set_last_error(std::string("unknown option ") + (name ? name : "(null)"));
         return 1;

@@ -8,6 +8,7 @@
 //
 // Radix passes follow the self-sorting Stockham formulation of Govindaraju et al.,
 // "High performance discrete Fourier transforms on graphics processors" (SC08).
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -221,28 +222,45 @@ __global__ void __launch_bounds__(kFftThreads) axis_fft_kernel(AxisKernelArgs<R>
     }
     __syncthreads();
 
-    // Load (pad / centre / multiply / scale).
+    // Load (pad / centre / multiply / scale). Each thread issues kLoadUnroll independent
+    // loads before its first LDS store, so a workgroup keeps kLoadUnroll x 256 element
+    // loads in flight instead of 256.
+    constexpr int kLoadUnroll = sizeof(R) == 4 ? 8 : 4;
     const bool in_fast = a.ain.stride != 1;
-    for (int e = tid; e < nb * L; e += kFftThreads) {
-        int li, i;
-        if (in_fast) {
-            li = e % nb;
-            i = e / nb;
-        } else {
-            li = e / L;
-            i = e - li * L;
-        }
-        const int n = in_element(i, L, a.n_in, a.in_centered);
-        cplx<R> v = {(R)0, (R)0};
-        if (n >= 0) {
-            v = a.in[b_in[li] + (int64_t)n * a.ain.stride];
-            if (a.inmul) {
-                const cplx<R> m = a.inmul[b_mul[li] + (int64_t)n * a.amul.stride];
-                v = a.inmul_conj ? cmulc(m, v) : cmul(m, v);
+    const int total = nb * L;
+    for (int e0 = tid; e0 < total; e0 += kFftThreads * kLoadUnroll) {
+        cplx<R> v[kLoadUnroll];
+        int slot[kLoadUnroll];
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; ++u) {
+            const int e = e0 + u * kFftThreads;
+            v[u] = cplx<R>{(R)0, (R)0};
+            slot[u] = -1;
+            if (e < total) {
+                int li, i;
+                if (in_fast) {
+                    li = e % nb;
+                    i = e / nb;
+                } else {
+                    li = e / L;
+                    i = e - li * L;
+                }
+                slot[u] = li * Lp + i;
+                const int n = in_element(i, L, a.n_in, a.in_centered);
+                if (n >= 0) {
+                    cplx<R> x = a.in[b_in[li] + (int64_t)n * a.ain.stride];
+                    if (a.inmul) {
+                        const cplx<R> m = a.inmul[b_mul[li] + (int64_t)n * a.amul.stride];
+                        x = a.inmul_conj ? cmulc(m, x) : cmul(m, x);
+                    }
+                    if (a.pre_scale) x = cscale(x, a.pre_scale[n]);
+                    v[u] = x;
+                }
             }
-            if (a.pre_scale) v = cscale(v, a.pre_scale[n]);
         }
-        bufA[li * Lp + i] = v;
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; ++u)
+            if (slot[u] >= 0) bufA[slot[u]] = v[u];
     }
     __syncthreads();
 
@@ -312,10 +330,15 @@ RadixPlan make_radix_plan(int L)
     return p;
 }
 
+std::atomic<int> g_strided_lines{0};  // lines per workgroup of a strided pass, 0 = auto
+
 std::mutex g_tw_mutex;
 std::map<std::tuple<int, int, int>, void*> g_tw_cache;  // (device, L, sizeof(R)) -> table
 
 }  // namespace
+
+void fft_set_strided_lines(int v) { g_strided_lines = v; }
+int fft_get_strided_lines() { return g_strided_lines; }
 
 bool fft_size_supported(int L)
 {
@@ -371,18 +394,31 @@ template <typename R> void run_axis_fft(const AxisFft<R>& a, hipStream_t stream)
     const int target = sizeof(R) == 4 ? 2048 : 1024;
     int B = target / a.L;
     B = B < 1 ? 1 : (B > 64 ? 64 : B);
+    // Strided lines (adjacent lines adjacent in memory: a bundle of B lines reads B
+    // consecutive elements of each row) may take another bundle size. Wider bundles read
+    // whole 128-byte L2 lines, but at L = 256 they ran slower (66, 86 and 128 us per
+    // 256^3 pass at 8, 16 and 32 lines): fewer workgroups fit a CU's LDS, and the pass is
+    // latency-bound, not bound by the re-read halves of L2 lines.
+    const int want = g_strided_lines.load();
+    if (want > 0 && (a.ain.stride != 1 || a.aout.stride != 1)) B = want;
+    auto lds_bytes = [&](int b) { return sizeof(cplx<R>) * 2 * (size_t)b * (a.L + 1) + sizeof(int64_t) * 5 * b; };
+    while (B > 1 && lds_bytes(B) > 160 * 1024) B /= 2;
     args.lines_per_block = B;
-    const size_t smem = sizeof(cplx<R>) * 2 * (size_t)B * (a.L + 1) + sizeof(int64_t) * 5 * B;
+    const size_t smem = lds_bytes(B);
     const int64_t blocks = ceil_div(a.nlines, B);
     HASTY_REQUIRE(blocks < (1ll << 31), "too many FFT lines for one launch");
     const int mode = a.conv_real ? 1 : (a.conv_cplx ? 2 : 0);
     KernelTimer t("fft_axis", stream);
-    if (mode == 0)
+    if (mode == 0) {
+        allow_lds(axis_fft_kernel<R, 0>, smem);
         hipLaunchKernelGGL((axis_fft_kernel<R, 0>), dim3((unsigned)blocks), dim3(kFftThreads), smem, stream, args);
-    else if (mode == 1)
+    } else if (mode == 1) {
+        allow_lds(axis_fft_kernel<R, 1>, smem);
         hipLaunchKernelGGL((axis_fft_kernel<R, 1>), dim3((unsigned)blocks), dim3(kFftThreads), smem, stream, args);
-    else
+    } else {
+        allow_lds(axis_fft_kernel<R, 2>, smem);
         hipLaunchKernelGGL((axis_fft_kernel<R, 2>), dim3((unsigned)blocks), dim3(kFftThreads), smem, stream, args);
+    }
     check_launch("axis_fft_kernel");
 }
 

@@ -50,6 +50,10 @@ template <typename R> struct AxisFft {
 template <typename R> void run_axis_fft(const AxisFft<R>& a, hipStream_t stream);
 
 bool fft_size_supported(int L);
+// Lines per workgroup of a pass whose lines are strided (0 = auto, the same as a pass of
+// contiguous lines).
+void fft_set_strided_lines(int v);
+int fft_get_strided_lines();
 // Smallest even 2,3,5-smooth integer >= n.
 int64_t next235even(int64_t n);
 

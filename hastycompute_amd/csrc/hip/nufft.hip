@@ -36,7 +36,7 @@ namespace {
 inline int chunk_cap(bool fp64) { return fp64 ? 1024 : 128; }
 std::atomic<int> g_tile{0};   // 3-D tile edge with halo of new plans (0: by precision)
 std::atomic<int> g_chunk{0};  // points per subproblem of new plans (0: chunk_cap)
-std::atomic<int> g_spread_variant{3};  // 3-D spreader, values as nufft_set_spread_variant
+std::atomic<int> g_spread_variant{6};  // 3-D spreader, values as nufft_set_spread_variant
 // point ordering of new plans: bit 0 super-tile bin order, bit 1 XCD-aware block order;
 // 4 (default): 3 for type-1 plans (spreading), 0 for type-2 plans (interpolation)
 std::atomic<int> g_locality{4};
@@ -487,6 +487,501 @@ __global__ void __launch_bounds__(64 * NW) spread_rows_kernel(SpreadArgs<R> a)
     }
 }
 
+// 3-D spreading with the tile's planes dealt to the waves: wave w owns the tile planes
+// i0 with i0 mod NW == w and applies every point of a staged batch to the planes of the
+// point's window that it owns, lane r of a chunk of 64 updating cell (k, ib, q) of its
+// k-th owned plane. No two waves touch one cell and one wave's LDS operations execute in
+// order, so the read-modify-writes need no atomics while all NW waves accumulate at
+// once (the window-row form above leaves the accumulation to one wave).
+template <typename R, int NS, int NW>
+__global__ void __launch_bounds__(64 * NW) spread_planes_kernel(SpreadArgs<R> a)
+{
+    constexpr int S = 3 * NS + 2;                 // staged values per point
+    constexpr int KP = (NS + NW - 1) / NW;        // owned window planes per point, at most
+    constexpr int NR = (KP * NS * NS + 63) / 64;  // 64-lane chunks of a point's owned cells
+    __shared__ R stage[kRowBatch * S];
+    __shared__ int stage_off[kRowBatch];
+    __shared__ int stage_p0[kRowBatch];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<R>* tile = reinterpret_cast<cplx<R>*>(smem_raw);
+    const PtsArgs<R>& pa = a.pa;
+    const int64_t sp = xcd_block(a.xcd);
+    if (sp >= a.nsub) return;
+    const int bin = a.subbin[sp];
+    const int start = a.binstart[bin] + a.suboff[sp];
+    const int end = min(a.binstart[bin + 1], start + a.maxsub);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+
+    int64_t bb[3];
+    bin_coords(pa.ord, bin, bb);
+    int T[3];
+    int64_t o[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        T[d] = pa.bs[d] + NS + 1;
+        o[d] = bb[d] * pa.bs[d] - (NS + 1) / 2;
+    }
+    const int rp = a.rp, pp = a.pp;
+    for (int i = tid; i < T[0] * pp; i += 64 * NW) tile[i] = cplx<R>{(R)0, (R)0};
+
+    // this lane's cells of a point, chunk u: owned plane k, window row ib, column q
+    int ck[NR], cib[NR], cq[NR];
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+        const int r = u * 64 + lane;
+        ck[u] = r / (NS * NS);
+        const int rem = r - ck[u] * NS * NS;
+        cib[u] = rem / NS;
+        cq[u] = rem - cib[u] * NS;
+    }
+
+    for (int b0 = start; b0 < end; b0 += kRowBatch) {
+        const int np = min(kRowBatch, end - b0);
+        __syncthreads();
+        if (tid < np) {
+            const int j = a.sortidx[b0 + tid];
+            const cplx<R> cj = a.c[j];
+            R* st = stage + tid * S;
+            int off = 0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double t = fine_pos((double)pa.pts[d][j], pa.nf[d]);
+                const double l0 = ceil(t - 0.5 * NS);
+                const int rel = (int)((int64_t)l0 - o[d]);
+                off += rel * (d == 0 ? pp : (d == 1 ? rp : 1));
+                if (d == 0) stage_p0[tid] = rel;
+#pragma unroll
+                for (int q = 0; q < NS; ++q) st[2 + d * NS + q] = es_eval<R>((R)(l0 + q - t), a.beta, a.cfac);
+            }
+            stage_off[tid] = off;
+            st[0] = cj.x;
+            st[1] = cj.y;
+        }
+        __syncthreads();
+        for (int pt = 0; pt < np; ++pt) {
+            const R* st = stage + pt * S;
+            const int off = stage_off[pt];
+            // first window plane ia with (p0 + ia) mod NW == wave (p0 >= 0)
+            const int ia0 = (wave - stage_p0[pt] % NW + NW) % NW;
+            const R cx = st[0], cy = st[1];
+            cplx<R> v[NR];
+            int cell[NR];
+            R ww[NR];
+            bool on[NR];
+            // all loads of the point first (distinct cells), then all stores: the next
+            // point's loads follow them in program order, which one wave's LDS keeps
+#pragma unroll
+            for (int u = 0; u < NR; ++u) {
+                const int ia = ia0 + ck[u] * NW;
+                on[u] = ck[u] < KP && ia < NS;
+                if (on[u]) {
+                    cell[u] = off + ia * pp + cib[u] * rp + cq[u];
+                    ww[u] = st[2 + ia] * st[2 + NS + cib[u]] * st[2 + 2 * NS + cq[u]];
+                    v[u] = tile[cell[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < NR; ++u) {
+                if (on[u]) {
+                    v[u].x += ww[u] * cx;
+                    v[u].y += ww[u] * cy;
+                    tile[cell[u]] = v[u];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    const int cells = T[0] * T[1] * T[2];
+    const float r1 = 1.0f / (float)T[1], r2 = 1.0f / (float)T[2];
+    for (int i = tid; i < cells; i += 64 * NW) {
+        int i0, i1, i2;
+        split_cell(i, T[1], T[2], r1, r2, i0, i1, i2);
+        const cplx<R> v = tile[i0 * pp + i1 * rp + i2];
+        if (v.x == (R)0 && v.y == (R)0) continue;
+        const int64_t g0 = wrapi(o[0] + i0, pa.nf[0]);
+        const int64_t g1 = wrapi(o[1] + i1, pa.nf[1]);
+        const int64_t g2 = wrapi(o[2] + i2, pa.nf[2]);
+        cplx<R>* dst = a.fine + (g0 * pa.nf[1] + g1) * pa.nf[2] + g2;
+        atomicAdd(&dst->x, v.x);
+        atomicAdd(&dst->y, v.y);
+    }
+}
+
+// ---- output-driven 3-D spreading (complex64) ------------------------------------------
+//
+// The fine grid is cut into cores of kGC^3 cells (kGC divides every fine-grid dim). A
+// window of ns <= 16 cells overlaps one or two cores per dim, so each point belongs to at
+// most 8 cores; setpts lists, per core, the points whose windows overlap it (the pairs,
+// with the window corner relative to the core). The first launch gives every core one
+// workgroup that spreads up to kGatherPart of its points into an LDS copy of the core and
+// then stores the whole core, zeros included: each fine-grid cell is written once with
+// plain, whole-line stores, and neither a zero pass nor global atomics are needed. A
+// crowded core (the k-space centre of a radial trajectory) has its further points split
+// into parts of kGatherRest for a second launch, whose workgroups add their LDS cores to
+// the grid with atomics. Cores are listed most-crowded first, so the long ones start
+// first.
+constexpr int kGC = 16;
+constexpr int kGatherPart = 128;
+constexpr int kGatherRest = 128;
+constexpr int kGatherWaves = 4;
+
+// ES kernel for complex64 staging with the hardware exp2 and square root (about 1 ulp
+// each, far below the tolerances a complex64 plan is made for)
+__device__ __forceinline__ float es_eval_fast(float z, float beta, float c)
+{
+    const float t = 1.f - c * z * z;
+    return t > 0.f ? __builtin_amdgcn_exp2f(beta * 1.4426950408889634f * (__builtin_amdgcn_sqrtf(t) - 1.f)) : 0.f;
+}
+
+struct GatherBins {
+    const float* pts[3];
+    int64_t nf[3];
+    int nb[3];
+    int64_t M;
+    int ns;
+};
+
+__device__ __forceinline__ int gather_point_bin(const GatherBins& g, int64_t j, double* t)
+{
+    int b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        t[d] = fine_pos((double)g.pts[d][j], g.nf[d]);
+        int bi = (int)(t[d] * (1.0 / kGC));
+        b[d] = bi < g.nb[d] ? bi : g.nb[d] - 1;
+    }
+    return (b[0] * g.nb[1] + b[1]) * g.nb[2] + b[2];
+}
+
+// One atomic per distinct bin of a wave: points along a trajectory come in runs that share
+// a bin, and the k-space centre of a radial trajectory would otherwise take one atomic
+// per spoke on a single counter.
+__global__ void gather_count_kernel(GatherBins g, int* counts, int* rank)
+{
+    const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    double t[3];
+    const int bin = j < g.M ? gather_point_bin(g, j, t) : -1;
+    bool pending = j < g.M;
+    while (__ballot(pending)) {
+        const unsigned long long act = __ballot(pending);
+        const int leader = __ffsll((long long)act) - 1;
+        const int lb = __shfl(bin, leader);
+        const bool same = pending && bin == lb;
+        const unsigned long long m = __ballot(same);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counts[lb], __popcll(m));
+        base = __shfl(base, leader);
+        if (same) {
+            rank[j] = base + __popcll(m & ((1ull << lane) - 1ull));
+            pending = false;
+        }
+    }
+}
+
+// points in core-bin order: first window cell l0 and fine-grid position minus l0 per dim
+__global__ void gather_scatter_kernel(GatherBins g, const int* binstart, const int* rank, int* l0, float* frac,
+                                      int* idx)
+{
+    const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (j >= g.M) return;
+    double t[3];
+    const int64_t p = binstart[gather_point_bin(g, j, t)] + rank[j];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double l = ceil(t[d] - 0.5 * g.ns);
+        l0[d * g.M + p] = (int)l;
+        frac[d * g.M + p] = (float)(t[d] - l);
+    }
+    idx[p] = (int)j;
+}
+
+// (point, core) pairs of core ci with neighbour k of its 27 (periodic images included):
+// the points of that neighbouring core bin whose windows overlap the core, with the
+// window corner relative to the core packed 8 bits per dim (+16). One workgroup per
+// (core, neighbour), no atomics: COUNT writes the count to pcount[27 ci + k], else the
+// pairs go to pstart[27 ci + k] on in bin order (ballot compaction), so a core's pairs
+// are contiguous and their order is fixed by the core-bin order of the points.
+template <bool COUNT>
+__global__ void __launch_bounds__(kThreads) gather_pairs_kernel(GatherBins g, const int* binstart, const int* l0,
+                                                                 int* pcount, const int* pstart, int* pairs, int* prel)
+{
+    __shared__ int wave_cnt[kThreads / 64];
+    const int ci = blockIdx.x / 27, k = blockIdx.x % 27;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int cb[3] = {ci / (g.nb[2] * g.nb[1]), (ci / g.nb[2]) % g.nb[1], ci % g.nb[2]};
+    const int dd[3] = {k / 9 - 1, (k / 3) % 3 - 1, k % 3 - 1};
+    int bn[3], rel0[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int x = cb[d] + dd[d];
+        int sh = 0;
+        if (x < 0) {
+            x += g.nb[d];
+            sh = -(int)g.nf[d];
+        } else if (x >= g.nb[d]) {
+            x -= g.nb[d];
+            sh = (int)g.nf[d];
+        }
+        bn[d] = x;
+        rel0[d] = sh - cb[d] * kGC;  // window cell + this = cell relative to the core
+    }
+    const int bin = (bn[0] * g.nb[1] + bn[1]) * g.nb[2] + bn[2];
+    const int b0 = binstart[bin], V = binstart[bin + 1] - b0;
+    int out = COUNT ? 0 : pstart[blockIdx.x];
+    for (int v0 = 0; v0 < V; v0 += kThreads) {
+        const int v = v0 + tid;
+        bool acc = false;
+        int p = 0, rel = 0;
+        if (v < V) {
+            p = b0 + v;
+            acc = true;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int r = l0[d * g.M + p] + rel0[d];
+                acc = acc && r < kGC && r + g.ns > 0;
+                rel |= (r + 16) << (8 * d);
+            }
+        }
+        const unsigned long long m = __ballot(acc);
+        if (lane == 0) wave_cnt[wave] = __popcll(m);
+        __syncthreads();
+        int base = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; ++w) {
+            base += w < wave ? wave_cnt[w] : 0;
+            total += wave_cnt[w];
+        }
+        if (!COUNT && acc) {
+            const int e = out + base + __popcll(m & ((1ull << lane) - 1ull));
+            pairs[e] = p;
+            prel[e] = rel;
+        }
+        out += total;
+        __syncthreads();  // wave_cnt is rewritten by the next batch
+    }
+    if (COUNT && tid == 0) pcount[blockIdx.x] = out;
+}
+
+struct GatherArgs {
+    const float* frac;   // [3][M], core-bin order
+    const int* idx;      // [M]
+    const int* pstart;   // 27 cores + 1: core ci's pairs are [pstart[27 ci], pstart[27 ci + 27])
+    const int* pairs;    // sorted point of each pair
+    const int* prel;     // window corner relative to the core, 8 bits per dim (+16)
+    const int* list;     // (core, part) pairs of this launch
+    const float2* c;
+    float2* fine;
+    int64_t M;
+    int nb[3];
+    int64_t nf[3];
+    float beta, cfac;
+    int rp;    // LDS row pitch of the core (float2), gather_row_pitch
+    int rest;  // second launch: add non-zero cells with atomics
+};
+
+template <int NS>
+__global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(GatherArgs a)
+{
+    constexpr int NW = kGatherWaves;
+    constexpr int NT = 64 * NW;
+    constexpr int S = 3 * NS + 2;
+    constexpr int KP = (NS + NW - 1) / NW;   // owned window planes per point, at most
+    constexpr int NQ = (NS * NS + 63) / 64;  // 64-lane chunks of a plane's NS x NS cells
+    static_assert(kGatherPart <= NT, "one staging round per part");
+    static_assert((NW & (NW - 1)) == 0, "plane ownership by a mask");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int rp = a.rp;
+    float2* core = reinterpret_cast<float2*>(smem_raw);
+    float* stage = reinterpret_cast<float*>(core + kGC * kGC * rp);
+    int* stage_p = reinterpret_cast<int*>(stage + kGatherPart * S);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int ci = a.list[2 * blockIdx.x];
+    const int part = a.list[2 * blockIdx.x + 1];
+    // pairs of this part: the first part holds kGatherPart of the core's pairs, each
+    // further part kGatherRest (staged kGatherPart at a time)
+    const int pbase = a.pstart[27 * ci];
+    const int cnt = a.pstart[27 * ci + 27] - pbase;
+    const int lo = part == 0 ? 0 : kGatherPart + (part - 1) * kGatherRest;
+    const int hi = min(cnt, part == 0 ? kGatherPart : lo + kGatherRest);
+
+    for (int i = tid; i < kGC * kGC * rp; i += NT) core[i] = float2{0.f, 0.f};
+    // this lane's window rows and columns: chunk u holds (ib, q) = divmod(64 u + lane, NS)
+    int cib[NQ], cq[NQ];
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+        const int r = u * 64 + lane;
+        cib[u] = r / NS;
+        cq[u] = r - cib[u] * NS;
+    }
+
+    for (int r0 = lo; r0 < hi; r0 += kGatherPart) {
+        const int np = min(kGatherPart, hi - r0);
+        __syncthreads();  // the previous round is accumulated (and the core cleared)
+        if (tid < np) {
+            const int p = a.pairs[pbase + r0 + tid];
+            const int rel = a.prel[pbase + r0 + tid];
+            const float2 cj = a.c[a.idx[p]];
+            float* st = stage + tid * S;
+            st[0] = cj.x;
+            st[1] = cj.y;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const float fr = a.frac[d * a.M + p];
+#pragma unroll
+                for (int q = 0; q < NS; ++q) st[2 + d * NS + q] = es_eval_fast((float)q - fr, a.beta, a.cfac);
+                stage_p[3 * tid + d] = ((rel >> (8 * d)) & 255) - 16;
+            }
+        }
+        __syncthreads();
+        // Wave w owns the core planes i0 with i0 mod NW == w. Per point: each lane's
+        // row-column cell offset and weight (strength folded in), then per owned plane
+        // (wave-uniform values in scalar registers) the loads of its cells; while they
+        // are in flight, the next point's row-column part; then the adds and stores. One
+        // wave's LDS operations execute in order, so the next point's loads see these
+        // stores.
+        int off[NQ];
+        float2 wc[NQ];
+        int p0 = 0;
+        auto prep = [&](int e, int* o, float2* w, int& q0) {
+            const float* st = stage + e * S;
+            const int p1 = stage_p[3 * e + 1];
+            const int p2 = stage_p[3 * e + 2];
+            q0 = stage_p[3 * e];
+            const float cx = st[0], cy = st[1];
+#pragma unroll
+            for (int u = 0; u < NQ; ++u) {
+                const int i1 = p1 + cib[u], i2 = p2 + cq[u];
+                const bool in = u * 64 + lane < NS * NS && (unsigned)i1 < (unsigned)kGC && (unsigned)i2 < (unsigned)kGC;
+                o[u] = in ? i1 * rp + i2 : -1;
+                const float w12 = in ? st[2 + NS + cib[u]] * st[2 + 2 * NS + cq[u]] : 0.f;
+                w[u] = float2{w12 * cx, w12 * cy};
+            }
+        };
+        if (np > 0) prep(0, off, wc, p0);
+        for (int e = 0; e < np; ++e) {
+            const float* st = stage + e * S;
+            const int ia0 = (wave - p0) & (NW - 1);
+            float2 val[KP][NQ];
+            int cell0[KP];
+            int w0[KP];  // bits of the plane weight, in a scalar register
+            bool pl[KP];
+#pragma unroll
+            for (int k = 0; k < KP; ++k) {
+                const int ia = ia0 + k * NW;
+                const int i0 = p0 + ia;
+                pl[k] = ia < NS && (unsigned)i0 < (unsigned)kGC;  // wave-uniform
+                cell0[k] = i0 * kGC * rp;
+                w0[k] = pl[k] ? __float_as_int(st[2 + ia]) : 0;
+                if (pl[k]) {
+#pragma unroll
+                    for (int u = 0; u < NQ; ++u)
+                        if (off[u] >= 0) val[k][u] = core[cell0[k] + off[u]];
+                }
+            }
+            int noff[NQ];
+            float2 nwc[NQ];
+            int np0 = 0;
+            if (e + 1 < np) {
+                prep(e + 1, noff, nwc, np0);
+            } else {
+#pragma unroll
+                for (int u = 0; u < NQ; ++u) {
+                    noff[u] = -1;
+                    nwc[u] = float2{0.f, 0.f};
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < KP; ++k) {
+                if (pl[k]) {
+                    const float wa = __int_as_float(w0[k]);
+#pragma unroll
+                    for (int u = 0; u < NQ; ++u) {
+                        if (off[u] >= 0) {
+                            val[k][u].x += wa * wc[u].x;
+                            val[k][u].y += wa * wc[u].y;
+                            core[cell0[k] + off[u]] = val[k][u];
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < NQ; ++u) {
+                off[u] = noff[u];
+                wc[u] = nwc[u];
+            }
+            p0 = np0;
+        }
+    }
+    __syncthreads();
+
+    int b[3];
+    b[2] = ci % a.nb[2];
+    b[1] = (ci / a.nb[2]) % a.nb[1];
+    b[0] = ci / (a.nb[2] * a.nb[1]);
+    const int64_t o0 = (int64_t)b[0] * kGC, o1 = (int64_t)b[1] * kGC, o2 = (int64_t)b[2] * kGC;
+    for (int i = tid; i < kGC * kGC * kGC; i += NT) {
+        const int i0 = i / (kGC * kGC), i1 = (i / kGC) % kGC, i2 = i % kGC;
+        const float2 v = core[(i0 * kGC + i1) * rp + i2];
+        float2* dst = a.fine + ((o0 + i0) * a.nf[1] + (o1 + i1)) * a.nf[2] + (o2 + i2);
+        if (!a.rest) {
+            *dst = v;
+        } else if (v.x != 0.f || v.y != 0.f) {
+            atomicAdd(&dst->x, v.x);
+            atomicAdd(&dst->y, v.y);
+        }
+    }
+}
+
+// LDS row pitch of a core (float2): lane (ib, q) of a window plane starts at bank pair
+// (3 ib + q) mod 16, at most 4 lanes per pair for ns <= 9, and the core with its stage
+// stays under 53 KB (three workgroups per CU) at ns = 7; a window of 16 columns covers
+// the banks by itself
+inline int gather_row_pitch(int ns) { return ns < kGC ? kGC + 3 : kGC; }
+inline size_t gather_lds_bytes(int ns)
+{
+    return sizeof(float2) * kGC * kGC * gather_row_pitch(ns) + sizeof(float) * kGatherPart * (3 * ns + 2) +
+           sizeof(int) * kGatherPart * 3;
+}
+
+template <int NS> void launch_gather_ns(const GatherArgs& a, int64_t blocks, hipStream_t s)
+{
+    const size_t smem = gather_lds_bytes(NS);
+    allow_lds(gather_spread_kernel<NS>, smem);
+    hipLaunchKernelGGL((gather_spread_kernel<NS>), dim3((unsigned)blocks), dim3(64 * kGatherWaves), smem, s, a);
+}
+
+void launch_gather(int ns, const GatherArgs& a, int64_t blocks, hipStream_t s)
+{
+    switch (ns) {
+#define HASTY_GATHER_CASE(K) \
+    case K: launch_gather_ns<K>(a, blocks, s); break;
+        HASTY_GATHER_CASE(2)
+        HASTY_GATHER_CASE(3)
+        HASTY_GATHER_CASE(4)
+        HASTY_GATHER_CASE(5)
+        HASTY_GATHER_CASE(6)
+        HASTY_GATHER_CASE(7)
+        HASTY_GATHER_CASE(8)
+        HASTY_GATHER_CASE(9)
+        HASTY_GATHER_CASE(10)
+        HASTY_GATHER_CASE(11)
+        HASTY_GATHER_CASE(12)
+        HASTY_GATHER_CASE(13)
+        HASTY_GATHER_CASE(14)
+        HASTY_GATHER_CASE(15)
+        HASTY_GATHER_CASE(16)
+#undef HASTY_GATHER_CASE
+    default: throw HastyError(kUnsupported, "kernel width out of range");
+    }
+    check_launch("gather_spread_kernel");
+}
+
 template <typename R, int NS, int DIM>
 __global__ void __launch_bounds__(kThreads) interp_kernel(SpreadArgs<R> a)
 {
@@ -583,6 +1078,20 @@ void launch_spread_ns(const SpreadArgs<R>& a, int64_t nsub, size_t smem, hipStre
                                s, a);
             return;
         }
+        if (variant == 4) {
+            allow_lds(spread_planes_kernel<R, NS, 4>, smem);
+            KernelTimer t("nufft_spread", s);
+            hipLaunchKernelGGL((spread_planes_kernel<R, NS, 4>), dim3((unsigned)xcd_grid(nsub, a.xcd)), dim3(256), smem, s,
+                               a);
+            return;
+        }
+        if (variant == 5) {
+            allow_lds(spread_planes_kernel<R, NS, 8>, smem);
+            KernelTimer t("nufft_spread", s);
+            hipLaunchKernelGGL((spread_planes_kernel<R, NS, 8>), dim3((unsigned)xcd_grid(nsub, a.xcd)), dim3(512), smem, s,
+                               a);
+            return;
+        }
     }
     allow_lds(spread_kernel<R, NS, DIM>, smem);
     KernelTimer t("nufft_spread", s);
@@ -627,8 +1136,24 @@ void dispatch_ns(bool spread, int ns, const SpreadArgs<R>& a, int64_t n, size_t 
     check_launch(spread ? "spread_kernel" : "interp_kernel");
 }
 
-// the plan's 3-D spreader (nufft_autotune) or else the library option
-int spread_variant(const NufftPlan* p) { return p->spread_variant >= 0 ? p->spread_variant : g_spread_variant.load(); }
+// plans the output-driven spreader (variant 6) serves: complex64 type 1 in 3-D with every
+// fine-grid dim a multiple of the core edge and at least three cores long (so the 27
+// neighbours of a core are distinct cores)
+bool gather_fits(const NufftPlan* p)
+{
+    if (p->fp64 || p->type != 1 || p->dim != 3) return false;
+    for (int d = 0; d < 3; ++d)
+        if (p->nf[d] % kGC != 0 || p->nf[d] < 3 * kGC) return false;
+    return true;
+}
+
+// the plan's 3-D spreader (nufft_autotune) or else the library option; variant 6 on a
+// plan it does not serve runs variant 3
+int spread_variant(const NufftPlan* p)
+{
+    const int v = p->spread_variant >= 0 ? p->spread_variant : g_spread_variant.load();
+    return v == 6 && !gather_fits(p) ? 3 : v;
+}
 
 template <typename R> void dispatch_dim(bool spread, const NufftPlan* p, const SpreadArgs<R>& a, int64_t n, size_t smem,
                                         hipStream_t s)
@@ -830,6 +1355,17 @@ void nufft_destroy(NufftPlan* p)
     dev_free(p->sortidx);
     dev_free(p->subbin);
     dev_free(p->suboff);
+    dev_free(p->g_counts);
+    dev_free(p->g_binstart);
+    dev_free(p->g_l0);
+    dev_free(p->g_frac);
+    dev_free(p->g_idx);
+    dev_free(p->g_rank);
+    dev_free(p->g_pcount);
+    dev_free(p->g_pstart);
+    dev_free(p->g_pairs);
+    dev_free(p->g_prel);
+    dev_free(p->g_list);
     delete p;
 }
 
@@ -907,6 +1443,144 @@ template <typename R> SpreadArgs<R> spread_args(const NufftPlan* p)
 }
 
 template <typename R> const R* phi(const NufftPlan* p, int d) { return reinterpret_cast<const R*>(p->phiinv[d]); }
+
+// Core bins, (point, core) pairs and the two launch lists of the output-driven spreader,
+// for the current points. The pair counts come back to the host (one synchronisation)
+// to size the pair list, place each (core, neighbour) run, order the cores and split
+// the crowded ones into parts.
+void gather_build(NufftPlan* p, hipStream_t s)
+{
+    GatherBins g;
+    int nb[3];
+    for (int d = 0; d < 3; ++d) {
+        g.pts[d] = reinterpret_cast<const float*>(p->pts[d]);
+        g.nf[d] = p->nf[d];
+        nb[d] = g.nb[d] = (int)(p->nf[d] / kGC);
+    }
+    g.M = p->M;
+    g.ns = p->ns;
+    const int64_t ncore = (int64_t)nb[0] * nb[1] * nb[2];
+    if (!p->g_counts) {
+        p->g_counts = (int*)dev_alloc(sizeof(int) * ncore);
+        p->g_binstart = (int*)dev_alloc(sizeof(int) * (ncore + 1));
+        p->g_pcount = (int*)dev_alloc(sizeof(int) * 27 * ncore);
+        p->g_pstart = (int*)dev_alloc(sizeof(int) * (27 * ncore + 1));
+    }
+    if (p->M > p->g_cap_M) {
+        dev_free(p->g_l0);
+        dev_free(p->g_frac);
+        dev_free(p->g_idx);
+        dev_free(p->g_rank);
+        p->g_l0 = nullptr;
+        p->g_frac = nullptr;
+        p->g_idx = p->g_rank = nullptr;
+        p->g_l0 = (int*)dev_alloc(sizeof(int) * 3 * p->M);
+        p->g_frac = (float*)dev_alloc(sizeof(float) * 3 * p->M);
+        p->g_idx = (int*)dev_alloc(sizeof(int) * p->M);
+        p->g_rank = (int*)dev_alloc(sizeof(int) * p->M);
+        p->g_cap_M = p->M;
+    }
+    const unsigned pblocks = (unsigned)ceil_div(std::max<int64_t>(p->M, 1), kThreads);
+    launch_fill(p->g_counts, ncore, 0, s);
+    if (p->M > 0) {
+        hipLaunchKernelGGL(gather_count_kernel, dim3(pblocks), dim3(kThreads), 0, s, g, p->g_counts, p->g_rank);
+        check_launch("gather_count_kernel");
+    }
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, (const int*)p->g_counts, p->g_binstart, ncore);
+    check_launch("scan_kernel");
+    if (p->M > 0) {
+        hipLaunchKernelGGL(gather_scatter_kernel, dim3(pblocks), dim3(kThreads), 0, s, g, (const int*)p->g_binstart,
+                           (const int*)p->g_rank, p->g_l0, p->g_frac, p->g_idx);
+        check_launch("gather_scatter_kernel");
+    }
+    hipLaunchKernelGGL((gather_pairs_kernel<true>), dim3((unsigned)(27 * ncore)), dim3(kThreads), 0, s, g,
+                       (const int*)p->g_binstart, (const int*)p->g_l0, p->g_pcount, (const int*)nullptr, (int*)nullptr,
+                       (int*)nullptr);
+    check_launch("gather_pairs_kernel");
+    std::vector<int> cnt27((size_t)(27 * ncore));
+    HASTY_HIP_CHECK(hipMemcpyAsync(cnt27.data(), p->g_pcount, sizeof(int) * 27 * ncore, hipMemcpyDeviceToHost, s));
+    HASTY_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<int> cnt((size_t)ncore, 0);
+    int64_t npairs = 0;
+    for (int64_t i = 0; i < 27 * ncore; ++i) {
+        cnt[(size_t)(i / 27)] += cnt27[(size_t)i];
+        npairs += cnt27[(size_t)i];
+    }
+    HASTY_REQUIRE(npairs < (int64_t)1 << 31, "too many points for the output-driven spreader");
+    if (npairs > p->g_cap_pairs) {
+        dev_free(p->g_pairs);
+        dev_free(p->g_prel);
+        p->g_pairs = p->g_prel = nullptr;
+        p->g_pairs = (int*)dev_alloc(sizeof(int) * npairs);
+        p->g_prel = (int*)dev_alloc(sizeof(int) * npairs);
+        p->g_cap_pairs = npairs;
+    }
+    std::vector<int> start27((size_t)(27 * ncore + 1));
+    start27[0] = 0;
+    for (int64_t i = 0; i < 27 * ncore; ++i) start27[(size_t)i + 1] = start27[(size_t)i] + cnt27[(size_t)i];
+    HASTY_HIP_CHECK(hipMemcpyAsync(p->g_pstart, start27.data(), sizeof(int) * start27.size(), hipMemcpyHostToDevice, s));
+    if (npairs > 0) {
+        hipLaunchKernelGGL((gather_pairs_kernel<false>), dim3((unsigned)(27 * ncore)), dim3(kThreads), 0, s, g,
+                           (const int*)p->g_binstart, (const int*)p->g_l0, (int*)nullptr, (const int*)p->g_pstart,
+                           p->g_pairs, p->g_prel);
+        check_launch("gather_pairs_kernel");
+    }
+    std::vector<int> order((size_t)ncore);
+    for (int64_t i = 0; i < ncore; ++i) order[(size_t)i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cnt[x] > cnt[y]; });
+    std::vector<int> list;
+    list.reserve((size_t)ncore * 2 + 64);
+    for (int ci : order) {
+        list.push_back(ci);
+        list.push_back(0);
+    }
+    for (int ci : order)
+        for (int k = 1; k <= (int)ceil_div(std::max(0, cnt[ci] - kGatherPart), kGatherRest); ++k) {
+            list.push_back(ci);
+            list.push_back(k);
+        }
+    const int64_t entries = (int64_t)list.size() / 2;
+    if (entries > p->g_cap_list) {
+        dev_free(p->g_list);
+        p->g_list = nullptr;
+        p->g_list = (int*)dev_alloc(sizeof(int) * list.size());
+        p->g_cap_list = entries;
+    }
+    HASTY_HIP_CHECK(hipMemcpyAsync(p->g_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, s));
+    HASTY_HIP_CHECK(hipStreamSynchronize(s));  // the host list goes out of scope
+    p->g_first = ncore;
+    p->g_rest = entries - ncore;
+    p->gather_built = 1;
+}
+
+void gather_spread(NufftPlan* p, const float2* c, float2* fine, hipStream_t s)
+{
+    GatherArgs a;
+    a.frac = p->g_frac;
+    a.idx = p->g_idx;
+    a.pstart = p->g_pstart;
+    a.pairs = p->g_pairs;
+    a.prel = p->g_prel;
+    a.list = p->g_list;
+    a.c = c;
+    a.fine = fine;
+    a.M = p->M;
+    for (int d = 0; d < 3; ++d) {
+        a.nb[d] = (int)(p->nf[d] / kGC);
+        a.nf[d] = p->nf[d];
+    }
+    a.beta = (float)p->beta;
+    a.cfac = (float)(4.0 / ((double)p->ns * p->ns));
+    a.rp = gather_row_pitch(p->ns);
+    a.rest = 0;
+    KernelTimer t("nufft_spread", s);
+    launch_gather(p->ns, a, p->g_first, s);
+    if (p->g_rest > 0) {
+        a.list = p->g_list + 2 * p->g_first;
+        a.rest = 1;
+        launch_gather(p->ns, a, p->g_rest, s);
+    }
+}
 
 template <typename R> void execute_type2(NufftPlan* p, cplx<R>* c, const cplx<R>* f, hipStream_t s)
 {
@@ -986,14 +1660,25 @@ template <typename R> void execute_type1(NufftPlan* p, const cplx<R>* c, cplx<R>
     tile_pitches(T, p->ns, (int)sizeof(cplx<R>), pad_tile(p), rp, pp);
     const size_t smem = sizeof(cplx<R>) * (size_t)T[0] * pp;
     HASTY_REQUIRE(smem <= 160 * 1024, "spreading tile exceeds LDS");
+    const bool gather = std::is_same<R, float>::value && spread_variant(p) == 6;
+    if (gather && !p->gather_built) {
+        HASTY_REQUIRE(!stream_capturing(s),
+                      "the output-driven spreader bins the points on its first execute; run one before capturing");
+        gather_build(p, s);
+    }
     for (int t = 0; t < p->ntransf; ++t) {
         const cplx<R>* ct = c + t * p->M;
         cplx<R>* ft = f + t * nvox;
-        zero_fill<R>(fine, f0 * f1 * f2, s);
-        if (p->nsub > 0) {
-            SpreadArgs<R> sa = spread_args<R>(p);
-            sa.c = ct;
-            dispatch_dim<R>(true, p, sa, p->nsub, smem, s);
+        if (gather) {
+            // writes every fine-grid cell: no zero pass
+            gather_spread(p, reinterpret_cast<const float2*>(ct), reinterpret_cast<float2*>(fine), s);
+        } else {
+            zero_fill<R>(fine, f0 * f1 * f2, s);
+            if (p->nsub > 0) {
+                SpreadArgs<R> sa = spread_args<R>(p);
+                sa.c = ct;
+                dispatch_dim<R>(true, p, sa, p->nsub, smem, s);
+            }
         }
         if (p->dim == 3) {  // dim 0 in place, crop to n0
             AxisFft<R> a;
@@ -1081,6 +1766,7 @@ void nufft_setpts(NufftPlan* p, int64_t M, const void* x, const void* y, const v
         p->pts[2 - d] = xyz[d];
     }
     p->M = M;
+    p->gather_built = 0;
     const int64_t nb = bin_range(p->nbins, (p->locality & 1) ? kSuper : 1);
     const int64_t need = std::max<int64_t>(M, nb);
     if (need > p->cap_M) {
@@ -1096,7 +1782,10 @@ void nufft_setpts(NufftPlan* p, int64_t M, const void* x, const void* y, const v
         setpts_typed<double>(p, s);
     else
         setpts_typed<float>(p, s);
+    if (spread_variant(p) == 6) gather_build(p, s);
 }
+
+bool nufft_gather_fits(const NufftPlan* p) { return gather_fits(p); }
 
 void nufft_execute(NufftPlan* p, void* c, void* f, hipStream_t s)
 {

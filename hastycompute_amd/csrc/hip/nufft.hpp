@@ -41,6 +41,25 @@ struct NufftPlan {
     int spread_variant = -1;  // 3-D spreader of this plan (-1: the library option)
     int locality = 3;         // bit 0 super-tile bin order, bit 1 XCD-aware blocks (fixed at makeplan)
     int chunk = 256;          // points per spreader subproblem at most (fixed at makeplan)
+    // output-driven 3-D spreader (variant 6, complex64): points binned by 16^3-cell cores,
+    // built on the first execute after setpts
+    int gather_built = 0;
+    int* g_counts = nullptr;    // points per core bin
+    int* g_binstart = nullptr;  // core bins + 1 (scan)
+    int* g_l0 = nullptr;        // [3][M] first window cell per dim, in core-bin order
+    float* g_frac = nullptr;    // [3][M] fine-grid position minus that cell
+    int* g_idx = nullptr;       // [M] point index, in core-bin order
+    int* g_rank = nullptr;      // [M] rank within the core bin
+    int* g_pcount = nullptr;    // (point, core) pairs per (core, neighbour bin): 27 per core
+    int* g_pstart = nullptr;    // 27 cores + 1 (scan)
+    int* g_pairs = nullptr;     // sorted point of each pair, grouped by core
+    int* g_prel = nullptr;      // window corner relative to the core
+    int* g_list = nullptr;      // (core, part) pairs: every core's first part, then the rest
+    int64_t g_cap_M = 0;
+    int64_t g_cap_pairs = 0;
+    int64_t g_cap_list = 0;
+    int64_t g_first = 0;        // list entries of the first launch (one per core)
+    int64_t g_rest = 0;         // list entries of the second launch
 };
 
 // nmodes in FINUFFT order (fastest-varying first), as the reference passes
@@ -53,9 +72,15 @@ void nufft_setpts(NufftPlan* p, int64_t M, const void* x, const void* y, const v
 void nufft_execute(NufftPlan* p, void* c, void* f, hipStream_t s);
 void nufft_destroy(NufftPlan* p);
 // 3-D spreader: 0 one lane per point with LDS atomics; window rows with
-// 1 one wave per tile, 2 four waves and LDS atomics, 3 (default) four waves clearing and
-// flushing the tile while the first wave accumulates without atomics
+// 1 one wave per tile, 2 four waves and LDS atomics, 3 four waves clearing and
+// flushing the tile while the first wave accumulates without atomics; 4 and 5 four and
+// eight waves each owning the tile planes congruent to its index; 6 (default) output-driven:
+// one workgroup per 16^3-cell core gathers the points of the 27 neighbouring cores and
+// writes the core once (complex64 plans whose fine-grid dims are multiples of 16 and at
+// least 48; other plans take 3)
 void nufft_set_spread_variant(int v);
+// whether the output-driven spreader (variant 6) serves the plan
+bool nufft_gather_fits(const NufftPlan* p);
 int nufft_get_spread_variant();
 // Point ordering of new plans: bit 0 groups bins into 4^3 super-tiles, bit 1 maps the
 // spreading and interpolation workgroups to XCDs in contiguous runs; 4 (default): 3 for

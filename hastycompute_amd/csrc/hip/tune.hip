@@ -237,8 +237,8 @@ int nufft_autotune(NufftPlan* p, void* c, void* f, hipStream_t s, double* ms)
     double tbest = 1e300;
     try {
         std::vector<int> order = {best};
-        for (int v : {3, 1, 2, 0})
-            if (v != best) order.push_back(v);
+        for (int v : {6, 3, 5, 4, 1, 2, 0})
+            if (v != best && (v != 6 || nufft_gather_fits(p))) order.push_back(v);
         for (int v : order) {
             p->spread_variant = v;
             const double t = time_best([&] { nufft_execute(p, c, f, s); }, s, 3);
@@ -252,9 +252,10 @@ int nufft_autotune(NufftPlan* p, void* c, void* f, hipStream_t s, double* ms)
         throw;
     }
     p->spread_variant = best;
-    // spreading tile (edge with halo) and points per workgroup, complex64 plans: the
-    // points are re-binned for each candidate; the current shape is timed first
-    if (!p->fp64) {
+    // spreading tile (edge with halo) and points per workgroup of the tile spreaders,
+    // complex64 plans: the points are re-binned for each candidate; the current shape is
+    // timed first (the output-driven spreader has fixed cores)
+    if (!p->fp64 && best != 6) {
         const int t0 = nufft_tile_edge(p), c0 = p->chunk;
         int bt = t0, bc = c0;
         const std::pair<int, int> cands[] = {{20, 128}, {16, 256}, {20, 256}, {16, 128}, {20, 64}};

@@ -141,10 +141,19 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
  *     cells, halo included, of new plans (the tile must fit in LDS: 20 for complex128).
  *   "nufft_spread_chunk" 0 or 32..8192 (default 0 = 128 for complex64, 1024 for
  *     complex128): points per spreading workgroup of new plans.
- *   "nufft_spread_variant" 0..3 (default 3): 3-D type-1 spreader; 0 one lane per
+ *   "nufft_spread_variant" 0..6 (default 6): 3-D type-1 spreader; 0 one lane per
  *     point with LDS atomics, 1 window rows with one wave per tile, 2 window rows with four
  *     waves and LDS atomics, 3 four waves clearing and flushing the tile while one wave
- *     accumulates without atomics.
+ *     accumulates without atomics, 4 and 5 four and eight waves each owning the tile
+ *     planes congruent to its index, 6 output-driven: one workgroup per 16^3-cell core
+ *     of the fine grid spreads the points whose windows overlap it and stores the core
+ *     once (no zero pass, no global atomics outside crowded cores; complex64 plans whose
+ *     fine-grid dims are multiples of 16 and at least 48, binned at setpts; other plans
+ *     run 3).
+ *   "fft_strided_lines" 0..64 (default 0 = auto): lines per workgroup of the axis FFT
+ *     passes whose lines are strided (the NUFFT's outer axes, the Toeplitz kernel build,
+ *     the generic normal operator); auto gives every pass 2048 / L (complex64) or
+ *     1024 / L (complex128) lines, 1..64.
  * Returns 0, or 1 for an unknown name or a value out of range. */
 int hasty_set_option(const char* name, int64_t value);
 /* Current value of an option of hasty_set_option (0, or 1 for an unknown name). */

@@ -24,7 +24,7 @@ def restore_options():
     from hastycompute_amd import _capi as C
 
     names = ("sense_xy_variant", "sense_zconv_batch", "sense_zconv_variant", "nufft_spread_variant",
-             "nufft_spread_tile", "nufft_spread_chunk", "llr_eig_threads")
+             "nufft_spread_tile", "nufft_spread_chunk", "llr_eig_threads", "fft_strided_lines")
     saved = {k: C.get_option(k) for k in names}
     yield
     for k, v in saved.items():
@@ -70,7 +70,7 @@ def test_autotune_nufft_type1(N, restore_options):
     f = torch.empty_like(f0)
     t = autotune.nufft(plan, c, f)
     print(f"autotune nufft: {t}", flush=True)
-    assert t["nufft_spread_variant"] in (0, 1, 2, 3) and t["execute_ms"] > 0
+    assert t["nufft_spread_variant"] in range(7) and t["execute_ms"] > 0
     assert t["nufft_spread_tile"] in (16, 20) and t["nufft_spread_chunk"] in (64, 128, 256)
     plan.execute(c, f)
     torch.cuda.synchronize()
@@ -106,6 +106,7 @@ def test_nufft_spread_tile_options_vs_direct_sum(N, restore_options, tile, chunk
     M = co.shape[1]
     rng = np.random.default_rng(tile + chunk)
     c = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
+    C.set_option("nufft_spread_variant", 3)  # a tile spreader (the output-driven one has fixed cores)
     C.set_option("nufft_spread_tile", tile)
     C.set_option("nufft_spread_chunk", chunk)
     plan = N.NufftPlan(torch.as_tensor(co.astype(np.float32)).cuda(), [n, n, n], 1, 1, 1, 1e-6)
@@ -117,6 +118,52 @@ def test_nufft_spread_tile_options_vs_direct_sum(N, restore_options, tile, chunk
     e = O.rel_l2(f.cpu().numpy()[0].astype(np.complex128), ref)
     print(f"tile {tile} chunk {chunk}: {e:.2e}", flush=True)
     assert e < 1e-5
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_fft_strided_lines_same_bits(N, restore_options, fp64):
+    # the lines per workgroup of the strided axis FFT passes (1, 8, auto = 8 or 4, 64,
+    # which the LDS limit halves) only regroup independent lines: type-2 NUFFTs (FFT
+    # passes, then a gather) give the same bits for every value; type 1 and the FP64
+    # Toeplitz kernel build, whose tile flushes add with global atomics in no fixed
+    # order, agree to rounding; type 1 matches the direct sum
+    from hastycompute_amd import _capi as C
+
+    n = 24
+    rdt, cdt = (torch.float64, torch.complex128) if fp64 else (torch.float32, torch.complex64)
+    co = O.kooshball(120, 48)
+    M = co.shape[1]
+    rng = np.random.default_rng(7)
+    c = torch.as_tensor((rng.standard_normal((1, M)) + 1j * rng.standard_normal((1, M)))).to(cdt).cuda()
+    img = torch.as_tensor((rng.standard_normal((1, n, n, n)) + 1j * rng.standard_normal((1, n, n, n)))).to(cdt).cuda()
+    cot = torch.as_tensor(co).to(rdt).cuda()
+    tol = 1e-10 if fp64 else 1e-6
+    outs = []
+    for lines in (1, 8, 0, 64):
+        C.set_option("fft_strided_lines", lines)
+        p1 = N.NufftPlan(cot, [n, n, n], 1, 1, 1, tol)
+        f = torch.empty((1, n, n, n), dtype=cdt, device="cuda")
+        p1.execute(c, f)
+        p2 = N.NufftPlan(cot, [n, n, n], 2, -1, 1, tol)
+        d = torch.empty((1, M), dtype=cdt, device="cuda")
+        p2.execute(d, img)
+        torch.cuda.synchronize()
+        p1.close()
+        p2.close()
+        _, k = N.toeplitz_build(cot.float(), [n, n, n], diag=False)
+        outs.append((f.clone(), d.clone(), k.clone()))
+    rt = 1e-13 if fp64 else 1e-6
+    for lines, o in zip((8, 0, 64), outs[1:]):
+        assert torch.equal(outs[0][1], o[1])
+        e1 = O.rel_l2(o[0].cpu().numpy(), outs[0][0].cpu().numpy())
+        ek = O.rel_l2(o[2].cpu().numpy(), outs[0][2].cpu().numpy())
+        print(f"fp64={fp64} lines {lines}: type 1 {e1:.1e}, kernel {ek:.1e}", flush=True)
+        assert e1 < rt and ek < 1e-6
+    ref = O.nudft_type1(co.astype(np.float64) if fp64 else co.astype(np.float32).astype(np.float64),
+                        c.cpu().numpy()[0].astype(np.complex128), [n, n, n], 1)
+    e = O.rel_l2(outs[0][0].cpu().numpy()[0].astype(np.complex128), ref)
+    print(f"fp64={fp64}: type 1 vs direct sum {e:.2e}", flush=True)
+    assert e < (1e-9 if fp64 else 1e-5)
 
 
 @pytest.mark.parametrize("threads", [256, 512, 768])

@@ -118,7 +118,7 @@ def test_nufft_cfg1_sizes_vs_oracle(N):
     assert e2 < 1e-5 and e1 < 1e-5
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("traj", ["kooshball", "uniform", "pileup"])
 @pytest.mark.parametrize("tol,bound", [(1e-6, 1e-5), (1e-3, 1e-2)])
 def test_nufft_type1_spread_variants(N, variant, traj, tol, bound):
@@ -141,21 +141,23 @@ def test_nufft_type1_spread_variants(N, variant, traj, tol, bound):
     co32 = co.astype(np.float32).astype(np.float64)
     y = rnd(rng, co.shape[1])
     print(f"spread variant {variant} {traj} tol {tol}", flush=True)
-    _capi.call("hasty_set_option", b"nufft_spread_variant", variant)
+    prev = _capi.get_option("nufft_spread_variant")
+    _capi.set_option("nufft_spread_variant", variant)
     try:
         img = h(N.nufft(g(co, torch.float32), g(y[None]), [n, n, n], 1, +1, tol))[0]
     finally:
-        _capi.call("hasty_set_option", b"nufft_spread_variant", 3)
+        _capi.set_option("nufft_spread_variant", prev)
     e = O.rel_l2(img, O.nufft_type1(co32, y, (n, n, n), +1, 1e-12))
     print(f"  rel err {e:.2e}", flush=True)
     assert e < bound
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("fp64,tol,bound", [(True, 1e-10, 1e-8), (False, 1e-8, 1e-5)])
 def test_nufft_type1_spread_variants_fp64_and_wide(N, variant, fp64, tol, bound):
     # The spreaders with complex128 data (ns = 11: 121 window rows, two per lane in the
-    # window-row spreaders) and complex64 with a kernel wider than 8 (tol 1e-8: ns = 9),
+    # window-row spreaders, 363 or 242 owned cells per wave and point in the plane-dealt
+    # ones) and complex64 with a kernel wider than 8 (tol 1e-8: ns = 9),
     # on a kooshball with every spoke's k = 0 sample at the origin, against the direct sum.
     from hastycompute_amd import _capi
 
@@ -179,6 +181,52 @@ def test_nufft_type1_spread_variants_fp64_and_wide(N, variant, fp64, tol, bound)
     e = O.rel_l2(img, O.nudft_type1(co_used, y, (n, n, n), +1))
     print(f"spread variant {variant} {'fp64' if fp64 else 'fp32'} tol {tol} (ns {ns}): {e:.2e}", flush=True)
     assert ns >= 9 and e < bound
+
+
+@pytest.mark.parametrize("n,traj,tol,bound", [(24, "kooshball", 1e-6, 1e-5), (24, "pileup", 1e-6, 1e-5),
+                                              (32, "uniform", 1e-6, 1e-5), (40, "kooshball", 1e-8, 1e-6),
+                                              (24, "pileup", 1e-3, 1e-2), (32, "kooshball", 1e-12, 1e-5)])
+def test_nufft_type1_gather_spreader(N, n, traj, tol, bound):
+    # The output-driven spreader (variant 6): 16^3-cell cores gathering from their 27
+    # neighbours, fine grids 48^3 (three cores per dim: every neighbour of a core at the
+    # edge is a periodic image), 64^3 and 80^3; the kooshball and the pileup (1500 points
+    # on one position) make cores with more than 128 window-overlapping points, whose further
+    # parts go through the second, atomic launch; ns from 4 to 16. Against the direct sum,
+    # with two transforms per plan (the core bins are built once), and against the tile
+    # spreader (variant 3) to rounding.
+    from hastycompute_amd import _capi
+
+    rng = np.random.default_rng(n + int(-np.log10(tol)))
+    if traj == "kooshball":
+        co = O.kooshball(400, 2 * n)
+    elif traj == "uniform":
+        co = rng.uniform(-np.pi, np.pi, (3, 8000))
+    else:
+        p0 = rng.uniform(-np.pi, np.pi, (3, 1))
+        co = np.concatenate([np.repeat(p0, 1500, axis=1), p0 + rng.uniform(-0.3, 0.3, (3, 3000)),
+                             rng.uniform(-np.pi, np.pi, (3, 2000))], axis=1)
+    co32 = co.astype(np.float32).astype(np.float64)
+    M = co.shape[1]
+    y = rnd(rng, (2, M))
+    prev = _capi.get_option("nufft_spread_variant")
+    outs = {}
+    try:
+        for v in (6, 3):
+            _capi.set_option("nufft_spread_variant", v)
+            plan = N.NufftPlan(g(co, torch.float32), [n, n, n], 1, +1, 2, tol)
+            nf, ns, _ = plan.info()
+            img = torch.empty((2, n, n, n), dtype=torch.complex64, device="cuda")
+            plan.execute(g(y), img)
+            outs[v] = h(img)
+            plan.close()
+    finally:
+        _capi.set_option("nufft_spread_variant", prev)
+    assert all(f % 16 == 0 and f >= 48 for f in nf)
+    for t in range(2):
+        e = O.rel_l2(outs[6][t], O.nudft_type1(co32, y[t], (n, n, n), +1))
+        e3 = O.rel_l2(outs[6][t], outs[3][t])
+        print(f"gather n {n} {traj} tol {tol} (ns {ns}) transform {t}: {e:.2e}, vs tiles {e3:.1e}", flush=True)
+        assert e < bound and e3 < 1e-5
 
 
 @pytest.mark.parametrize("locality", [0, 1, 2, 3])
