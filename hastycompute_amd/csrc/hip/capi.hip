@@ -395,6 +395,10 @@ int hasty_set_option(const char* name, int64_t value)
         nufft_set_chunk((int)value);
         return 0;
     }
+    if (name && std::strcmp(name, "fft_r16") == 0 && (value == 0 || value == 1)) {
+        fft_set_r16((int)value);
+        return 0;
+    }
     if (name && std::strcmp(name, "fft_strided_lines") == 0 && value >= 0 && value <= 64) {
         fft_set_strided_lines((int)value);
         return 0;
@@ -420,6 +424,7 @@ int hasty_get_option(const char* name, int64_t* value)
     else if (name && std::strcmp(name, "nufft_spread_chunk") == 0) *value = nufft_get_chunk();
     else if (name && std::strcmp(name, "nufft_locality") == 0) *value = nufft_get_locality();
     else if (name && std::strcmp(name, "fft_strided_lines") == 0) *value = fft_get_strided_lines();
+    else if (name && std::strcmp(name, "fft_r16") == 0) *value = fft_get_r16();
     else {
         set_last_error(std::string("unknown option ") + (name ? name : "(null)"));
         return 1;

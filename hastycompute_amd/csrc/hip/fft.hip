@@ -1,7 +1,7 @@
 // LDS-staged mixed-radix (2,3,4,5,8) Stockham FFT passes for gfx950.
 //
 // One workgroup of 256 threads transforms a bundle of B lines held in LDS (two
-// ping-pong buffers, line stride L+1 complex to spread banks). Loads and stores go
+// ping-pong buffers, line stride L+1 complex to spread banks, and the line's twiddles). Loads and stores go
 // straight between HBM and LDS with the zero-pad / crop / centre mapping, optional
 // per-element multipliers and per-element real scales, so pad+deapodise+FFT or
 // FFT+crop+coil-multiply never need a separate elementwise pass.
@@ -11,6 +11,7 @@
 #include <atomic>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 #include <cmath>
 
@@ -187,6 +188,7 @@ template <typename R> struct AxisKernelArgs {
     uint64_t radcode;
     int nrad;
     int lines_per_block;
+    int stage_tw;  // twiddles staged in LDS
 };
 
 // MODE 0: plain FFT with sign a.sign; MODE 1: conv with real kernel; MODE 2: conv with
@@ -205,7 +207,10 @@ __global__ void __launch_bounds__(kFftThreads) axis_fft_kernel(AxisKernelArgs<R>
 
     cplx<R>* bufA = reinterpret_cast<cplx<R>*>(smem_raw);
     cplx<R>* bufB = bufA + B * Lp;
-    int64_t* bases = reinterpret_cast<int64_t*>(bufB + B * Lp);  // 5 * B entries
+    // the L twiddles, staged with the data when they fit in LDS
+    cplx<R>* tws = bufB + B * Lp;
+    const cplx<R>* twl = args.stage_tw ? tws : args.tw;
+    int64_t* bases = reinterpret_cast<int64_t*>(tws + (args.stage_tw ? L : 0));  // 5 * B entries
     int64_t* b_in = bases;
     int64_t* b_mul = bases + B;
     int64_t* b_out = bases + 2 * B;
@@ -222,6 +227,8 @@ __global__ void __launch_bounds__(kFftThreads) axis_fft_kernel(AxisKernelArgs<R>
     }
     __syncthreads();
 
+    if (args.stage_tw)
+        for (int k = tid; k < L; k += kFftThreads) tws[k] = args.tw[k];
     // Load (pad / centre / multiply / scale). Each thread issues kLoadUnroll independent
     // loads before its first LDS store, so a workgroup keeps kLoadUnroll x 256 element
     // loads in flight instead of 256.
@@ -268,11 +275,11 @@ __global__ void __launch_bounds__(kFftThreads) axis_fft_kernel(AxisKernelArgs<R>
     cplx<R>* dst = bufB;
     if (MODE == 0) {
         if (a.sign < 0)
-            fft_bundle<R, -1>(src, dst, L, Lp, nb, args.radcode, args.nrad, args.tw, tid);
+            fft_bundle<R, -1>(src, dst, L, Lp, nb, args.radcode, args.nrad, twl, tid);
         else
-            fft_bundle<R, 1>(src, dst, L, Lp, nb, args.radcode, args.nrad, args.tw, tid);
+            fft_bundle<R, 1>(src, dst, L, Lp, nb, args.radcode, args.nrad, twl, tid);
     } else {
-        fft_bundle<R, -1>(src, dst, L, Lp, nb, args.radcode, args.nrad, args.tw, tid);
+        fft_bundle<R, -1>(src, dst, L, Lp, nb, args.radcode, args.nrad, twl, tid);
         for (int e = tid; e < nb * L; e += kFftThreads) {
             const int li = e / L;
             const int i = e - li * L;
@@ -285,7 +292,7 @@ __global__ void __launch_bounds__(kFftThreads) axis_fft_kernel(AxisKernelArgs<R>
             src[li * Lp + i] = v;
         }
         __syncthreads();
-        fft_bundle<R, 1>(src, dst, L, Lp, nb, args.radcode, args.nrad, args.tw, tid);
+        fft_bundle<R, 1>(src, dst, L, Lp, nb, args.radcode, args.nrad, twl, tid);
     }
 
     // Store (crop / centre / scale / multiply / accumulate).
@@ -312,6 +319,118 @@ __global__ void __launch_bounds__(kFftThreads) axis_fft_kernel(AxisKernelArgs<R>
     }
 }
 
+// ---- L = 256 complex64 lines as a 16 x 16 four-step FFT in registers -------------------
+//
+// 16 lines per workgroup of 256 threads, 16 threads per line. With n = g + 16 m and
+// k = k2 + 16 k1, X[k] = sum_g W16^(g k1) W256^(g k2) sum_m x[g + 16 m] W16^(m k2):
+// thread g of a line holds x[g + 16 m] (m < 16), takes the 16-point DFT over m and the
+// twiddle W256^(g k2), the line passes through LDS (row pitch 17), and thread k2 takes
+// the 16-point DFT over g, holding X[k2 + 16 k1]. Contiguous lines give a line's 16
+// threads adjacent lanes (each load and store covers 16 consecutive elements); strided
+// lines give the 16 lines of one g adjacent lanes (each covers 16 adjacent lines). The
+// pad / centre / scale / crop mapping is the general kernel's; lines with coil
+// multipliers, accumulation or a convolution take the general kernel.
+constexpr int kR16Lines = 16;
+
+template <int S> __device__ __forceinline__ float2 w16(int j)
+{
+    // exp(S 2 pi i j / 16), j < 16
+    const float c[16] = {1.f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f, 0.f,
+                         -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f, -1.f,
+                         -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508977f, 0.f,
+                         0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f};
+    const float sn = c[(j + 12) & 15];  // sin(2 pi j / 16) = cos(2 pi (j - 4) / 16)
+    return float2{c[j & 15], (float)S * sn};
+}
+
+// 16-point DFT with sign S, natural order in and out: n = 4 n1 + n2, k = k1 + 4 k2
+template <int S> __device__ __forceinline__ void dft16(float2* v)
+{
+    float2 a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        float2 t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+        Dft<4, S, float2, float>::run(t);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = (n2 * k1 == 0) ? t[k1] : cmul(t[k1], w16<S>(n2 * k1));
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        float2 t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+        Dft<4, S, float2, float>::run(t);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+}
+
+template <int S>
+__global__ void __launch_bounds__(kFftThreads) axis_fft_r16_kernel(AxisKernelArgs<float> args)
+{
+    __shared__ float2 tr[kR16Lines * 16 * 17];
+    __shared__ float2 tws[256];
+    const AxisFft<float>& a = args.a;
+    const int tid = threadIdx.x;
+    tws[tid] = args.tw[tid];
+    int li, g;
+    if (a.ain.stride != 1) {
+        g = tid >> 4;
+        li = tid & 15;
+    } else {
+        li = tid >> 4;
+        g = tid & 15;
+    }
+    const int64_t l = (int64_t)blockIdx.x * kR16Lines + li;
+    const bool lv = l < a.nlines;
+    const int64_t bin = lv ? line_base(a.ain, l) : 0;
+    float2 v[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int n = in_element(g + 16 * m, 256, a.n_in, a.in_centered);
+        float2 x = float2{0.f, 0.f};
+        if (lv && n >= 0) {
+            x = a.in[bin + (int64_t)n * a.ain.stride];
+            if (a.pre_scale) x = cscale(x, a.pre_scale[n]);
+        }
+        v[m] = x;
+    }
+    dft16<S>(v);
+    __syncthreads();  // tws
+#pragma unroll
+    for (int k2 = 1; k2 < 16; ++k2) {
+        float2 w = tws[g * k2];
+        if (S > 0) w.y = -w.y;
+        v[k2] = cmul(v[k2], w);
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) tr[(li * 16 + k2) * 17 + g] = v[k2];
+    __syncthreads();
+    int lo, k2;
+    if (a.aout.stride != 1) {
+        k2 = tid >> 4;
+        lo = tid & 15;
+    } else {
+        lo = tid >> 4;
+        k2 = tid & 15;
+    }
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = tr[(lo * 16 + k2) * 17 + m];
+    dft16<S>(v);
+    const int64_t l2 = (int64_t)blockIdx.x * kR16Lines + lo;
+    if (l2 >= a.nlines) return;
+    const int64_t bout = line_base(a.aout, l2);
+    const int h = a.n_out / 2;
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) {
+        const int k = k2 + 16 * k1;
+        const int n = a.out_centered ? ((k + h) & 255) : k;  // inverse of k = (n - h) mod 256
+        if (n < a.n_out) {
+            float2 y = cscale(v[k1], a.scale);
+            if (a.post_scale) y = cscale(y, a.post_scale[n]);
+            a.out[bout + (int64_t)n * a.aout.stride] = y;
+        }
+    }
+}
+
 RadixPlan make_radix_plan(int L)
 {
     RadixPlan p;
@@ -331,12 +450,15 @@ RadixPlan make_radix_plan(int L)
 }
 
 std::atomic<int> g_strided_lines{0};  // lines per workgroup of a strided pass, 0 = auto
+std::atomic<int> g_r16{1};            // L = 256 complex64 lines by the register four-step kernel
 
 std::mutex g_tw_mutex;
 std::map<std::tuple<int, int, int>, void*> g_tw_cache;  // (device, L, sizeof(R)) -> table
 
 }  // namespace
 
+void fft_set_r16(int v) { g_r16 = v; }
+int fft_get_r16() { return g_r16; }
 void fft_set_strided_lines(int v) { g_strided_lines = v; }
 int fft_get_strided_lines() { return g_strided_lines; }
 
@@ -401,14 +523,31 @@ template <typename R> void run_axis_fft(const AxisFft<R>& a, hipStream_t stream)
     // latency-bound, not bound by the re-read halves of L2 lines.
     const int want = g_strided_lines.load();
     if (want > 0 && (a.ain.stride != 1 || a.aout.stride != 1)) B = want;
-    auto lds_bytes = [&](int b) { return sizeof(cplx<R>) * 2 * (size_t)b * (a.L + 1) + sizeof(int64_t) * 5 * b; };
-    while (B > 1 && lds_bytes(B) > 160 * 1024) B /= 2;
+    auto lds_bytes = [&](int b, bool tw) {
+        return sizeof(cplx<R>) * (2 * (size_t)b * (a.L + 1) + (tw ? a.L : 0)) + sizeof(int64_t) * 5 * b;
+    };
+    while (B > 1 && lds_bytes(B, false) > 160 * 1024) B /= 2;
     args.lines_per_block = B;
-    const size_t smem = lds_bytes(B);
+    args.stage_tw = lds_bytes(B, true) <= 160 * 1024 ? 1 : 0;
+    const size_t smem = lds_bytes(B, args.stage_tw != 0);
     const int64_t blocks = ceil_div(a.nlines, B);
     HASTY_REQUIRE(blocks < (1ll << 31), "too many FFT lines for one launch");
     const int mode = a.conv_real ? 1 : (a.conv_cplx ? 2 : 0);
     KernelTimer t("fft_axis", stream);
+    if constexpr (std::is_same<R, float>::value) {
+        if (g_r16.load() && a.L == 256 && mode == 0 && !a.inmul && !a.outmul && !a.accumulate) {
+            const int64_t rblocks = ceil_div(a.nlines, kR16Lines);
+            HASTY_REQUIRE(rblocks < (1ll << 31), "too many FFT lines for one launch");
+            if (a.sign < 0)
+                hipLaunchKernelGGL((axis_fft_r16_kernel<-1>), dim3((unsigned)rblocks), dim3(kFftThreads), 0, stream,
+                                   args);
+            else
+                hipLaunchKernelGGL((axis_fft_r16_kernel<1>), dim3((unsigned)rblocks), dim3(kFftThreads), 0, stream,
+                                   args);
+            check_launch("axis_fft_r16_kernel");
+            return;
+        }
+    }
     if (mode == 0) {
         allow_lds(axis_fft_kernel<R, 0>, smem);
         hipLaunchKernelGGL((axis_fft_kernel<R, 0>), dim3((unsigned)blocks), dim3(kFftThreads), smem, stream, args);

@@ -229,6 +229,38 @@ def test_nufft_type1_gather_spreader(N, n, traj, tol, bound):
         assert e < bound and e3 < 1e-5
 
 
+def test_nufft_128_register_fft(N):
+    # 128^3 modes (fine grid 256^3): every axis pass runs complex64 lines of 256 through
+    # the register 16 x 16 four-step kernel (option fft_r16 = 1): contiguous and strided
+    # lines, zero-padded and centred inputs (type 2), cropped and centred outputs (type 1),
+    # deconvolution scales. Type 2 and type 1 against the oracle's FP64 NUFFT (tol 1e-12)
+    # and against the LDS Stockham kernel (fft_r16 = 0) to rounding.
+    from hastycompute_amd import _capi
+
+    n = 128
+    rng = np.random.default_rng(128)
+    co = O.kooshball(150, 256)
+    co32 = co.astype(np.float32).astype(np.float64)
+    M = co.shape[1]
+    f = rnd(rng, (n, n, n))
+    y = rnd(rng, M)
+    prev = _capi.get_option("fft_r16")
+    outs = {}
+    try:
+        for r16 in (1, 0):
+            _capi.set_option("fft_r16", r16)
+            outs[r16] = (h(N.nufft(g(co, torch.float32), g(f[None]), [n, n, n], 2, -1, 1e-6))[0],
+                         h(N.nufft(g(co, torch.float32), g(y[None]), [n, n, n], 1, +1, 1e-6))[0])
+    finally:
+        _capi.set_option("fft_r16", prev)
+    e2 = O.rel_l2(outs[1][0], O.nufft_type2(co32, f, -1, 1e-12))
+    e1 = O.rel_l2(outs[1][1], O.nufft_type1(co32, y, (n, n, n), +1, 1e-12))
+    d2 = O.rel_l2(outs[1][0], outs[0][0])
+    d1 = O.rel_l2(outs[1][1], outs[0][1])
+    print(f"128^3 register FFT: type 2 {e2:.2e}, type 1 {e1:.2e}; vs Stockham {d2:.1e}, {d1:.1e}", flush=True)
+    assert e2 < 1e-5 and e1 < 1e-5 and d2 < 1e-6 and d1 < 1e-6
+
+
 @pytest.mark.parametrize("locality", [0, 1, 2, 3])
 @pytest.mark.parametrize("traj", ["kooshball", "pileup"])
 def test_nufft_point_orders(N, locality, traj):
