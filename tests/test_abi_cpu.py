@@ -131,3 +131,36 @@ def test_llr_grid_corners_match_the_oracle_grid():
         for it in range(4):
             got = native.llr_grid_corners(dims, strides, shape, shifts[:, it])
             assert [tuple(c) for c in got] == [tuple(c) for c in ref[it]], (dims, it)
+
+
+def test_options_round_trip_on_host():
+    """Every option the header documents for hasty_set_option can be read; the ranged ones
+    accept their bounds and reject the values just outside (host code, no kernel)."""
+    import re
+
+    text = open(_capi.HEADER).read()
+    block = text[text.index("/* Library options"):text.index("int hasty_set_option")]
+    names = re.findall(r'^ \*   "(\w+)"', block, flags=re.M)
+    assert len(names) >= 12
+    L = _capi.lib()
+    saved = {n: _capi.get_option(n) for n in names}
+    ranges = {"sense_xy_variant": ([0, 3], [-1, 4]), "sense_zconv_batch": ([1, 16], [0, 17]),
+              "nufft_locality": ([0, 4], [-1, 5]), "sense_zconv_variant": ([0, 3], [-1, 4]),
+              "llr_max_sweeps": ([1, 15], [0, 16]), "llr_eig_threads": ([256, 512, 768], [128, 300, 1024]),
+              "nufft_spread_tile": ([0, 12, 28], [11, 29]), "nufft_spread_chunk": ([0, 32, 8192], [31, 8193]),
+              "nufft_spread_variant": ([0, 6], [-1, 7]), "fft_r16": ([0, 1], [-1, 2]),
+              "fft_strided_lines": ([0, 64], [-1, 65])}
+    try:
+        for n, (good, bad) in ranges.items():
+            assert n in names, n
+            for v in good:
+                _capi.set_option(n, v)
+                assert _capi.get_option(n) == v, (n, v)
+            before = _capi.get_option(n)
+            for v in bad:
+                assert L.hasty_set_option(n.encode(), v) == 1, (n, v)
+                assert _capi.get_option(n) == before, (n, v)
+        assert L.hasty_set_option(b"no_such_option", 1) == 1
+    finally:
+        for n, v in saved.items():
+            _capi.set_option(n, v)
