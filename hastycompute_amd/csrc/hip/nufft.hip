@@ -1790,7 +1790,8 @@ bool nufft_gather_fits(const NufftPlan* p) { return gather_fits(p); }
 void nufft_execute(NufftPlan* p, void* c, void* f, hipStream_t s)
 {
     HASTY_REQUIRE(p != nullptr, "null plan");
-    HASTY_REQUIRE(c != nullptr && f != nullptr, "null data pointer");
+    // c may be null only when there are no points (an empty tensor's pointer)
+    HASTY_REQUIRE((c != nullptr || p->M == 0) && f != nullptr, "null data pointer");
     if (p->fp64) {
         if (p->type == 1)
             execute_type1<double>(p, (const double2*)c, (double2*)f, s);

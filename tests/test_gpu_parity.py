@@ -229,6 +229,33 @@ def test_nufft_type1_gather_spreader(N, n, traj, tol, bound):
         assert e < bound and e3 < 1e-5
 
 
+@pytest.mark.parametrize("n", [24, 20])
+def test_nufft_no_points(N, n):
+    # An empty point set (M = 0, empty tensors with null data pointers): type 1 gives a
+    # zero image, type 2 writes nothing; at n = 24 (fine grid 48^3) through the
+    # output-driven spreader, whose first launch alone stores the zeros, at n = 20 through
+    # the tile spreader and the zero pass. A plan reused after points are set again still
+    # matches the direct sum.
+    co = torch.empty((3, 0), dtype=torch.float32, device="cuda")
+    c = torch.empty((1, 0), dtype=torch.complex64, device="cuda")
+    p1 = N.NufftPlan(co, [n, n, n], 1, +1, 1, 1e-6)
+    f = torch.full((1, n, n, n), 7.0 + 0j, dtype=torch.complex64, device="cuda")
+    p1.execute(c, f)
+    torch.cuda.synchronize()
+    assert torch.count_nonzero(f).item() == 0
+    p1.close()
+    p2 = N.NufftPlan(co, [n, n, n], 2, -1, 1, 1e-6)
+    p2.execute(c, torch.ones((1, n, n, n), dtype=torch.complex64, device="cuda"))
+    torch.cuda.synchronize()
+    p2.close()
+    # the same plan shape with points
+    rng = np.random.default_rng(n)
+    pts = rng.uniform(-np.pi, np.pi, (3, 500))
+    y = rnd(rng, 500)
+    img = h(N.nufft(g(pts, torch.float32), g(y[None]), [n, n, n], 1, +1, 1e-6))[0]
+    assert O.rel_l2(img, O.nudft_type1(pts.astype(np.float32).astype(np.float64), y, (n, n, n), +1)) < 1e-5
+
+
 def test_nufft_128_register_fft(N):
     # 128^3 modes (fine grid 256^3): every axis pass runs complex64 lines of 256 through
     # the register 16 x 16 four-step kernel (option fft_r16 = 1): contiguous and strided
