@@ -841,10 +841,9 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
         __syncthreads();
         // Wave w owns the core planes i0 with i0 mod NW == w. Per point: each lane's
         // row-column cell offset and weight (strength folded in), then per owned plane
-        // (wave-uniform values in scalar registers) the loads of its cells; while they
-        // are in flight, the next point's row-column part; then the adds and stores. One
-        // wave's LDS operations execute in order, so the next point's loads see these
-        // stores.
+        // (a wave-uniform branch) the loads of its cells; while they are in flight, the
+        // next point's row-column part; then the adds and stores. One wave's LDS
+        // operations execute in order, so the next point's loads see these stores.
         int off[NQ];
         float2 wc[NQ];
         int p0 = 0;
@@ -869,7 +868,7 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
             const int ia0 = (wave - p0) & (NW - 1);
             float2 val[KP][NQ];
             int cell0[KP];
-            int w0[KP];  // bits of the plane weight, in a scalar register
+            float w0[KP];
             bool pl[KP];
 #pragma unroll
             for (int k = 0; k < KP; ++k) {
@@ -877,7 +876,7 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
                 const int i0 = p0 + ia;
                 pl[k] = ia < NS && (unsigned)i0 < (unsigned)kGC;  // wave-uniform
                 cell0[k] = i0 * kGC * rp;
-                w0[k] = pl[k] ? __float_as_int(st[2 + ia]) : 0;
+                w0[k] = pl[k] ? st[2 + ia] : 0.f;
                 if (pl[k]) {
 #pragma unroll
                     for (int u = 0; u < NQ; ++u)
@@ -899,7 +898,7 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
 #pragma unroll
             for (int k = 0; k < KP; ++k) {
                 if (pl[k]) {
-                    const float wa = __int_as_float(w0[k]);
+                    const float wa = w0[k];
 #pragma unroll
                     for (int u = 0; u < NQ; ++u) {
                         if (off[u] >= 0) {
