@@ -319,116 +319,154 @@ __global__ void __launch_bounds__(kFftThreads) axis_fft_kernel(AxisKernelArgs<R>
     }
 }
 
-// ---- L = 256 complex64 lines as a 16 x 16 four-step FFT in registers -------------------
+// ---- complex64 lines of 16 R (R = 8, 12, 16, 20) as a 16 x R four-step FFT in registers -
 //
-// 16 lines per workgroup of 256 threads, 16 threads per line. With n = g + 16 m and
-// k = k2 + 16 k1, X[k] = sum_g W16^(g k1) W256^(g k2) sum_m x[g + 16 m] W16^(m k2):
-// thread g of a line holds x[g + 16 m] (m < 16), takes the 16-point DFT over m and the
-// twiddle W256^(g k2), the line passes through LDS (row pitch 17), and thread k2 takes
-// the 16-point DFT over g, holding X[k2 + 16 k1]. Contiguous lines give a line's 16
-// threads adjacent lanes (each load and store covers 16 consecutive elements); strided
-// lines give the 16 lines of one g adjacent lanes (each covers 16 adjacent lines). The
-// pad / centre / scale / crop mapping is the general kernel's; lines with coil
-// multipliers, accumulation or a convolution take the general kernel.
+// 16 lines per workgroup. With n = g + 16 m (g < 16, m < R) and k = k2 + R k1 (k2 < R,
+// k1 < 16), X[k] = sum_g W16^(g k1) W_L^(g k2) sum_m x[g + 16 m] W_R^(m k2): thread g of a
+// line holds x[g + 16 m], takes the R-point DFT over m and the twiddle W_L^(g k2), the
+// line passes through LDS (row pitch 17), and thread k2 takes the 16-point DFT over g,
+// holding X[k2 + R k1]. The workgroup has 16 max(16, R) threads: 16 per line in the first
+// half, R per line in the second. Contiguous lines give a line's threads adjacent lanes
+// (each load and store covers consecutive elements); strided lines give the 16 lines of
+// one g or k2 adjacent lanes (each covers 16 adjacent lines). The pad / centre / scale /
+// crop mapping is the general kernel's; lines with coil multipliers, accumulation or a
+// convolution take the general kernel.
 constexpr int kR16Lines = 16;
 
-template <int S> __device__ __forceinline__ float2 w16(int j)
+// DFT of size A B with sign S in registers, natural order in and out: n = A n2 + n1...
+// written as n = B n1 + n2 (n1 < A, n2 < B), k = k1 + A k2: A-point DFTs over n1, the
+// twiddles W_AB^(n2 k1), B-point DFTs over n2.
+template <int A, int B, int S> __device__ __forceinline__ void dft_ab(float2* v)
 {
-    // exp(S 2 pi i j / 16), j < 16
-    const float c[16] = {1.f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f, 0.f,
-                         -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f, -1.f,
-                         -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508977f, 0.f,
-                         0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f};
-    const float sn = c[(j + 12) & 15];  // sin(2 pi j / 16) = cos(2 pi (j - 4) / 16)
-    return float2{c[j & 15], (float)S * sn};
-}
-
-// 16-point DFT with sign S, natural order in and out: n = 4 n1 + n2, k = k1 + 4 k2
-template <int S> __device__ __forceinline__ void dft16(float2* v)
-{
-    float2 a[4][4];
+    float2 t[B][A];
 #pragma unroll
-    for (int n2 = 0; n2 < 4; ++n2) {
-        float2 t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
-        Dft<4, S, float2, float>::run(t);
+    for (int n2 = 0; n2 < B; ++n2) {
+        float2 u[A];
 #pragma unroll
-        for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = (n2 * k1 == 0) ? t[k1] : cmul(t[k1], w16<S>(n2 * k1));
+        for (int n1 = 0; n1 < A; ++n1) u[n1] = v[B * n1 + n2];
+        Dft<A, S, float2, float>::run(u);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) {
+            if (n2 * k1 == 0) {
+                t[n2][k1] = u[k1];
+            } else {
+                const float ang = (float)S * 6.28318530717958648f * (float)(n2 * k1) / (float)(A * B);
+                t[n2][k1] = cmul(u[k1], float2{cosf(ang), sinf(ang)});
+            }
+        }
     }
 #pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) {
-        float2 t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
-        Dft<4, S, float2, float>::run(t);
+    for (int k1 = 0; k1 < A; ++k1) {
+        float2 u[B];
 #pragma unroll
-        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+        for (int n2 = 0; n2 < B; ++n2) u[n2] = t[n2][k1];
+        Dft<B, S, float2, float>::run(u);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = u[k2];
     }
 }
 
-template <int S>
-__global__ void __launch_bounds__(kFftThreads) axis_fft_r16_kernel(AxisKernelArgs<float> args)
+template <int R, int S> __device__ __forceinline__ void dft_r(float2* v)
 {
-    __shared__ float2 tr[kR16Lines * 16 * 17];
-    __shared__ float2 tws[256];
+    if constexpr (R == 8) Dft<8, S, float2, float>::run(v);
+    else if constexpr (R == 12) dft_ab<4, 3, S>(v);
+    else if constexpr (R == 16) dft_ab<4, 4, S>(v);
+    else dft_ab<4, 5, S>(v);  // 20
+}
+
+template <int R, int S>
+__global__ void __launch_bounds__(kR16Lines * (R > 16 ? R : 16)) axis_fft_r16_kernel(AxisKernelArgs<float> args)
+{
+    constexpr int L = 16 * R;
+    __shared__ float2 tr[kR16Lines * R * 17];
+    __shared__ float2 tws[L];
     const AxisFft<float>& a = args.a;
     const int tid = threadIdx.x;
-    tws[tid] = args.tw[tid];
-    int li, g;
-    if (a.ain.stride != 1) {
-        g = tid >> 4;
-        li = tid & 15;
-    } else {
-        li = tid >> 4;
-        g = tid & 15;
-    }
-    const int64_t l = (int64_t)blockIdx.x * kR16Lines + li;
-    const bool lv = l < a.nlines;
-    const int64_t bin = lv ? line_base(a.ain, l) : 0;
-    float2 v[16];
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const int n = in_element(g + 16 * m, 256, a.n_in, a.in_centered);
-        float2 x = float2{0.f, 0.f};
-        if (lv && n >= 0) {
-            x = a.in[bin + (int64_t)n * a.ain.stride];
-            if (a.pre_scale) x = cscale(x, a.pre_scale[n]);
+    for (int k = tid; k < L; k += kR16Lines * (R > 16 ? R : 16)) tws[k] = args.tw[k];
+    float2 v[R > 16 ? R : 16];
+    if (tid < 256) {
+        int li, g;
+        if (a.ain.stride != 1) {
+            g = tid >> 4;
+            li = tid & 15;
+        } else {
+            li = tid >> 4;
+            g = tid & 15;
         }
-        v[m] = x;
+        const int64_t l = (int64_t)blockIdx.x * kR16Lines + li;
+        const bool lv = l < a.nlines;
+        const int64_t bin = lv ? line_base(a.ain, l) : 0;
+#pragma unroll
+        for (int m = 0; m < R; ++m) {
+            const int n = in_element(g + 16 * m, L, a.n_in, a.in_centered);
+            float2 x = float2{0.f, 0.f};
+            if (lv && n >= 0) {
+                x = a.in[bin + (int64_t)n * a.ain.stride];
+                if (a.pre_scale) x = cscale(x, a.pre_scale[n]);
+            }
+            v[m] = x;
+        }
+        dft_r<R, S>(v);
     }
-    dft16<S>(v);
     __syncthreads();  // tws
+    if (tid < 256) {
+        int li, g;
+        if (a.ain.stride != 1) {
+            g = tid >> 4;
+            li = tid & 15;
+        } else {
+            li = tid >> 4;
+            g = tid & 15;
+        }
 #pragma unroll
-    for (int k2 = 1; k2 < 16; ++k2) {
-        float2 w = tws[g * k2];
-        if (S > 0) w.y = -w.y;
-        v[k2] = cmul(v[k2], w);
+        for (int k2 = 1; k2 < R; ++k2) {
+            float2 w = tws[g * k2];
+            if (S > 0) w.y = -w.y;
+            v[k2] = cmul(v[k2], w);
+        }
+#pragma unroll
+        for (int k2 = 0; k2 < R; ++k2) tr[(li * R + k2) * 17 + g] = v[k2];
     }
-#pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) tr[(li * 16 + k2) * 17 + g] = v[k2];
     __syncthreads();
+    if (tid >= 16 * R) return;
     int lo, k2;
     if (a.aout.stride != 1) {
         k2 = tid >> 4;
         lo = tid & 15;
     } else {
-        lo = tid >> 4;
-        k2 = tid & 15;
+        lo = tid / R;
+        k2 = tid - lo * R;
     }
 #pragma unroll
-    for (int m = 0; m < 16; ++m) v[m] = tr[(lo * 16 + k2) * 17 + m];
-    dft16<S>(v);
+    for (int m = 0; m < 16; ++m) v[m] = tr[(lo * R + k2) * 17 + m];
+    dft_r<16, S>(v);
     const int64_t l2 = (int64_t)blockIdx.x * kR16Lines + lo;
     if (l2 >= a.nlines) return;
     const int64_t bout = line_base(a.aout, l2);
     const int h = a.n_out / 2;
 #pragma unroll
     for (int k1 = 0; k1 < 16; ++k1) {
-        const int k = k2 + 16 * k1;
-        const int n = a.out_centered ? ((k + h) & 255) : k;  // inverse of k = (n - h) mod 256
+        const int k = k2 + R * k1;
+        int n = k;
+        if (a.out_centered) {
+            n = k + h;  // inverse of k = (n - h) mod L
+            if (n >= L) n -= L;
+        }
         if (n < a.n_out) {
             float2 y = cscale(v[k1], a.scale);
             if (a.post_scale) y = cscale(y, a.post_scale[n]);
             a.out[bout + (int64_t)n * a.aout.stride] = y;
         }
     }
+}
+
+template <int R> void launch_r16(const AxisKernelArgs<float>& args, int64_t blocks, hipStream_t stream)
+{
+    const unsigned threads = kR16Lines * (R > 16 ? R : 16);
+    if (args.a.sign < 0)
+        hipLaunchKernelGGL((axis_fft_r16_kernel<R, -1>), dim3((unsigned)blocks), dim3(threads), 0, stream, args);
+    else
+        hipLaunchKernelGGL((axis_fft_r16_kernel<R, 1>), dim3((unsigned)blocks), dim3(threads), 0, stream, args);
 }
 
 RadixPlan make_radix_plan(int L)
@@ -535,15 +573,16 @@ template <typename R> void run_axis_fft(const AxisFft<R>& a, hipStream_t stream)
     const int mode = a.conv_real ? 1 : (a.conv_cplx ? 2 : 0);
     KernelTimer t("fft_axis", stream);
     if constexpr (std::is_same<R, float>::value) {
-        if (g_r16.load() && a.L == 256 && mode == 0 && !a.inmul && !a.outmul && !a.accumulate) {
+        const bool reg = a.L == 128 || a.L == 192 || a.L == 256 || a.L == 320;
+        if (g_r16.load() && reg && mode == 0 && !a.inmul && !a.outmul && !a.accumulate) {
             const int64_t rblocks = ceil_div(a.nlines, kR16Lines);
             HASTY_REQUIRE(rblocks < (1ll << 31), "too many FFT lines for one launch");
-            if (a.sign < 0)
-                hipLaunchKernelGGL((axis_fft_r16_kernel<-1>), dim3((unsigned)rblocks), dim3(kFftThreads), 0, stream,
-                                   args);
-            else
-                hipLaunchKernelGGL((axis_fft_r16_kernel<1>), dim3((unsigned)rblocks), dim3(kFftThreads), 0, stream,
-                                   args);
+            switch (a.L / 16) {
+            case 8: launch_r16<8>(args, rblocks, stream); break;
+            case 12: launch_r16<12>(args, rblocks, stream); break;
+            case 16: launch_r16<16>(args, rblocks, stream); break;
+            default: launch_r16<20>(args, rblocks, stream); break;
+            }
             check_launch("axis_fft_r16_kernel");
             return;
         }

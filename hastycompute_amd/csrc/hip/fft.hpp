@@ -50,8 +50,9 @@ template <typename R> struct AxisFft {
 template <typename R> void run_axis_fft(const AxisFft<R>& a, hipStream_t stream);
 
 bool fft_size_supported(int L);
-// 1 (default): complex64 lines of length 256 without coil multipliers, accumulation or
-// convolution go through the register 16 x 16 four-step kernel; 0: the LDS Stockham one.
+// 1 (default): complex64 lines of length 128, 192, 256 or 320 without coil multipliers,
+// accumulation or convolution go through the register 16 x R four-step kernel; 0: the
+// LDS Stockham one.
 void fft_set_r16(int v);
 int fft_get_r16();
 // Lines per workgroup of a pass whose lines are strided (0 = auto, the same as a pass of

@@ -150,9 +150,10 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
  *     once (no zero pass, no global atomics outside crowded cores; complex64 plans whose
  *     fine-grid dims are multiples of 16 and at least 48, binned at setpts; other plans
  *     run 3).
- *   "fft_r16" 0 or 1 (default 1): complex64 axis FFT passes of length 256 (the NUFFT at
- *     128^3 modes) without coil multipliers, accumulation or convolution run as a 16 x 16
- *     four-step FFT in registers; 0 keeps them on the LDS Stockham passes.
+ *   "fft_r16" 0 or 1 (default 1): complex64 axis FFT passes of length 128, 192, 256 or
+ *     320 (the NUFFT at 64^3 .. 160^3 modes) without coil multipliers, accumulation or
+ *     convolution run as a 16 x R four-step FFT in registers; 0 keeps them on the LDS
+ *     Stockham passes.
  *   "fft_strided_lines" 0..64 (default 0 = auto): lines per workgroup of the axis FFT
  *     passes whose lines are strided (the NUFFT's outer axes, the Toeplitz kernel build,
  *     the generic normal operator); auto gives every pass 2048 / L (complex64) or

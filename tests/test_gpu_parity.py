@@ -256,17 +256,18 @@ def test_nufft_no_points(N, n):
     assert O.rel_l2(img, O.nudft_type1(pts.astype(np.float32).astype(np.float64), y, (n, n, n), +1)) < 1e-5
 
 
-def test_nufft_128_register_fft(N):
-    # 128^3 modes (fine grid 256^3): every axis pass runs complex64 lines of 256 through
-    # the register 16 x 16 four-step kernel (option fft_r16 = 1): contiguous and strided
-    # lines, zero-padded and centred inputs (type 2), cropped and centred outputs (type 1),
-    # deconvolution scales. Type 2 and type 1 against the oracle's FP64 NUFFT (tol 1e-12)
-    # and against the LDS Stockham kernel (fft_r16 = 0) to rounding.
+@pytest.mark.parametrize("n", [64, 96, 128, 160])
+def test_nufft_register_fft(N, n):
+    # n^3 modes with fine grids 128^3 .. 320^3: every axis pass runs complex64 lines of
+    # 16 R (R = 8, 12, 16, 20) through the register 16 x R four-step kernel (option
+    # fft_r16 = 1): contiguous and strided lines, zero-padded and centred inputs (type 2),
+    # cropped and centred outputs (type 1), deconvolution scales. Type 2 and type 1
+    # against the oracle's FP64 NUFFT (tol 1e-12) and against the LDS Stockham kernel
+    # (fft_r16 = 0) to rounding.
     from hastycompute_amd import _capi
 
-    n = 128
-    rng = np.random.default_rng(128)
-    co = O.kooshball(150, 256)
+    rng = np.random.default_rng(n)
+    co = O.kooshball(150, 2 * n)
     co32 = co.astype(np.float32).astype(np.float64)
     M = co.shape[1]
     f = rnd(rng, (n, n, n))
@@ -284,7 +285,7 @@ def test_nufft_128_register_fft(N):
     e1 = O.rel_l2(outs[1][1], O.nufft_type1(co32, y, (n, n, n), +1, 1e-12))
     d2 = O.rel_l2(outs[1][0], outs[0][0])
     d1 = O.rel_l2(outs[1][1], outs[0][1])
-    print(f"128^3 register FFT: type 2 {e2:.2e}, type 1 {e1:.2e}; vs Stockham {d2:.1e}, {d1:.1e}", flush=True)
+    print(f"{n}^3 register FFT: type 2 {e2:.2e}, type 1 {e1:.2e}; vs Stockham {d2:.1e}, {d1:.1e}", flush=True)
     assert e2 < 1e-5 and e1 < 1e-5 and d2 < 1e-6 and d1 < 1e-6
 
 
