@@ -560,7 +560,11 @@ template <typename R> void run_axis_fft(const AxisFft<R>& a, hipStream_t stream)
     // 256^3 pass at 8, 16 and 32 lines): fewer workgroups fit a CU's LDS, and the pass is
     // latency-bound, not bound by the re-read halves of L2 lines.
     const int want = g_strided_lines.load();
-    if (want > 0 && (a.ain.stride != 1 || a.aout.stride != 1)) B = want;
+    const bool strided = a.ain.stride != 1 || a.aout.stride != 1;
+    if (want > 0 && strided) B = want;
+    // complex128 strided lines of 512 / 640 (the Toeplitz kernel build) get 2 / 1 lines
+    // from the rule above; 4 built the 128^3 and 160^3 kernels 3-4 % faster
+    if (want == 0 && strided && sizeof(R) == 8 && B < 4) B = 4;
     auto lds_bytes = [&](int b, bool tw) {
         return sizeof(cplx<R>) * (2 * (size_t)b * (a.L + 1) + (tw ? a.L : 0)) + sizeof(int64_t) * 5 * b;
     };

@@ -610,7 +610,7 @@ __global__ void __launch_bounds__(64 * NW) spread_planes_kernel(SpreadArgs<R> a)
     }
 }
 
-// ---- output-driven 3-D spreading (complex64) ------------------------------------------
+// ---- output-driven 3-D spreading ----------------------------------------------------------
 //
 // The fine grid is cut into cores of kGC^3 cells (kGC divides every fine-grid dim). A
 // window of ns <= 16 cells overlaps one or two cores per dim, so each point belongs to at
@@ -624,9 +624,15 @@ __global__ void __launch_bounds__(64 * NW) spread_planes_kernel(SpreadArgs<R> a)
 // the grid with atomics. Cores are listed most-crowded first, so the long ones start
 // first.
 constexpr int kGC = 16;
-constexpr int kGatherPart = 128;
-constexpr int kGatherRest = 128;
 constexpr int kGatherWaves = 4;
+// points per part (staged at once), and the core's LDS row pitch: complex64 cores take
+// 128 points and a 19-cell pitch (three workgroups per CU at 51 KB for ns = 7), complex128
+// cores 32 points and a 17-cell pitch (two workgroups per CU at 77 KB for ns = 11)
+template <typename R> constexpr int gather_part() { return sizeof(R) == 4 ? 128 : 32; }
+template <typename R> inline int gather_row_pitch(int ns)
+{
+    return sizeof(R) == 4 ? (ns < kGC ? kGC + 3 : kGC) : kGC + 1;
+}
 
 // ES kernel for complex64 staging with the hardware exp2 and square root (about 1 ulp
 // each, far below the tolerances a complex64 plan is made for)
@@ -636,15 +642,15 @@ __device__ __forceinline__ float es_eval_fast(float z, float beta, float c)
     return t > 0.f ? __builtin_amdgcn_exp2f(beta * 1.4426950408889634f * (__builtin_amdgcn_sqrtf(t) - 1.f)) : 0.f;
 }
 
-struct GatherBins {
-    const float* pts[3];
+template <typename R> struct GatherBins {
+    const R* pts[3];
     int64_t nf[3];
     int nb[3];
     int64_t M;
     int ns;
 };
 
-__device__ __forceinline__ int gather_point_bin(const GatherBins& g, int64_t j, double* t)
+template <typename R> __device__ __forceinline__ int gather_point_bin(const GatherBins<R>& g, int64_t j, double* t)
 {
     int b[3];
 #pragma unroll
@@ -659,7 +665,7 @@ __device__ __forceinline__ int gather_point_bin(const GatherBins& g, int64_t j, 
 // One atomic per distinct bin of a wave: points along a trajectory come in runs that share
 // a bin, and the k-space centre of a radial trajectory would otherwise take one atomic
 // per spoke on a single counter.
-__global__ void gather_count_kernel(GatherBins g, int* counts, int* rank)
+template <typename R> __global__ void gather_count_kernel(GatherBins<R> g, int* counts, int* rank)
 {
     const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -683,8 +689,8 @@ __global__ void gather_count_kernel(GatherBins g, int* counts, int* rank)
 }
 
 // points in core-bin order: first window cell l0 and fine-grid position minus l0 per dim
-__global__ void gather_scatter_kernel(GatherBins g, const int* binstart, const int* rank, int* l0, float* frac,
-                                      int* idx)
+template <typename R>
+__global__ void gather_scatter_kernel(GatherBins<R> g, const int* binstart, const int* rank, int* l0, R* frac, int* idx)
 {
     const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (j >= g.M) return;
@@ -694,7 +700,7 @@ __global__ void gather_scatter_kernel(GatherBins g, const int* binstart, const i
     for (int d = 0; d < 3; ++d) {
         const double l = ceil(t[d] - 0.5 * g.ns);
         l0[d * g.M + p] = (int)l;
-        frac[d * g.M + p] = (float)(t[d] - l);
+        frac[d * g.M + p] = (R)(t[d] - l);
     }
     idx[p] = (int)j;
 }
@@ -705,8 +711,8 @@ __global__ void gather_scatter_kernel(GatherBins g, const int* binstart, const i
 // (core, neighbour), no atomics: COUNT writes the count to pcount[27 ci + k], else the
 // pairs go to pstart[27 ci + k] on in bin order (ballot compaction), so a core's pairs
 // are contiguous and their order is fixed by the core-bin order of the points.
-template <bool COUNT>
-__global__ void __launch_bounds__(kThreads) gather_pairs_kernel(GatherBins g, const int* binstart, const int* l0,
+template <typename R, bool COUNT>
+__global__ void __launch_bounds__(kThreads) gather_pairs_kernel(GatherBins<R> g, const int* binstart, const int* l0,
                                                                  int* pcount, const int* pstart, int* pairs, int* prel)
 {
     __shared__ int wave_cnt[kThreads / 64];
@@ -766,26 +772,37 @@ __global__ void __launch_bounds__(kThreads) gather_pairs_kernel(GatherBins g, co
     if (COUNT && tid == 0) pcount[blockIdx.x] = out;
 }
 
-struct GatherArgs {
-    const float* frac;   // [3][M], core-bin order
+template <typename R> struct GatherArgs {
+    const R* frac;       // [3][M], core-bin order
     const int* idx;      // [M]
     const int* pstart;   // 27 cores + 1: core ci's pairs are [pstart[27 ci], pstart[27 ci + 27])
     const int* pairs;    // sorted point of each pair
     const int* prel;     // window corner relative to the core, 8 bits per dim (+16)
     const int* list;     // (core, part) pairs of this launch
-    const float2* c;
-    float2* fine;
+    const cplx<R>* c;
+    cplx<R>* fine;
     int64_t M;
     int nb[3];
     int64_t nf[3];
-    float beta, cfac;
+    R beta, cfac;
     int rp;    // LDS row pitch of the core (float2), gather_row_pitch
     int rest;  // second launch: add non-zero cells with atomics
 };
 
-template <int NS>
-__global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(GatherArgs a)
+template <typename R> __device__ __forceinline__ R gather_es(R z, R beta, R c)
 {
+    if constexpr (sizeof(R) == 4)
+        return es_eval_fast(z, beta, c);
+    else
+        return es_eval<R>(z, beta, c);
+}
+
+template <typename R, int NS>
+__global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(GatherArgs<R> a)
+{
+    using C2 = cplx<R>;
+    constexpr int kGatherPart = gather_part<R>();
+    constexpr int kGatherRest = gather_part<R>();
     constexpr int NW = kGatherWaves;
     constexpr int NT = 64 * NW;
     constexpr int S = 3 * NS + 2;
@@ -795,8 +812,8 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
     static_assert((NW & (NW - 1)) == 0, "plane ownership by a mask");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int rp = a.rp;
-    float2* core = reinterpret_cast<float2*>(smem_raw);
-    float* stage = reinterpret_cast<float*>(core + kGC * kGC * rp);
+    C2* core = reinterpret_cast<C2*>(smem_raw);
+    R* stage = reinterpret_cast<R*>(core + kGC * kGC * rp);
     int* stage_p = reinterpret_cast<int*>(stage + kGatherPart * S);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -810,7 +827,7 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
     const int lo = part == 0 ? 0 : kGatherPart + (part - 1) * kGatherRest;
     const int hi = min(cnt, part == 0 ? kGatherPart : lo + kGatherRest);
 
-    for (int i = tid; i < kGC * kGC * rp; i += NT) core[i] = float2{0.f, 0.f};
+    for (int i = tid; i < kGC * kGC * rp; i += NT) core[i] = C2{(R)0, (R)0};
     // this lane's window rows and columns: chunk u holds (ib, q) = divmod(64 u + lane, NS)
     int cib[NQ], cq[NQ];
 #pragma unroll
@@ -826,15 +843,15 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
         if (tid < np) {
             const int p = a.pairs[pbase + r0 + tid];
             const int rel = a.prel[pbase + r0 + tid];
-            const float2 cj = a.c[a.idx[p]];
-            float* st = stage + tid * S;
+            const C2 cj = a.c[a.idx[p]];
+            R* st = stage + tid * S;
             st[0] = cj.x;
             st[1] = cj.y;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const float fr = a.frac[d * a.M + p];
+                const R fr = a.frac[d * a.M + p];
 #pragma unroll
-                for (int q = 0; q < NS; ++q) st[2 + d * NS + q] = es_eval_fast((float)q - fr, a.beta, a.cfac);
+                for (int q = 0; q < NS; ++q) st[2 + d * NS + q] = gather_es<R>((R)q - fr, a.beta, a.cfac);
                 stage_p[3 * tid + d] = ((rel >> (8 * d)) & 255) - 16;
             }
         }
@@ -845,30 +862,30 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
         // next point's row-column part; then the adds and stores. One wave's LDS
         // operations execute in order, so the next point's loads see these stores.
         int off[NQ];
-        float2 wc[NQ];
+        C2 wc[NQ];
         int p0 = 0;
-        auto prep = [&](int e, int* o, float2* w, int& q0) {
-            const float* st = stage + e * S;
+        auto prep = [&](int e, int* o, C2* w, int& q0) {
+            const R* st = stage + e * S;
             const int p1 = stage_p[3 * e + 1];
             const int p2 = stage_p[3 * e + 2];
             q0 = stage_p[3 * e];
-            const float cx = st[0], cy = st[1];
+            const R cx = st[0], cy = st[1];
 #pragma unroll
             for (int u = 0; u < NQ; ++u) {
                 const int i1 = p1 + cib[u], i2 = p2 + cq[u];
                 const bool in = u * 64 + lane < NS * NS && (unsigned)i1 < (unsigned)kGC && (unsigned)i2 < (unsigned)kGC;
                 o[u] = in ? i1 * rp + i2 : -1;
-                const float w12 = in ? st[2 + NS + cib[u]] * st[2 + 2 * NS + cq[u]] : 0.f;
-                w[u] = float2{w12 * cx, w12 * cy};
+                const R w12 = in ? st[2 + NS + cib[u]] * st[2 + 2 * NS + cq[u]] : (R)0;
+                w[u] = C2{w12 * cx, w12 * cy};
             }
         };
         if (np > 0) prep(0, off, wc, p0);
         for (int e = 0; e < np; ++e) {
-            const float* st = stage + e * S;
+            const R* st = stage + e * S;
             const int ia0 = (wave - p0) & (NW - 1);
-            float2 val[KP][NQ];
+            C2 val[KP][NQ];
             int cell0[KP];
-            float w0[KP];
+            R w0[KP];
             bool pl[KP];
 #pragma unroll
             for (int k = 0; k < KP; ++k) {
@@ -876,7 +893,7 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
                 const int i0 = p0 + ia;
                 pl[k] = ia < NS && (unsigned)i0 < (unsigned)kGC;  // wave-uniform
                 cell0[k] = i0 * kGC * rp;
-                w0[k] = pl[k] ? st[2 + ia] : 0.f;
+                w0[k] = pl[k] ? st[2 + ia] : (R)0;
                 if (pl[k]) {
 #pragma unroll
                     for (int u = 0; u < NQ; ++u)
@@ -884,7 +901,7 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
                 }
             }
             int noff[NQ];
-            float2 nwc[NQ];
+            C2 nwc[NQ];
             int np0 = 0;
             if (e + 1 < np) {
                 prep(e + 1, noff, nwc, np0);
@@ -892,13 +909,13 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
 #pragma unroll
                 for (int u = 0; u < NQ; ++u) {
                     noff[u] = -1;
-                    nwc[u] = float2{0.f, 0.f};
+                    nwc[u] = C2{(R)0, (R)0};
                 }
             }
 #pragma unroll
             for (int k = 0; k < KP; ++k) {
                 if (pl[k]) {
-                    const float wa = w0[k];
+                    const R wa = w0[k];
 #pragma unroll
                     for (int u = 0; u < NQ; ++u) {
                         if (off[u] >= 0) {
@@ -926,40 +943,38 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
     const int64_t o0 = (int64_t)b[0] * kGC, o1 = (int64_t)b[1] * kGC, o2 = (int64_t)b[2] * kGC;
     for (int i = tid; i < kGC * kGC * kGC; i += NT) {
         const int i0 = i / (kGC * kGC), i1 = (i / kGC) % kGC, i2 = i % kGC;
-        const float2 v = core[(i0 * kGC + i1) * rp + i2];
-        float2* dst = a.fine + ((o0 + i0) * a.nf[1] + (o1 + i1)) * a.nf[2] + (o2 + i2);
+        const C2 v = core[(i0 * kGC + i1) * rp + i2];
+        C2* dst = a.fine + ((o0 + i0) * a.nf[1] + (o1 + i1)) * a.nf[2] + (o2 + i2);
         if (!a.rest) {
             *dst = v;
-        } else if (v.x != 0.f || v.y != 0.f) {
+        } else if (v.x != (R)0 || v.y != (R)0) {
             atomicAdd(&dst->x, v.x);
             atomicAdd(&dst->y, v.y);
         }
     }
 }
 
-// LDS row pitch of a core (float2): lane (ib, q) of a window plane starts at bank pair
-// (3 ib + q) mod 16, at most 4 lanes per pair for ns <= 9, and the core with its stage
-// stays under 53 KB (three workgroups per CU) at ns = 7; a window of 16 columns covers
-// the banks by itself
-inline int gather_row_pitch(int ns) { return ns < kGC ? kGC + 3 : kGC; }
-inline size_t gather_lds_bytes(int ns)
+// LDS bytes of a core and its stage. The complex64 pitch puts lane (ib, q) of a window
+// plane at bank pair (3 ib + q) mod 16, at most 4 lanes per pair for ns <= 9; a window of
+// 16 columns covers the banks by itself.
+template <typename R> inline size_t gather_lds_bytes(int ns)
 {
-    return sizeof(float2) * kGC * kGC * gather_row_pitch(ns) + sizeof(float) * kGatherPart * (3 * ns + 2) +
-           sizeof(int) * kGatherPart * 3;
+    return sizeof(cplx<R>) * kGC * kGC * gather_row_pitch<R>(ns) + sizeof(R) * gather_part<R>() * (3 * ns + 2) +
+           sizeof(int) * gather_part<R>() * 3;
 }
 
-template <int NS> void launch_gather_ns(const GatherArgs& a, int64_t blocks, hipStream_t s)
+template <typename R, int NS> void launch_gather_ns(const GatherArgs<R>& a, int64_t blocks, hipStream_t s)
 {
-    const size_t smem = gather_lds_bytes(NS);
-    allow_lds(gather_spread_kernel<NS>, smem);
-    hipLaunchKernelGGL((gather_spread_kernel<NS>), dim3((unsigned)blocks), dim3(64 * kGatherWaves), smem, s, a);
+    const size_t smem = gather_lds_bytes<R>(NS);
+    allow_lds(gather_spread_kernel<R, NS>, smem);
+    hipLaunchKernelGGL((gather_spread_kernel<R, NS>), dim3((unsigned)blocks), dim3(64 * kGatherWaves), smem, s, a);
 }
 
-void launch_gather(int ns, const GatherArgs& a, int64_t blocks, hipStream_t s)
+template <typename R> void launch_gather(int ns, const GatherArgs<R>& a, int64_t blocks, hipStream_t s)
 {
     switch (ns) {
 #define HASTY_GATHER_CASE(K) \
-    case K: launch_gather_ns<K>(a, blocks, s); break;
+    case K: launch_gather_ns<R, K>(a, blocks, s); break;
         HASTY_GATHER_CASE(2)
         HASTY_GATHER_CASE(3)
         HASTY_GATHER_CASE(4)
@@ -1140,7 +1155,7 @@ void dispatch_ns(bool spread, int ns, const SpreadArgs<R>& a, int64_t n, size_t 
 // neighbours of a core are distinct cores)
 bool gather_fits(const NufftPlan* p)
 {
-    if (p->fp64 || p->type != 1 || p->dim != 3) return false;
+    if (p->type != 1 || p->dim != 3) return false;
     for (int d = 0; d < 3; ++d)
         if (p->nf[d] % kGC != 0 || p->nf[d] < 3 * kGC) return false;
     return true;
@@ -1447,12 +1462,12 @@ template <typename R> const R* phi(const NufftPlan* p, int d) { return reinterpr
 // for the current points. The pair counts come back to the host (one synchronisation)
 // to size the pair list, place each (core, neighbour) run, order the cores and split
 // the crowded ones into parts.
-void gather_build(NufftPlan* p, hipStream_t s)
+template <typename R> void gather_build(NufftPlan* p, hipStream_t s)
 {
-    GatherBins g;
+    GatherBins<R> g;
     int nb[3];
     for (int d = 0; d < 3; ++d) {
-        g.pts[d] = reinterpret_cast<const float*>(p->pts[d]);
+        g.pts[d] = reinterpret_cast<const R*>(p->pts[d]);
         g.nf[d] = p->nf[d];
         nb[d] = g.nb[d] = (int)(p->nf[d] / kGC);
     }
@@ -1474,7 +1489,7 @@ void gather_build(NufftPlan* p, hipStream_t s)
         p->g_frac = nullptr;
         p->g_idx = p->g_rank = nullptr;
         p->g_l0 = (int*)dev_alloc(sizeof(int) * 3 * p->M);
-        p->g_frac = (float*)dev_alloc(sizeof(float) * 3 * p->M);
+        p->g_frac = dev_alloc(sizeof(R) * 3 * p->M);
         p->g_idx = (int*)dev_alloc(sizeof(int) * p->M);
         p->g_rank = (int*)dev_alloc(sizeof(int) * p->M);
         p->g_cap_M = p->M;
@@ -1482,17 +1497,17 @@ void gather_build(NufftPlan* p, hipStream_t s)
     const unsigned pblocks = (unsigned)ceil_div(std::max<int64_t>(p->M, 1), kThreads);
     launch_fill(p->g_counts, ncore, 0, s);
     if (p->M > 0) {
-        hipLaunchKernelGGL(gather_count_kernel, dim3(pblocks), dim3(kThreads), 0, s, g, p->g_counts, p->g_rank);
+        hipLaunchKernelGGL((gather_count_kernel<R>), dim3(pblocks), dim3(kThreads), 0, s, g, p->g_counts, p->g_rank);
         check_launch("gather_count_kernel");
     }
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, (const int*)p->g_counts, p->g_binstart, ncore);
     check_launch("scan_kernel");
     if (p->M > 0) {
-        hipLaunchKernelGGL(gather_scatter_kernel, dim3(pblocks), dim3(kThreads), 0, s, g, (const int*)p->g_binstart,
-                           (const int*)p->g_rank, p->g_l0, p->g_frac, p->g_idx);
+        hipLaunchKernelGGL((gather_scatter_kernel<R>), dim3(pblocks), dim3(kThreads), 0, s, g,
+                           (const int*)p->g_binstart, (const int*)p->g_rank, p->g_l0, (R*)p->g_frac, p->g_idx);
         check_launch("gather_scatter_kernel");
     }
-    hipLaunchKernelGGL((gather_pairs_kernel<true>), dim3((unsigned)(27 * ncore)), dim3(kThreads), 0, s, g,
+    hipLaunchKernelGGL((gather_pairs_kernel<R, true>), dim3((unsigned)(27 * ncore)), dim3(kThreads), 0, s, g,
                        (const int*)p->g_binstart, (const int*)p->g_l0, p->g_pcount, (const int*)nullptr, (int*)nullptr,
                        (int*)nullptr);
     check_launch("gather_pairs_kernel");
@@ -1519,7 +1534,7 @@ void gather_build(NufftPlan* p, hipStream_t s)
     for (int64_t i = 0; i < 27 * ncore; ++i) start27[(size_t)i + 1] = start27[(size_t)i] + cnt27[(size_t)i];
     HASTY_HIP_CHECK(hipMemcpyAsync(p->g_pstart, start27.data(), sizeof(int) * start27.size(), hipMemcpyHostToDevice, s));
     if (npairs > 0) {
-        hipLaunchKernelGGL((gather_pairs_kernel<false>), dim3((unsigned)(27 * ncore)), dim3(kThreads), 0, s, g,
+        hipLaunchKernelGGL((gather_pairs_kernel<R, false>), dim3((unsigned)(27 * ncore)), dim3(kThreads), 0, s, g,
                            (const int*)p->g_binstart, (const int*)p->g_l0, (int*)nullptr, (const int*)p->g_pstart,
                            p->g_pairs, p->g_prel);
         check_launch("gather_pairs_kernel");
@@ -1534,7 +1549,7 @@ void gather_build(NufftPlan* p, hipStream_t s)
         list.push_back(0);
     }
     for (int ci : order)
-        for (int k = 1; k <= (int)ceil_div(std::max(0, cnt[ci] - kGatherPart), kGatherRest); ++k) {
+        for (int k = 1; k <= (int)ceil_div(std::max(0, cnt[ci] - gather_part<R>()), gather_part<R>()); ++k) {
             list.push_back(ci);
             list.push_back(k);
         }
@@ -1552,10 +1567,10 @@ void gather_build(NufftPlan* p, hipStream_t s)
     p->gather_built = 1;
 }
 
-void gather_spread(NufftPlan* p, const float2* c, float2* fine, hipStream_t s)
+template <typename R> void gather_spread(NufftPlan* p, const cplx<R>* c, cplx<R>* fine, hipStream_t s)
 {
-    GatherArgs a;
-    a.frac = p->g_frac;
+    GatherArgs<R> a;
+    a.frac = reinterpret_cast<const R*>(p->g_frac);
     a.idx = p->g_idx;
     a.pstart = p->g_pstart;
     a.pairs = p->g_pairs;
@@ -1568,16 +1583,16 @@ void gather_spread(NufftPlan* p, const float2* c, float2* fine, hipStream_t s)
         a.nb[d] = (int)(p->nf[d] / kGC);
         a.nf[d] = p->nf[d];
     }
-    a.beta = (float)p->beta;
-    a.cfac = (float)(4.0 / ((double)p->ns * p->ns));
-    a.rp = gather_row_pitch(p->ns);
+    a.beta = (R)p->beta;
+    a.cfac = (R)(4.0 / ((double)p->ns * p->ns));
+    a.rp = gather_row_pitch<R>(p->ns);
     a.rest = 0;
     KernelTimer t("nufft_spread", s);
-    launch_gather(p->ns, a, p->g_first, s);
+    launch_gather<R>(p->ns, a, p->g_first, s);
     if (p->g_rest > 0) {
         a.list = p->g_list + 2 * p->g_first;
         a.rest = 1;
-        launch_gather(p->ns, a, p->g_rest, s);
+        launch_gather<R>(p->ns, a, p->g_rest, s);
     }
 }
 
@@ -1659,18 +1674,18 @@ template <typename R> void execute_type1(NufftPlan* p, const cplx<R>* c, cplx<R>
     tile_pitches(T, p->ns, (int)sizeof(cplx<R>), pad_tile(p), rp, pp);
     const size_t smem = sizeof(cplx<R>) * (size_t)T[0] * pp;
     HASTY_REQUIRE(smem <= 160 * 1024, "spreading tile exceeds LDS");
-    const bool gather = std::is_same<R, float>::value && spread_variant(p) == 6;
+    const bool gather = spread_variant(p) == 6;
     if (gather && !p->gather_built) {
         HASTY_REQUIRE(!stream_capturing(s),
                       "the output-driven spreader bins the points on its first execute; run one before capturing");
-        gather_build(p, s);
+        gather_build<R>(p, s);
     }
     for (int t = 0; t < p->ntransf; ++t) {
         const cplx<R>* ct = c + t * p->M;
         cplx<R>* ft = f + t * nvox;
         if (gather) {
             // writes every fine-grid cell: no zero pass
-            gather_spread(p, reinterpret_cast<const float2*>(ct), reinterpret_cast<float2*>(fine), s);
+            gather_spread<R>(p, ct, fine, s);
         } else {
             zero_fill<R>(fine, f0 * f1 * f2, s);
             if (p->nsub > 0) {
@@ -1781,7 +1796,12 @@ void nufft_setpts(NufftPlan* p, int64_t M, const void* x, const void* y, const v
         setpts_typed<double>(p, s);
     else
         setpts_typed<float>(p, s);
-    if (spread_variant(p) == 6) gather_build(p, s);
+    if (spread_variant(p) == 6) {
+        if (p->fp64)
+            gather_build<double>(p, s);
+        else
+            gather_build<float>(p, s);
+    }
 }
 
 bool nufft_gather_fits(const NufftPlan* p) { return gather_fits(p); }

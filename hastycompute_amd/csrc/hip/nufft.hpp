@@ -41,13 +41,13 @@ struct NufftPlan {
     int spread_variant = -1;  // 3-D spreader of this plan (-1: the library option)
     int locality = 3;         // bit 0 super-tile bin order, bit 1 XCD-aware blocks (fixed at makeplan)
     int chunk = 256;          // points per spreader subproblem at most (fixed at makeplan)
-    // output-driven 3-D spreader (variant 6, complex64): points binned by 16^3-cell cores,
+    // output-driven 3-D spreader (variant 6): points binned by 16^3-cell cores,
     // built on the first execute after setpts
     int gather_built = 0;
     int* g_counts = nullptr;    // points per core bin
     int* g_binstart = nullptr;  // core bins + 1 (scan)
     int* g_l0 = nullptr;        // [3][M] first window cell per dim, in core-bin order
-    float* g_frac = nullptr;    // [3][M] fine-grid position minus that cell
+    void* g_frac = nullptr;     // [3][M] fine-grid position minus that cell (plan precision)
     int* g_idx = nullptr;       // [M] point index, in core-bin order
     int* g_rank = nullptr;      // [M] rank within the core bin
     int* g_pcount = nullptr;    // (point, core) pairs per (core, neighbour bin): 27 per core
@@ -75,9 +75,9 @@ void nufft_destroy(NufftPlan* p);
 // 1 one wave per tile, 2 four waves and LDS atomics, 3 four waves clearing and
 // flushing the tile while the first wave accumulates without atomics; 4 and 5 four and
 // eight waves each owning the tile planes congruent to its index; 6 (default) output-driven:
-// one workgroup per 16^3-cell core gathers the points of the 27 neighbouring cores and
-// writes the core once (complex64 plans whose fine-grid dims are multiples of 16 and at
-// least 48; other plans take 3)
+// one workgroup per 16^3-cell core spreads the points whose windows overlap it and writes
+// the core once (plans whose fine-grid dims are multiples of 16 and at least 48; other
+// plans take 3)
 void nufft_set_spread_variant(int v);
 // whether the output-driven spreader (variant 6) serves the plan
 bool nufft_gather_fits(const NufftPlan* p);

@@ -147,9 +147,8 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
  *     accumulates without atomics, 4 and 5 four and eight waves each owning the tile
  *     planes congruent to its index, 6 output-driven: one workgroup per 16^3-cell core
  *     of the fine grid spreads the points whose windows overlap it and stores the core
- *     once (no zero pass, no global atomics outside crowded cores; complex64 plans whose
- *     fine-grid dims are multiples of 16 and at least 48, binned at setpts; other plans
- *     run 3).
+ *     once (no zero pass, no global atomics outside crowded cores; plans whose fine-grid
+ *     dims are multiples of 16 and at least 48, binned at setpts; other plans run 3).
  *   "fft_r16" 0 or 1 (default 1): complex64 axis FFT passes of length 128, 192, 256 or
  *     320 (the NUFFT at 64^3 .. 160^3 modes) without coil multipliers, accumulation or
  *     convolution run as a 16 x R four-step FFT in registers; 0 keeps them on the LDS
@@ -157,7 +156,7 @@ int hasty_sense_normal_toeplitz_ex(const void* x, void* out, const void* smaps, 
  *   "fft_strided_lines" 0..64 (default 0 = auto): lines per workgroup of the axis FFT
  *     passes whose lines are strided (the NUFFT's outer axes, the Toeplitz kernel build,
  *     the generic normal operator); auto gives every pass 2048 / L (complex64) or
- *     1024 / L (complex128) lines, 1..64.
+ *     1024 / L (complex128) lines, 1..64, and strided complex128 passes at least 4.
  * Returns 0, or 1 for an unknown name or a value out of range. */
 int hasty_set_option(const char* name, int64_t value);
 /* Current value of an option of hasty_set_option (0, or 1 for an unknown name). */

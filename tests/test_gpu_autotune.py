@@ -122,7 +122,7 @@ def test_nufft_spread_tile_options_vs_direct_sum(N, restore_options, tile, chunk
 
 @pytest.mark.parametrize("fp64", [False, True])
 def test_fft_strided_lines_same_bits(N, restore_options, fp64):
-    # the lines per workgroup of the strided axis FFT passes (1, 8, auto = 8 or 4, 64,
+    # the lines per workgroup of the strided axis FFT passes (1, 8, auto, 64,
     # which the LDS limit halves) only regroup independent lines: type-2 NUFFTs (FFT
     # passes, then a gather) give the same bits for every value; type 1 and the FP64
     # Toeplitz kernel build, whose tile flushes add with global atomics in no fixed

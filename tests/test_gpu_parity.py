@@ -183,16 +183,23 @@ def test_nufft_type1_spread_variants_fp64_and_wide(N, variant, fp64, tol, bound)
     assert ns >= 9 and e < bound
 
 
-@pytest.mark.parametrize("n,traj,tol,bound", [(24, "kooshball", 1e-6, 1e-5), (24, "pileup", 1e-6, 1e-5),
-                                              (32, "uniform", 1e-6, 1e-5), (40, "kooshball", 1e-8, 1e-6),
-                                              (24, "pileup", 1e-3, 1e-2), (32, "kooshball", 1e-12, 1e-5)])
-def test_nufft_type1_gather_spreader(N, n, traj, tol, bound):
-    # The output-driven spreader (variant 6): 16^3-cell cores gathering from their 27
-    # neighbours, fine grids 48^3 (three cores per dim: every neighbour of a core at the
-    # edge is a periodic image), 64^3 and 80^3; the kooshball and the pileup (1500 points
-    # on one position) make cores with more than 128 window-overlapping points, whose further
-    # parts go through the second, atomic launch; ns from 4 to 16. Against the direct sum,
-    # with two transforms per plan (the core bins are built once), and against the tile
+@pytest.mark.parametrize("n,traj,tol,bound,fp64", [(24, "kooshball", 1e-6, 1e-5, False),
+                                                   (24, "pileup", 1e-6, 1e-5, False),
+                                                   (32, "uniform", 1e-6, 1e-5, False),
+                                                   (40, "kooshball", 1e-8, 1e-6, False),
+                                                   (24, "pileup", 1e-3, 1e-2, False),
+                                                   (32, "kooshball", 1e-12, 1e-5, False),
+                                                   (24, "kooshball", 1e-10, 1e-8, True),
+                                                   (24, "pileup", 1e-10, 1e-8, True),
+                                                   (32, "uniform", 1e-8, 1e-7, True)])
+def test_nufft_type1_gather_spreader(N, n, traj, tol, bound, fp64):
+    # The output-driven spreader (variant 6): 16^3-cell cores, fine grids 48^3 (three
+    # cores per dim: every neighbour of a core at the edge is a periodic image), 64^3 and
+    # 80^3; the kooshball and the pileup (1500 points on one position) make cores with
+    # more (point, core) pairs than one part holds, whose further parts go through the
+    # second, atomic launch; ns from 4 to 16; complex64 (128-point parts) and complex128
+    # (32-point parts, the FP64 Toeplitz-kernel build's spreader). Against the direct sum,
+    # with two transforms per plan (the core lists are built once), and against the tile
     # spreader (variant 3) to rounding.
     from hastycompute_amd import _capi
 
@@ -205,7 +212,8 @@ def test_nufft_type1_gather_spreader(N, n, traj, tol, bound):
         p0 = rng.uniform(-np.pi, np.pi, (3, 1))
         co = np.concatenate([np.repeat(p0, 1500, axis=1), p0 + rng.uniform(-0.3, 0.3, (3, 3000)),
                              rng.uniform(-np.pi, np.pi, (3, 2000))], axis=1)
-    co32 = co.astype(np.float32).astype(np.float64)
+    rt, ct = (torch.float64, torch.complex128) if fp64 else (torch.float32, torch.complex64)
+    co_used = co if fp64 else co.astype(np.float32).astype(np.float64)
     M = co.shape[1]
     y = rnd(rng, (2, M))
     prev = _capi.get_option("nufft_spread_variant")
@@ -213,20 +221,21 @@ def test_nufft_type1_gather_spreader(N, n, traj, tol, bound):
     try:
         for v in (6, 3):
             _capi.set_option("nufft_spread_variant", v)
-            plan = N.NufftPlan(g(co, torch.float32), [n, n, n], 1, +1, 2, tol)
+            plan = N.NufftPlan(g(co, rt), [n, n, n], 1, +1, 2, tol)
             nf, ns, _ = plan.info()
-            img = torch.empty((2, n, n, n), dtype=torch.complex64, device="cuda")
-            plan.execute(g(y), img)
+            img = torch.empty((2, n, n, n), dtype=ct, device="cuda")
+            plan.execute(g(y, ct), img)
             outs[v] = h(img)
             plan.close()
     finally:
         _capi.set_option("nufft_spread_variant", prev)
     assert all(f % 16 == 0 and f >= 48 for f in nf)
     for t in range(2):
-        e = O.rel_l2(outs[6][t], O.nudft_type1(co32, y[t], (n, n, n), +1))
+        e = O.rel_l2(outs[6][t], O.nudft_type1(co_used, y[t], (n, n, n), +1))
         e3 = O.rel_l2(outs[6][t], outs[3][t])
-        print(f"gather n {n} {traj} tol {tol} (ns {ns}) transform {t}: {e:.2e}, vs tiles {e3:.1e}", flush=True)
-        assert e < bound and e3 < 1e-5
+        print(f"gather n {n} {traj} tol {tol} (ns {ns}, fp64 {fp64}) transform {t}: {e:.2e}, vs tiles {e3:.1e}",
+              flush=True)
+        assert e < bound and e3 < (1e-12 if fp64 else 1e-5)
 
 
 @pytest.mark.parametrize("n", [24, 20])
