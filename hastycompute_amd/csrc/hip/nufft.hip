@@ -204,29 +204,49 @@ __global__ void subfill_kernel(const int* counts, const int* nsubstart, int* sub
 }
 
 // Exclusive scan of n ints into out[0..n] (out[n] = total), one workgroup.
+// Exclusive prefix sum of n ints into out[0..n], out[n] the total, by one workgroup of
+// 1024 threads walking tiles of 4096: each thread sums 4 consecutive elements (coalesced
+// loads), the waves scan their sums with shuffles, wave totals go through LDS, and a
+// running carry links the tiles.
 __global__ void __launch_bounds__(1024) scan_kernel(const int* in, int* out, int64_t n)
 {
-    __shared__ int64_t part[1024];
-    const int tid = threadIdx.x;
-    const int64_t chunk = (n + 1023) / 1024;
-    const int64_t b0 = tid * chunk;
-    const int64_t b1 = min(n, b0 + chunk);
-    int64_t s = 0;
-    for (int64_t i = b0; i < b1; ++i) s += in[i];
-    part[tid] = s;
+    __shared__ int wsum[16];
+    __shared__ int carry_s;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) carry_s = 0;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int64_t v = tid >= off ? part[tid - off] : 0;
+    for (int64_t t0 = 0; t0 < n; t0 += 4096) {
+        const int64_t b = t0 + 4 * (int64_t)tid;
+        int v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = b + k < n ? in[b + k] : 0;
+        const int mine = v[0] + v[1] + v[2] + v[3];
+        int inc = mine;  // inclusive scan over the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o);
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        const int carry = carry_s;
         __syncthreads();
-        part[tid] += v;
+        int wbase = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < 16; ++w) {
+            wbase += w < wave ? wsum[w] : 0;
+            total += wsum[w];
+        }
+        int run = carry + wbase + inc - mine;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (b + k < n) out[b + k] = run;
+            run += v[k];
+        }
+        __syncthreads();  // every thread has read carry_s and wsum
+        if (tid == 0) carry_s = carry + total;
         __syncthreads();
     }
-    int64_t run = part[tid] - s;
-    for (int64_t i = b0; i < b1; ++i) {
-        out[i] = (int)run;
-        run += in[i];
-    }
-    if (tid == 1023) out[n] = (int)part[1023];
+    if (tid == 0) out[n] = carry_s;
 }
 
 template <typename R> __global__ void zero_kernel(cplx<R>* p, int64_t n)
