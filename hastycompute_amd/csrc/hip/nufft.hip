@@ -727,17 +727,16 @@ __global__ void gather_scatter_kernel(GatherBins<R> g, const int* binstart, cons
 
 // (point, core) pairs of core ci with neighbour k of its 27 (periodic images included):
 // the points of that neighbouring core bin whose windows overlap the core, with the
-// window corner relative to the core packed 8 bits per dim (+16). One workgroup per
+// window corner relative to the core packed 8 bits per dim (+16). One wave per
 // (core, neighbour), no atomics: COUNT writes the count to pcount[27 ci + k], else the
 // pairs go to pstart[27 ci + k] on in bin order (ballot compaction), so a core's pairs
 // are contiguous and their order is fixed by the core-bin order of the points.
 template <typename R, bool COUNT>
-__global__ void __launch_bounds__(kThreads) gather_pairs_kernel(GatherBins<R> g, const int* binstart, const int* l0,
-                                                                 int* pcount, const int* pstart, int* pairs, int* prel)
+__global__ void __launch_bounds__(64) gather_pairs_kernel(GatherBins<R> g, const int* binstart, const int* l0,
+                                                          int* pcount, const int* pstart, int* pairs, int* prel)
 {
-    __shared__ int wave_cnt[kThreads / 64];
     const int ci = blockIdx.x / 27, k = blockIdx.x % 27;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lane = threadIdx.x;
     const int cb[3] = {ci / (g.nb[2] * g.nb[1]), (ci / g.nb[2]) % g.nb[1], ci % g.nb[2]};
     const int dd[3] = {k / 9 - 1, (k / 3) % 3 - 1, k % 3 - 1};
     int bn[3], rel0[3];
@@ -758,8 +757,8 @@ __global__ void __launch_bounds__(kThreads) gather_pairs_kernel(GatherBins<R> g,
     const int bin = (bn[0] * g.nb[1] + bn[1]) * g.nb[2] + bn[2];
     const int b0 = binstart[bin], V = binstart[bin + 1] - b0;
     int out = COUNT ? 0 : pstart[blockIdx.x];
-    for (int v0 = 0; v0 < V; v0 += kThreads) {
-        const int v = v0 + tid;
+    for (int v0 = 0; v0 < V; v0 += 64) {
+        const int v = v0 + lane;
         bool acc = false;
         int p = 0, rel = 0;
         if (v < V) {
@@ -773,23 +772,21 @@ __global__ void __launch_bounds__(kThreads) gather_pairs_kernel(GatherBins<R> g,
             }
         }
         const unsigned long long m = __ballot(acc);
-        if (lane == 0) wave_cnt[wave] = __popcll(m);
-        __syncthreads();
-        int base = 0, total = 0;
-#pragma unroll
-        for (int w = 0; w < kThreads / 64; ++w) {
-            base += w < wave ? wave_cnt[w] : 0;
-            total += wave_cnt[w];
-        }
         if (!COUNT && acc) {
-            const int e = out + base + __popcll(m & ((1ull << lane) - 1ull));
+            const int e = out + __popcll(m & ((1ull << lane) - 1ull));
             pairs[e] = p;
             prel[e] = rel;
         }
-        out += total;
-        __syncthreads();  // wave_cnt is rewritten by the next batch
+        out += __popcll(m);
     }
-    if (COUNT && tid == 0) pcount[blockIdx.x] = out;
+    if (COUNT && lane == 0) pcount[blockIdx.x] = out;
+}
+
+// pairs per core from the scanned (core, neighbour) starts
+__global__ void core_pairs_kernel(const int* pstart, int* cnt, int64_t ncore)
+{
+    const int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (c < ncore) cnt[c] = pstart[27 * c + 27] - pstart[27 * c];
 }
 
 template <typename R> struct GatherArgs {
@@ -1527,19 +1524,21 @@ template <typename R> void gather_build(NufftPlan* p, hipStream_t s)
                            (const int*)p->g_binstart, (const int*)p->g_rank, p->g_l0, (R*)p->g_frac, p->g_idx);
         check_launch("gather_scatter_kernel");
     }
-    hipLaunchKernelGGL((gather_pairs_kernel<R, true>), dim3((unsigned)(27 * ncore)), dim3(kThreads), 0, s, g,
+    hipLaunchKernelGGL((gather_pairs_kernel<R, true>), dim3((unsigned)(27 * ncore)), dim3(64), 0, s, g,
                        (const int*)p->g_binstart, (const int*)p->g_l0, p->g_pcount, (const int*)nullptr, (int*)nullptr,
                        (int*)nullptr);
     check_launch("gather_pairs_kernel");
-    std::vector<int> cnt27((size_t)(27 * ncore));
-    HASTY_HIP_CHECK(hipMemcpyAsync(cnt27.data(), p->g_pcount, sizeof(int) * 27 * ncore, hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, (const int*)p->g_pcount, p->g_pstart, 27 * ncore);
+    check_launch("scan_kernel");
+    // per-core totals (into the bin counts, no longer needed) to the host
+    hipLaunchKernelGGL(core_pairs_kernel, dim3((unsigned)ceil_div(ncore, kThreads)), dim3(kThreads), 0, s,
+                       (const int*)p->g_pstart, p->g_counts, ncore);
+    check_launch("core_pairs_kernel");
+    std::vector<int> cnt((size_t)ncore);
+    HASTY_HIP_CHECK(hipMemcpyAsync(cnt.data(), p->g_counts, sizeof(int) * ncore, hipMemcpyDeviceToHost, s));
     HASTY_HIP_CHECK(hipStreamSynchronize(s));
-    std::vector<int> cnt((size_t)ncore, 0);
     int64_t npairs = 0;
-    for (int64_t i = 0; i < 27 * ncore; ++i) {
-        cnt[(size_t)(i / 27)] += cnt27[(size_t)i];
-        npairs += cnt27[(size_t)i];
-    }
+    for (int v : cnt) npairs += v;
     HASTY_REQUIRE(npairs < (int64_t)1 << 31, "too many points for the output-driven spreader");
     if (npairs > p->g_cap_pairs) {
         dev_free(p->g_pairs);
@@ -1549,12 +1548,8 @@ template <typename R> void gather_build(NufftPlan* p, hipStream_t s)
         p->g_prel = (int*)dev_alloc(sizeof(int) * npairs);
         p->g_cap_pairs = npairs;
     }
-    std::vector<int> start27((size_t)(27 * ncore + 1));
-    start27[0] = 0;
-    for (int64_t i = 0; i < 27 * ncore; ++i) start27[(size_t)i + 1] = start27[(size_t)i] + cnt27[(size_t)i];
-    HASTY_HIP_CHECK(hipMemcpyAsync(p->g_pstart, start27.data(), sizeof(int) * start27.size(), hipMemcpyHostToDevice, s));
     if (npairs > 0) {
-        hipLaunchKernelGGL((gather_pairs_kernel<R, false>), dim3((unsigned)(27 * ncore)), dim3(kThreads), 0, s, g,
+        hipLaunchKernelGGL((gather_pairs_kernel<R, false>), dim3((unsigned)(27 * ncore)), dim3(64), 0, s, g,
                            (const int*)p->g_binstart, (const int*)p->g_l0, (int*)nullptr, (const int*)p->g_pstart,
                            p->g_pairs, p->g_prel);
         check_launch("gather_pairs_kernel");

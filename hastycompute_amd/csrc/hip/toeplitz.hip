@@ -150,10 +150,6 @@ void toeplitz_build(const void* coords, int coord_fp64, int64_t M, int ndim, con
     int64_t nm_flip[3];
     for (int d = 0; d < ndim; ++d) nm_flip[d] = 2 * nmodes[ndim - 1 - d];
     NufftPlan* plan = nufft_makeplan(1, ndim, nm_flip, 1, 1, tol, 1);
-    // A plan used once: the tile spreader. The output-driven one spreads this PSF faster
-    // (3.2 against 7.6 ms at 128^3) but its per-core point lists cost as much again at
-    // setpts on the 512^3 fine grid.
-    plan->spread_variant = 3;
     DevBuf psf(sizeof(double2) * ngrid);
     try {
         const double* row[3] = {nullptr, nullptr, nullptr};
