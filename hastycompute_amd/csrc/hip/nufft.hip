@@ -1688,8 +1688,8 @@ template <typename R> void execute_type1(NufftPlan* p, const cplx<R>* c, cplx<R>
     int rp = 0, pp = 0;
     tile_pitches(T, p->ns, (int)sizeof(cplx<R>), pad_tile(p), rp, pp);
     const size_t smem = sizeof(cplx<R>) * (size_t)T[0] * pp;
-    HASTY_REQUIRE(smem <= 160 * 1024, "spreading tile exceeds LDS");
     const bool gather = spread_variant(p) == 6;
+    HASTY_REQUIRE(gather || smem <= 160 * 1024, "spreading tile exceeds LDS");
     if (gather && !p->gather_built) {
         HASTY_REQUIRE(!stream_capturing(s),
                       "the output-driven spreader bins the points on its first execute; run one before capturing");

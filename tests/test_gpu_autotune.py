@@ -120,6 +120,29 @@ def test_nufft_spread_tile_options_vs_direct_sum(N, restore_options, tile, chunk
     assert e < 1e-5
 
 
+def test_tile_option_does_not_bind_the_output_driven_spreader(N, restore_options):
+    # a 28-cell tile (28^3 complex64 = 172 KB, more than a CU's LDS) is refused by the tile
+    # spreaders on the host, before any launch, but does not concern the output-driven
+    # spreader, which has no tile
+    from hastycompute_amd import _capi as C
+
+    n = 24
+    co = O.kooshball(100, 48)
+    M = co.shape[1]
+    rng = np.random.default_rng(28)
+    y = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    C.set_option("nufft_spread_tile", 28)
+    img = N.nufft(torch.as_tensor(co.astype(np.float32)).cuda(), torch.as_tensor(y[None]).to(torch.complex64).cuda(),
+                  [n, n, n], 1, +1, 1e-6)
+    ref = O.nudft_type1(co.astype(np.float32).astype(np.float64), y, (n, n, n), +1)
+    assert O.rel_l2(img.cpu().numpy()[0].astype(np.complex128), ref) < 1e-5
+    C.set_option("nufft_spread_variant", 3)
+    with pytest.raises(Exception, match="exceeds LDS"):
+        N.nufft(torch.as_tensor(co.astype(np.float32)).cuda(), torch.as_tensor(y[None]).to(torch.complex64).cuda(),
+                [n, n, n], 1, +1, 1e-6)
+    torch.cuda.synchronize()
+
+
 @pytest.mark.parametrize("fp64", [False, True])
 def test_fft_strided_lines_same_bits(N, restore_options, fp64):
     # the lines per workgroup of the strided axis FFT passes (1, 8, auto, 64,
