@@ -29,7 +29,8 @@ KERNEL_NAMES = {"fwd_xy_kernel": "sense_fwd_xy", "inv_xy_kernel": "sense_inv_xy"
                 "z_conv_ct_kernel": "sense_z_conv", "z_conv_kernel": "sense_z_conv", "spread_kernel": "nufft_spread",
                 "spread_rows_kernel": "nufft_spread", "spread_planes_kernel": "nufft_spread",
                 "gather_spread_kernel": "nufft_spread", "zero_kernel": "fill_zero",
-                "interp_kernel": "nufft_interp", "axis_fft_kernel": "fft_axis", "gram_kernel": "llr_gram",
+                "interp_kernel": "nufft_interp", "axis_fft_kernel": "fft_axis", "axis_fft_r16_kernel": "fft_axis",
+                "gram_kernel": "llr_gram",
                 "eig_w_kernel": "llr_eig", "apply_kernel": "llr_apply"}
 
 
@@ -63,7 +64,8 @@ KERNEL_SOURCES = {"fwd_xy_r16_kernel": "sense_fused.hip", "inv_xy_r16_kernel": "
                   "spread_kernel": "nufft.hip", "interp_kernel": "nufft.hip",
                   "spread_rows_kernel": "nufft.hip", "spread_planes_kernel": "nufft.hip",
                   "gather_spread_kernel": "nufft.hip", "zero_kernel": "nufft.hip",
-                  "axis_fft_kernel": "fft.hip", "gram_kernel": "llr.hip", "eig_w_kernel": "llr.hip",
+                  "axis_fft_kernel": "fft.hip", "axis_fft_r16_kernel": "fft.hip", "gram_kernel": "llr.hip",
+                  "eig_w_kernel": "llr.hip",
                   "apply_kernel": "llr.hip"}
 SRC_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "hip")
 
