@@ -191,3 +191,48 @@ def test_llr_prox_graph_replays_equal_eager(N):
     torch.cuda.synchronize()
     assert torch.count_nonzero(y[:, :, :8, :8, :8]) == 0
     assert torch.equal(y[:, :, 8:], before[:, :, 8:])
+
+
+def test_nufft_pair_graph_replays_equal_eager(N):
+    # a type-2 and a type-1 NUFFT (64^3 modes: register FFT passes, the output-driven
+    # spreader with a second launch for the crowded k-space centre) recorded as one HIP
+    # graph after setpts; before each replay the outputs are overwritten, so a replay must
+    # write every cell (no zero pass); type 2 equals the eager result bit for bit, type 1
+    # (atomic adds of the crowded cores' further parts, in no fixed order) to rounding
+    from oracle import oracle as O
+    import numpy as np
+
+    n = 64
+    co = torch.as_tensor(O.kooshball(300, 2 * n).astype(np.float32)).cuda()
+    M = co.shape[1]
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(21)
+    f = torch.randn((1, n, n, n), dtype=torch.complex64, device="cuda", generator=gen)
+    c = torch.randn((1, M), dtype=torch.complex64, device="cuda", generator=gen)
+    p2 = N.NufftPlan(co, [n, n, n], 2, -1, 1, 1e-6)
+    p1 = N.NufftPlan(co, [n, n, n], 1, +1, 1, 1e-6)
+    kout = torch.empty((1, M), dtype=torch.complex64, device="cuda")
+    img = torch.empty((1, n, n, n), dtype=torch.complex64, device="cuda")
+
+    def record():
+        p2.execute(kout, f)
+        p1.execute(c, img)
+
+    record()
+    torch.cuda.synchronize()
+    ref2, ref1 = kout.clone(), img.clone()
+    gr = N.Graph(record)
+    for _ in range(3):
+        kout.fill_(3.0)
+        img.fill_(7.0)
+        p1.execute(c, torch.empty_like(img))  # eager work between the replays
+        torch.cuda.synchronize()
+        gr.replay()
+        gr.stream.synchronize()
+        torch.cuda.synchronize()
+        assert torch.equal(kout, ref2)
+        e = (torch.linalg.vector_norm(img - ref1) / torch.linalg.vector_norm(ref1)).item()
+        assert e < 1e-6, e
+    gr.close()
+    p1.close()
+    p2.close()
