@@ -1041,6 +1041,25 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(SpreadArgs<R> a)
     }
     R sr = 0, si = 0;
     if (DIM == 3) {
+        // complex64 rows that do not wrap: the window's cells from the even cell at or
+        // below its first one, read as 16-byte pairs (W / 2 loads instead of NS) against
+        // the weights shifted by that offset
+        constexpr int W = (NS + 2) & ~1;
+        R w8[W];
+        bool pairs = false;
+        int64_t a2 = 0;
+        if constexpr (std::is_same<R, float>::value) {
+            const int off = (int)(idx[1][0] & 1);
+            a2 = idx[1][0] - off;
+            pairs = a2 + W <= pa.nf[2] && idx[1][NS - 1] == idx[1][0] + NS - 1;
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                R v = (R)0;
+#pragma unroll
+                for (int q = 0; q < NS; ++q) v = (k == q + off) ? w[1][q] : v;
+                w8[k] = v;
+            }
+        }
         for (int ia = 0; ia < NS; ++ia) {
             const R wa = es_eval<R>((R)(l00 + ia - t0), a.beta, a.cfac);
             const int64_t i0 = wrapi((int64_t)l00 + ia, pa.nf[0]);
@@ -1049,11 +1068,21 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(SpreadArgs<R> a)
                 const R wab = wa * w[0][ib];
                 const cplx<R>* row = a.fine + (i0 * pa.nf[1] + idx[0][ib]) * pa.nf[2];
                 R tr = 0, ti = 0;
+                if (pairs) {
+                    const float4* r4 = reinterpret_cast<const float4*>(row + a2);
 #pragma unroll
-                for (int q = 0; q < NS; ++q) {
-                    const cplx<R> g = row[idx[1][q]];
-                    tr += w[1][q] * g.x;
-                    ti += w[1][q] * g.y;
+                    for (int k = 0; k < W / 2; ++k) {
+                        const float4 g = r4[k];
+                        tr += w8[2 * k] * g.x + w8[2 * k + 1] * g.z;
+                        ti += w8[2 * k] * g.y + w8[2 * k + 1] * g.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) {
+                        const cplx<R> g = row[idx[1][q]];
+                        tr += w[1][q] * g.x;
+                        ti += w[1][q] * g.y;
+                    }
                 }
                 sr += wab * tr;
                 si += wab * ti;
