@@ -635,14 +635,14 @@ __global__ void __launch_bounds__(64 * NW) spread_planes_kernel(SpreadArgs<R> a)
 // The fine grid is cut into cores of kGC^3 cells (kGC divides every fine-grid dim). A
 // window of ns <= 16 cells overlaps one or two cores per dim, so each point belongs to at
 // most 8 cores; setpts lists, per core, the points whose windows overlap it (the pairs,
-// with the window corner relative to the core). The first launch gives every core one
-// workgroup that spreads up to kGatherPart of its points into an LDS copy of the core and
-// then stores the whole core, zeros included: each fine-grid cell is written once with
-// plain, whole-line stores, and neither a zero pass nor global atomics are needed. A
-// crowded core (the k-space centre of a radial trajectory) has its further points split
-// into parts of kGatherRest for a second launch, whose workgroups add their LDS cores to
-// the grid with atomics. Cores are listed most-crowded first, so the long ones start
-// first.
+// with the window corner relative to the core). One launch gives every core with at most
+// kGatherPart such points one workgroup that spreads them into an LDS copy of the core
+// and then stores the whole core, zeros included: each of its fine-grid cells is written
+// once with plain, whole-line stores, and neither a zero pass nor global atomics are
+// needed. A crowded core (the k-space centre of a radial trajectory) is zeroed by a small
+// kernel beforehand and split into parts of kGatherPart points whose workgroups add their
+// LDS cores to the grid with atomics. The crowded cores' parts are listed first, most
+// crowded first, so the long ones start first.
 constexpr int kGC = 16;
 constexpr int kGatherWaves = 4;
 // points per part (staged at once), and the core's LDS row pitch: complex64 cores take
@@ -803,7 +803,7 @@ template <typename R> struct GatherArgs {
     int64_t nf[3];
     R beta, cfac;
     int rp;    // LDS row pitch of the core (float2), gather_row_pitch
-    int rest;  // second launch: add non-zero cells with atomics
+    int natomic;  // list entries [0, natomic) (the crowded cores' parts) add with atomics
 };
 
 template <typename R> __device__ __forceinline__ R gather_es(R z, R beta, R c)
@@ -819,7 +819,7 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
 {
     using C2 = cplx<R>;
     constexpr int kGatherPart = gather_part<R>();
-    constexpr int kGatherRest = gather_part<R>();
+    constexpr int kGatherRest = gather_part<R>();  // points of the parts after the first
     constexpr int NW = kGatherWaves;
     constexpr int NT = 64 * NW;
     constexpr int S = 3 * NS + 2;
@@ -962,12 +962,25 @@ __global__ void __launch_bounds__(64 * kGatherWaves) gather_spread_kernel(Gather
         const int i0 = i / (kGC * kGC), i1 = (i / kGC) % kGC, i2 = i % kGC;
         const C2 v = core[(i0 * kGC + i1) * rp + i2];
         C2* dst = a.fine + ((o0 + i0) * a.nf[1] + (o1 + i1)) * a.nf[2] + (o2 + i2);
-        if (!a.rest) {
+        if ((int)blockIdx.x >= a.natomic) {
             *dst = v;
         } else if (v.x != (R)0 || v.y != (R)0) {
             atomicAdd(&dst->x, v.x);
             atomicAdd(&dst->y, v.y);
         }
+    }
+}
+
+// zeros the cores whose parts all add with atomics (before the spreading launch)
+template <typename R> __global__ void zero_cores_kernel(cplx<R>* fine, const int* cores, int nb1, int nb2, int64_t nf1,
+                                                        int64_t nf2)
+{
+    const int ci = cores[blockIdx.x];
+    const int64_t o0 = (int64_t)(ci / (nb2 * nb1)) * kGC, o1 = (int64_t)((ci / nb2) % nb1) * kGC,
+                  o2 = (int64_t)(ci % nb2) * kGC;
+    for (int i = threadIdx.x; i < kGC * kGC * kGC; i += kThreads) {
+        const int i0 = i / (kGC * kGC), i1 = (i / kGC) % kGC, i2 = i % kGC;
+        fine[((o0 + i0) * nf1 + (o1 + i1)) * nf2 + (o2 + i2)] = cplx<R>{(R)0, (R)0};
     }
 }
 
@@ -1586,28 +1599,42 @@ template <typename R> void gather_build(NufftPlan* p, hipStream_t s)
     std::vector<int> order((size_t)ncore);
     for (int64_t i = 0; i < ncore; ++i) order[(size_t)i] = (int)i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cnt[x] > cnt[y]; });
-    std::vector<int> list;
+    // one launch: first every part of the crowded cores (more pairs than one part holds;
+    // their regions are zeroed beforehand and every part adds with atomics), most crowded
+    // first, then one part per other core, stored with plain whole-row stores
+    std::vector<int> list, crowded;
     list.reserve((size_t)ncore * 2 + 64);
     for (int ci : order) {
-        list.push_back(ci);
-        list.push_back(0);
-    }
-    for (int ci : order)
-        for (int k = 1; k <= (int)ceil_div(std::max(0, cnt[ci] - gather_part<R>()), gather_part<R>()); ++k) {
+        const int parts = (int)std::max<int64_t>(1, ceil_div(cnt[ci], gather_part<R>()));
+        if (parts < 2) continue;
+        crowded.push_back(ci);
+        for (int k = 0; k < parts; ++k) {
             list.push_back(ci);
             list.push_back(k);
         }
-    const int64_t entries = (int64_t)list.size() / 2;
-    if (entries > p->g_cap_list) {
+    }
+    const int64_t natomic = (int64_t)list.size() / 2;
+    for (int ci : order) {
+        if (cnt[ci] > gather_part<R>()) continue;
+        list.push_back(ci);
+        list.push_back(0);
+    }
+    const size_t nlist = list.size();
+    list.insert(list.end(), crowded.begin(), crowded.end());
+    const int64_t entries = (int64_t)nlist / 2;
+    if ((int64_t)list.size() > 2 * p->g_cap_list) {
         dev_free(p->g_list);
         p->g_list = nullptr;
-        p->g_list = (int*)dev_alloc(sizeof(int) * list.size());
-        p->g_cap_list = entries;
+        p->g_cap_list = 0;
+        const int64_t cap = ((int64_t)list.size() + 1) / 2;
+        p->g_list = (int*)dev_alloc(sizeof(int) * 2 * cap);
+        p->g_cap_list = cap;
     }
     HASTY_HIP_CHECK(hipMemcpyAsync(p->g_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, s));
     HASTY_HIP_CHECK(hipStreamSynchronize(s));  // the host list goes out of scope
-    p->g_first = ncore;
-    p->g_rest = entries - ncore;
+    p->g_first = entries;
+    p->g_rest = natomic;
+    p->g_nzero = (int64_t)crowded.size();
     p->gather_built = 1;
 }
 
@@ -1630,14 +1657,14 @@ template <typename R> void gather_spread(NufftPlan* p, const cplx<R>* c, cplx<R>
     a.beta = (R)p->beta;
     a.cfac = (R)(4.0 / ((double)p->ns * p->ns));
     a.rp = gather_row_pitch<R>(p->ns);
-    a.rest = 0;
+    a.natomic = (int)p->g_rest;
     KernelTimer t("nufft_spread", s);
-    launch_gather<R>(p->ns, a, p->g_first, s);
-    if (p->g_rest > 0) {
-        a.list = p->g_list + 2 * p->g_first;
-        a.rest = 1;
-        launch_gather<R>(p->ns, a, p->g_rest, s);
+    if (p->g_nzero > 0) {
+        hipLaunchKernelGGL((zero_cores_kernel<R>), dim3((unsigned)p->g_nzero), dim3(kThreads), 0, s, fine,
+                           (const int*)(p->g_list + 2 * p->g_first), a.nb[1], a.nb[2], a.nf[1], a.nf[2]);
+        check_launch("zero_cores_kernel");
     }
+    launch_gather<R>(p->ns, a, p->g_first, s);
 }
 
 template <typename R> void execute_type2(NufftPlan* p, cplx<R>* c, const cplx<R>* f, hipStream_t s)

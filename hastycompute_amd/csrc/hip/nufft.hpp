@@ -54,12 +54,13 @@ struct NufftPlan {
     int* g_pstart = nullptr;    // 27 cores + 1 (scan)
     int* g_pairs = nullptr;     // sorted point of each pair, grouped by core
     int* g_prel = nullptr;      // window corner relative to the core
-    int* g_list = nullptr;      // (core, part) pairs: every core's first part, then the rest
+    int* g_list = nullptr;      // (core, part) entries, then the crowded cores
     int64_t g_cap_M = 0;
     int64_t g_cap_pairs = 0;
     int64_t g_cap_list = 0;
-    int64_t g_first = 0;        // list entries of the first launch (one per core)
-    int64_t g_rest = 0;         // list entries of the second launch
+    int64_t g_first = 0;        // (core, part) entries of the spreading launch
+    int64_t g_rest = 0;         // of them the crowded cores' parts, first, adding with atomics
+    int64_t g_nzero = 0;        // crowded cores (listed after the entries), zeroed beforehand
 };
 
 // nmodes in FINUFFT order (fastest-varying first), as the reference passes

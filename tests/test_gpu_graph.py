@@ -195,7 +195,7 @@ def test_llr_prox_graph_replays_equal_eager(N):
 
 def test_nufft_pair_graph_replays_equal_eager(N):
     # a type-2 and a type-1 NUFFT (64^3 modes: register FFT passes, the output-driven
-    # spreader with a second launch for the crowded k-space centre) recorded as one HIP
+    # spreader with atomic parts for the crowded k-space centre) recorded as one HIP
     # graph after setpts; before each replay the outputs are overwritten, so a replay must
     # write every cell (no zero pass); type 2 equals the eager result bit for bit, type 1
     # (atomic adds of the crowded cores' further parts, in no fixed order) to rounding

@@ -196,8 +196,8 @@ def test_nufft_type1_gather_spreader(N, n, traj, tol, bound, fp64):
     # The output-driven spreader (variant 6): 16^3-cell cores, fine grids 48^3 (three
     # cores per dim: every neighbour of a core at the edge is a periodic image), 64^3 and
     # 80^3; the kooshball and the pileup (1500 points on one position) make cores with
-    # more (point, core) pairs than one part holds, whose further parts go through the
-    # second, atomic launch; ns from 4 to 16; complex64 (128-point parts) and complex128
+    # more (point, core) pairs than one part holds, which are zeroed first and spread in
+    # atomic parts; ns from 4 to 16; complex64 (128-point parts) and complex128
     # (32-point parts, the FP64 Toeplitz-kernel build's spreader). Against the direct sum,
     # with two transforms per plan (the core lists are built once), and against the tile
     # spreader (variant 3) to rounding.
@@ -242,7 +242,7 @@ def test_nufft_type1_gather_spreader(N, n, traj, tol, bound, fp64):
 def test_nufft_no_points(N, n):
     # An empty point set (M = 0, empty tensors with null data pointers): type 1 gives a
     # zero image, type 2 writes nothing; at n = 24 (fine grid 48^3) through the
-    # output-driven spreader, whose first launch alone stores the zeros, at n = 20 through
+    # output-driven spreader, whose plain-stored cores hold the zeros, at n = 20 through
     # the tile spreader and the zero pass. A plan reused after points are set again still
     # matches the direct sum.
     co = torch.empty((3, 0), dtype=torch.float32, device="cuda")
