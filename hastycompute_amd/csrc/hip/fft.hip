@@ -382,6 +382,13 @@ __global__ void __launch_bounds__(kR16Lines * (R > 16 ? R : 16)) axis_fft_r16_ke
     __shared__ float2 tws[L];
     const AxisFft<float>& a = args.a;
     const int tid = threadIdx.x;
+    // Transpose layout: element (line, g, k2) at (line R + k2) 17 + g when the lanes of a
+    // wave hold one line's g (contiguous input), at (k2 16 + g) 17 + line when they hold 16
+    // lines (strided input). With input and output both contiguous or both strided, a
+    // half-wave's 8-byte accesses then fall on distinct banks (the strided layout's writes
+    // share one bank pair); the old layout gave strided lines 8-way conflicts, a line step
+    // being R 17 float2 = 32 banks mod 64.
+    const bool lines_fast = a.ain.stride != 1;
     for (int k = tid; k < L; k += kR16Lines * (R > 16 ? R : 16)) tws[k] = args.tw[k];
     float2 v[R > 16 ? R : 16];
     if (tid < 256) {
@@ -424,8 +431,13 @@ __global__ void __launch_bounds__(kR16Lines * (R > 16 ? R : 16)) axis_fft_r16_ke
             if (S > 0) w.y = -w.y;
             v[k2] = cmul(v[k2], w);
         }
+        if (lines_fast) {
 #pragma unroll
-        for (int k2 = 0; k2 < R; ++k2) tr[(li * R + k2) * 17 + g] = v[k2];
+            for (int k2 = 0; k2 < R; ++k2) tr[(k2 * 16 + g) * 17 + li] = v[k2];
+        } else {
+#pragma unroll
+            for (int k2 = 0; k2 < R; ++k2) tr[(li * R + k2) * 17 + g] = v[k2];
+        }
     }
     __syncthreads();
     if (tid >= 16 * R) return;
@@ -437,8 +449,13 @@ __global__ void __launch_bounds__(kR16Lines * (R > 16 ? R : 16)) axis_fft_r16_ke
         lo = tid / R;
         k2 = tid - lo * R;
     }
+    if (lines_fast) {
 #pragma unroll
-    for (int m = 0; m < 16; ++m) v[m] = tr[(lo * R + k2) * 17 + m];
+        for (int m = 0; m < 16; ++m) v[m] = tr[(k2 * 16 + m) * 17 + lo];
+    } else {
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = tr[(lo * R + k2) * 17 + m];
+    }
     dft_r<16, S>(v);
     const int64_t l2 = (int64_t)blockIdx.x * kR16Lines + lo;
     if (l2 >= a.nlines) return;
