@@ -171,7 +171,7 @@ def nufft_main(args, torch, native):
                "seconds": cdt}
     tr_s = 1e3 / sum(r["ms"] for r in res.values())
     dom = max(res, key=lambda k: res[k]["ms"])
-    line = {"metric": "NUFFT type-2 + type-1 pairs/s @128^3, M=147200 (kooshball 1150x128), tol 1e-6",
+    line = {"metric": f"NUFFT type-2 + type-1 pairs/s @{n}^3, M={M} (kooshball {spokes}x{samp}), tol 1e-6",
             "value": tr_s, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": 1e3 / tr_s, "higher_is_better": True, "dtype": "c64 (fp32)",
             "data": "synthetic (seeded kooshball, random image and samples)",
